@@ -1,0 +1,158 @@
+/*
+ * mnad.h -- C-ABI of libmnad, the MI355X (gfx950) implementation of the
+ * MNAD-MRTM hot path: the coupled (u, du/dx, du/dz) MNAD time step with CPML,
+ * source injection, receiver sampling, RTM receiver back-propagation and the
+ * zero-lag imaging condition.
+ *
+ * Plain C types only.  Array conventions follow what a MATLAB MEX gateway
+ * hands over (mxGetPr): column-major, z (row) index fastest.
+ *   v        [nz x nx]   padded velocity, v(j,i) = v[j + i*nz]
+ *   w        [nt]        source wavelet
+ *   datau    [nt x nr]   gather, datau(t,r) = datau[t + r*nt]
+ *   U, V, W  [nz x nx]   final fields (V = dU/dx-like field, W = z analogue,
+ *                        exactly as fwd_PDE_exact_MNAD.m carries them)
+ *   illum    [nzi x nxi] interior illumination, nzi = nz - 2 npml
+ *   Im       [nzi * nxi] interior image, column-major (Born_adj_MNAD.m:138)
+ * Grid indices (zs, xs, zr, xr) are 0-based padded indices; the MATLAB
+ * reference passes 1-based ones (main_Acoustic_Data.m:120-126), so a gateway
+ * subtracts 1.
+ *
+ * Every function returns MNAD_OK (0) or a negative MNAD_ERR_* code; the text
+ * of the last error on the calling thread is returned by mnad_last_error().
+ * All entry points are re-entrant: calls on one context are serialised.
+ */
+#ifndef MNAD_H
+#define MNAD_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define MNAD_API __attribute__((visibility("default")))
+#else
+#define MNAD_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MNAD_OK 0
+#define MNAD_ERR_ARG (-1)       /* invalid argument (shape, index, CFL, NaN) */
+#define MNAD_ERR_HIP (-2)       /* HIP runtime / device error */
+#define MNAD_ERR_NOMEM (-3)     /* device memory exhausted */
+#define MNAD_ERR_STATE (-4)     /* missing source wavefield, unknown handle */
+#define MNAD_ERR_CANCELLED (-5) /* request cancelled */
+#define MNAD_ERR_BUDGET (-6)    /* request larger than the HBM budget */
+#define MNAD_ERR_NOGPU (-7)     /* no HIP device */
+#define MNAD_ERR_TIMEOUT (-8)   /* wait timed out (request still running) */
+
+/* Storage format of the saved source-side d2u snapshots. */
+#define MNAD_SNAP_F32 0
+#define MNAD_SNAP_F16 1   /* FP16 with a per-shot power-of-two scale */
+#define MNAD_SNAP_BF16 2
+
+/* Flags */
+#define MNAD_FLAG_NO_GRAPH 1   /* plain launches, one step per launch */
+
+typedef struct {
+    int nz, nx;       /* padded grid (rows = z, columns = x) */
+    int npml;         /* CPML thickness in cells (d_pml) */
+    double dx, dz, dt;
+} mnad_grid;
+
+typedef struct mnad_ctx mnad_ctx;
+
+/* Library identification and last error of the calling thread. */
+MNAD_API const char *mnad_version(void);
+MNAD_API const char *mnad_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+MNAD_API int mnad_device_count(void);
+
+/* Create / destroy a context bound to one device.  A context owns the
+ * device stream, the memory pool and the source-wavefield handles. */
+MNAD_API int mnad_ctx_create(int device, mnad_ctx **out);
+MNAD_API void mnad_ctx_destroy(mnad_ctx *ctx);
+
+/* Forward shot.  Replaces
+ *   data = fwd_PDE_exact_MNAD(dx,xs,xr,dz,zs,zr,v,dt,nt,w,d_pml,ss,Auu,...,Aww,indx)
+ * (SRC/MNAD/fwd_PDE_exact_MNAD.m:1).  The nine sparse operators are not
+ * passed: the kernel evaluates their entries from v, dx, dz, dt.
+ * indx = 1 keeps the source-side d2u snapshots on the device under the key
+ * ss (replacing ./Wavefields/UU_<ss>.mat, fwd_PDE_exact_MNAD.m:161) and
+ * returns T*sum(d2u^2) in `illum` when non-NULL (./Illuminations/illum_<ss>,
+ * fwd_PDE_exact_MNAD.m:162-163).  U, V, W, illum may be NULL. */
+MNAD_API int mnad_fwd(mnad_ctx *ctx, const mnad_grid *g, const double *v, int nt,
+             const double *w, int zs, int xs, int nr, const int *zr,
+             const int *xr, int ss, int indx, int snap_fmt, int flags,
+             double *datau, double *U, double *V, double *W, double *illum);
+
+/* RTM adjoint of one shot.  Replaces
+ *   Im = Born_adj_MNAD(dx,dz,nt,dt,v,xr,zr,data,d_pml,ss,Auu,...,Aww)
+ * (SRC/migration/Born_adj_MNAD.m:1) with U_r read as U and `data` a matrix
+ * [nt x nr].  Consumes the snapshots kept by mnad_fwd(..., ss, indx=1) on the
+ * same v (MNAD_ERR_STATE if there are none) and releases them. */
+MNAD_API int mnad_born_adj(mnad_ctx *ctx, const mnad_grid *g, const double *v, int nt,
+                  int nr, const int *zr, const int *xr, const double *data,
+                  int ss, int flags, double *Im);
+
+/* Drop the snapshots kept for shot ss (no-op if none). */
+MNAD_API int mnad_release(mnad_ctx *ctx, int ss);
+
+/* Snapshot schedule of fwd_PDE_exact_MNAD.m:46-47 for a velocity range. */
+MNAD_API int mnad_snap_schedule(int nt, double vmin, double vmax, int *nf, int *T,
+                       int *K, int *ncol);
+
+/* ---- RTM workflow on the device (main_Acoustic_RTM.m:172-262) ----------
+ * A stack accumulates, over shots, the per-shot images Im_s and
+ * illuminations illum_s with compensated FP32 sums; mnad_stack_image returns
+ * img = mean_s(Im_s) ./ mean_s(illum_s) (main_Acoustic_RTM.m:243,262). */
+MNAD_API int mnad_stack_create(mnad_ctx *ctx, const mnad_grid *g, int *stack_id);
+MNAD_API int mnad_stack_destroy(mnad_ctx *ctx, int stack_id);
+MNAD_API int mnad_stack_image(mnad_ctx *ctx, int stack_id, double *img, double *illum,
+                     int *nshots);
+
+/* One receiver spread per shot: zs[k], xs[k] and rec_off[k]..rec_off[k+1]-1
+ * into zr/xr (rec_off has nshots+1 entries). */
+typedef struct {
+    int nshots;
+    const int *zs, *xs;
+    const int *rec_off;
+    const int *zr, *xr;
+} mnad_shots;
+
+/* RTM of several shots, batched: per shot, a forward on v_true (observed
+ * gather), a forward on v_smooth with snapshots (synthetic gather), the
+ * scattered data d_obs - d_syn (main_Acoustic_RTM.m:225) and the adjoint with
+ * imaging (Born_adj_MNAD.m); images and illuminations go into the stack.
+ * Up to `batch` shots share one launch sequence (0 = choose from the memory
+ * budget).  dobs / dsyn (may be NULL) receive the gathers of every shot,
+ * concatenated per shot, each [nt x nr_k] column-major. */
+MNAD_API int mnad_rtm_shots(mnad_ctx *ctx, int stack_id, const mnad_grid *g,
+                   const double *v_true, const double *v_smooth, int nt,
+                   const double *w, const mnad_shots *shots, int snap_fmt,
+                   int batch, int flags, double *dobs, double *dsyn);
+
+/* Forward modelling of several shots on one model, batched in one launch
+ * sequence (the parfor of main_Acoustic_Data.m:169-171).  datau receives the
+ * gathers concatenated per shot ([nt x nr_k] column-major each). */
+MNAD_API int mnad_fwd_shots(mnad_ctx *ctx, const mnad_grid *g, const double *v, int nt,
+                   const double *w, const mnad_shots *shots, int batch,
+                   int flags, double *datau);
+
+/* Step-kernel accounting since the last reset: GPU time of the step-kernel
+ * launch sequences measured with HIP events on the context's stream (ms,
+ * inter-launch gaps included), the number of step launches, and the
+ * algorithmic HBM bytes those launches move (40 B per grid point, 72 B per
+ * CPML strip point and direction, receivers, snapshots, image).  reset != 0
+ * clears the counters after reading. */
+MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches,
+                         double *bytes);
+
+/* Device memory in use / cached by the context's pool, in bytes. */
+MNAD_API int mnad_mem_info(mnad_ctx *ctx, uint64_t *in_use, uint64_t *cached);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

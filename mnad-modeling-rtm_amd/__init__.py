@@ -1,0 +1,367 @@
+"""MI355X (gfx950) implementation of the MNAD-MRTM hot path.
+
+The compute runs in libmnad.so (hand-written HIP kernels behind the C-ABI of
+include/mnad.h).  This module is the host-side mirror of the reference's two
+MATLAB entry points, with the same argument order, meaning and 1-based grid
+indices:
+
+    data = fwd_PDE_exact_MNAD(dx, xs, xr, dz, zs, zr, v, dt, nt, w, d_pml, ss,
+                              Auu, ..., Aww, indx=0)     # fwd_PDE_exact_MNAD.m:1
+    Im   = Born_adj_MNAD(dx, dz, nt, dt, v, xr, zr, data, d_pml, ss,
+                         Auu, ..., Aww)                   # Born_adj_MNAD.m:1
+
+`v` is the padded [nz x nx] velocity (v[j-1, i-1] is MATLAB's v(j,i)); the
+nine operator matrices are accepted and ignored because the kernels evaluate
+their entries from (v, dx, dz, dt).  indx = 1 keeps the source-side d2u
+snapshots on the GPU under the key ss (instead of ./Wavefields/UU_<ss>.mat);
+Born_adj_MNAD consumes them.  There is no CPU fallback: without a GPU or
+without the built library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libmnad.so")
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE = -1, -2, -3, -4
+ERR_CANCELLED, ERR_BUDGET, ERR_NOGPU, ERR_TIMEOUT = -5, -6, -7, -8
+SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
+FLAG_NO_GRAPH = 1
+SNAP_FORMATS = {"f32": SNAP_F32, "f16": SNAP_F16, "bf16": SNAP_BF16}
+
+# Every symbol include/mnad.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "mnad_version", "mnad_last_error", "mnad_device_count", "mnad_ctx_create",
+    "mnad_ctx_destroy", "mnad_fwd", "mnad_born_adj", "mnad_release",
+    "mnad_snap_schedule", "mnad_stack_create", "mnad_stack_destroy",
+    "mnad_stack_image", "mnad_rtm_shots", "mnad_fwd_shots", "mnad_mem_info",
+    "mnad_timing",
+)
+
+
+class MnadError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nz", C.c_int), ("nx", C.c_int), ("npml", C.c_int),
+                ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double)]
+
+
+class Shots(C.Structure):
+    _fields_ = [("nshots", C.c_int), ("zs", C.POINTER(C.c_int)), ("xs", C.POINTER(C.c_int)),
+                ("rec_off", C.POINTER(C.c_int)), ("zr", C.POINTER(C.c_int)),
+                ("xr", C.POINTER(C.c_int))]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def lib():
+    """Load libmnad.so; raises if it has not been built."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise MnadError(ERR_STATE, f"{LIB_PATH} is missing: build it with "
+                                "`make -C mnad-modeling-rtm_amd` (hipcc, gfx950)")
+            L = C.CDLL(LIB_PATH)
+            dp, ip, gp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(Grid)
+            vp = C.c_void_p
+            L.mnad_version.restype = C.c_char_p
+            L.mnad_last_error.restype = C.c_char_p
+            L.mnad_device_count.restype = C.c_int
+            L.mnad_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+            L.mnad_ctx_destroy.argtypes = [vp]
+            L.mnad_ctx_destroy.restype = None
+            L.mnad_fwd.argtypes = [vp, gp, dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, ip, ip,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp]
+            L.mnad_born_adj.argtypes = [vp, gp, dp, C.c_int, C.c_int, ip, ip, dp, C.c_int,
+                                        C.c_int, dp]
+            L.mnad_release.argtypes = [vp, C.c_int]
+            L.mnad_snap_schedule.argtypes = [C.c_int, C.c_double, C.c_double, ip, ip, ip, ip]
+            L.mnad_stack_create.argtypes = [vp, gp, ip]
+            L.mnad_stack_destroy.argtypes = [vp, C.c_int]
+            L.mnad_stack_image.argtypes = [vp, C.c_int, dp, dp, ip]
+            L.mnad_rtm_shots.argtypes = [vp, C.c_int, gp, dp, dp, C.c_int, dp,
+                                         C.POINTER(Shots), C.c_int, C.c_int, C.c_int, dp, dp]
+            L.mnad_fwd_shots.argtypes = [vp, gp, dp, C.c_int, dp, C.POINTER(Shots), C.c_int,
+                                         C.c_int, dp]
+            L.mnad_mem_info.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+            L.mnad_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+            for name in EXPORTS:
+                if name not in ("mnad_version", "mnad_last_error", "mnad_ctx_destroy"):
+                    getattr(L, name).restype = C.c_int
+            _lib = L
+    return _lib
+
+
+def version() -> str:
+    return lib().mnad_version().decode()
+
+
+def _check(rc):
+    if rc != OK:
+        raise MnadError(rc, lib().mnad_last_error().decode())
+
+
+def _f64(a, order="F"):
+    return np.require(np.asarray(a, dtype=np.float64), requirements=[order[0], "A"])
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a).ravel(), dtype=np.int32)
+
+
+def _as_index(x, name):
+    """MATLAB index value (double) -> int, rejecting non-integers."""
+    a = np.asarray(x, dtype=np.float64).ravel()
+    r = np.rint(a)
+    if a.size and np.abs(a - r).max() > 1e-9:
+        raise MnadError(ERR_ARG, f"{name} must hold integer grid indices")
+    return r.astype(np.int64)
+
+
+def snap_schedule(nt, vmin, vmax):
+    """(nf, T, K, ncol) of fwd_PDE_exact_MNAD.m:46-47,53."""
+    a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _check(lib().mnad_snap_schedule(int(nt), float(vmin), float(vmax), C.byref(a),
+                                    C.byref(b), C.byref(c), C.byref(d)))
+    return a.value, b.value, c.value, d.value
+
+
+def _shots_struct(zs, xs, recs):
+    """recs: list of (zr, xr) per shot (0-based). Keeps arrays alive."""
+    zs = _i32(zs)
+    xs = _i32(xs)
+    offs = [0]
+    zr_all, xr_all = [], []
+    for zr, xr in recs:
+        zr, xr = _i32(zr), _i32(xr)
+        if zr.size != xr.size:
+            raise MnadError(ERR_ARG, "zr and xr differ in length")
+        zr_all.append(zr)
+        xr_all.append(xr)
+        offs.append(offs[-1] + zr.size)
+    off = np.asarray(offs, dtype=np.int32)
+    zr = np.concatenate(zr_all) if zr_all else np.zeros(0, np.int32)
+    xr = np.concatenate(xr_all) if xr_all else np.zeros(0, np.int32)
+    zr = np.ascontiguousarray(zr, dtype=np.int32)
+    xr = np.ascontiguousarray(xr, dtype=np.int32)
+    s = Shots(int(zs.size), _ip(zs), _ip(xs), _ip(off), _ip(zr), _ip(xr))
+    s._keep = (zs, xs, off, zr, xr)
+    return s, off
+
+
+class Context:
+    """One device context (stream, memory pool, kept source wavefields)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().mnad_ctx_create(int(device), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().mnad_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---------------------------------------------------------------- shots
+    def fwd(self, grid: Grid, v, w, zs, xs, zr, xr, ss=1, indx=0, snap_fmt=SNAP_F32,
+            flags=0, want_state=True):
+        """Forward shot with 0-based indices; v [nz x nx].  Returns a dict with
+        gather [nt x nr] and, if want_state, final U, V, W [nz x nx]
+        (+ illum [nzi x nxi] when indx == 1)."""
+        v = _f64(v)
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.float64).ravel())
+        nt = w.size
+        zr, xr = _i32(zr), _i32(xr)
+        nr = zr.size
+        if xr.size != nr:
+            raise MnadError(ERR_ARG, "zr and xr differ in length")
+        gather = np.zeros((nt, nr), order="F")
+        U = V = W = None
+        if want_state:
+            U, V, W = (np.zeros(v.shape, order="F") for _ in range(3))
+        illum = None
+        if indx == 1:
+            illum = np.zeros((grid.nz - 2 * grid.npml, grid.nx - 2 * grid.npml), order="F")
+        _check(lib().mnad_fwd(self._h, C.byref(grid), _dp(v), nt, _dp(w), int(zs), int(xs), nr,
+                              _ip(zr), _ip(xr), int(ss), int(indx), int(snap_fmt), int(flags),
+                              _dp(gather), _dp(U), _dp(V), _dp(W), _dp(illum)))
+        return dict(gather=gather, U=U, V=V, W=W, illum=illum)
+
+    def born_adj(self, grid: Grid, v, data, zr, xr, ss=1, flags=0):
+        """RTM adjoint of shot ss; returns Im [nzi*nxi] (column-major interior)."""
+        v = _f64(v)
+        data = _f64(data)
+        nt = data.shape[0]
+        zr, xr = _i32(zr), _i32(xr)
+        Im = np.zeros((grid.nz - 2 * grid.npml) * (grid.nx - 2 * grid.npml))
+        _check(lib().mnad_born_adj(self._h, C.byref(grid), _dp(v), nt, zr.size, _ip(zr),
+                                   _ip(xr), _dp(data), int(ss), int(flags), _dp(Im)))
+        return Im
+
+    def release(self, ss):
+        _check(lib().mnad_release(self._h, int(ss)))
+
+    def stack_create(self, grid: Grid) -> int:
+        sid = C.c_int()
+        _check(lib().mnad_stack_create(self._h, C.byref(grid), C.byref(sid)))
+        return sid.value
+
+    def stack_destroy(self, sid: int):
+        _check(lib().mnad_stack_destroy(self._h, int(sid)))
+
+    def stack_image(self, sid: int, grid: Grid):
+        """(img, illum, nshots): img = mean(Im_s)./mean(illum_s), [nzi x nxi]."""
+        shp = (grid.nz - 2 * grid.npml, grid.nx - 2 * grid.npml)
+        img = np.zeros(shp, order="F")
+        ill = np.zeros(shp, order="F")
+        n = C.c_int()
+        _check(lib().mnad_stack_image(self._h, int(sid), _dp(img), _dp(ill), C.byref(n)))
+        return img, ill, n.value
+
+    def rtm_shots(self, sid, grid: Grid, v_true, v_smooth, w, zs, xs, recs,
+                  snap_fmt=SNAP_F16, batch=0, flags=0, want_gathers=False):
+        """Batched RTM of shots (0-based indices), stacked into stack sid.
+        recs: list of (zr, xr) per shot.  Returns per-shot (dobs, dsyn)
+        gathers if want_gathers."""
+        vt, vs = _f64(v_true), _f64(v_smooth)
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.float64).ravel())
+        nt = w.size
+        sh, off = _shots_struct(zs, xs, recs)
+        dobs = dsyn = None
+        if want_gathers:
+            dobs = np.zeros(nt * int(off[-1]))
+            dsyn = np.zeros(nt * int(off[-1]))
+        _check(lib().mnad_rtm_shots(self._h, int(sid), C.byref(grid), _dp(vt), _dp(vs), nt,
+                                    _dp(w), C.byref(sh), int(snap_fmt), int(batch), int(flags),
+                                    _dp(dobs), _dp(dsyn)))
+        if not want_gathers:
+            return None
+        split = lambda a: [a[off[k] * nt:off[k + 1] * nt].reshape(nt, -1, order="F")
+                           for k in range(len(off) - 1)]
+        return split(dobs), split(dsyn)
+
+    def fwd_shots(self, grid: Grid, v, w, zs, xs, recs, batch=0, flags=0):
+        """Batched forward modelling; returns the list of gathers [nt x nr_k]."""
+        v = _f64(v)
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.float64).ravel())
+        nt = w.size
+        sh, off = _shots_struct(zs, xs, recs)
+        out = np.zeros(nt * int(off[-1]))
+        _check(lib().mnad_fwd_shots(self._h, C.byref(grid), _dp(v), nt, _dp(w), C.byref(sh),
+                                    int(batch), int(flags), _dp(out)))
+        return [out[off[k] * nt:off[k + 1] * nt].reshape(nt, -1, order="F")
+                for k in range(len(off) - 1)]
+
+    def timing(self, reset=False):
+        """(ms, launches, bytes) of the step-kernel sequences (HIP events)."""
+        ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+        _check(lib().mnad_timing(self._h, int(bool(reset)), C.byref(ms), C.byref(n), C.byref(b)))
+        return ms.value, n.value, b.value
+
+    def mem_info(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        _check(lib().mnad_mem_info(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+# ------------------------------------------------ MATLAB-signature mirror
+_default_ctx = None
+_default_lock = threading.Lock()
+
+
+def default_context() -> Context:
+    global _default_ctx
+    with _default_lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+        return _default_ctx
+
+
+def fwd_PDE_exact_MNAD(dx, xs, xr, dz, zs, zr, v, dt, nt, w, d_pml, ss, *ops, indx=None,
+                       snap_fmt=SNAP_F32):
+    """data = fwd_PDE_exact_MNAD(dx,xs,xr,dz,zs,zr,v,dt,nt,w,d_pml,ss,Auu,...,Aww[,indx])
+
+    Mirrors SRC/MNAD/fwd_PDE_exact_MNAD.m:1 (1-based padded indices).  The
+    operator arguments are ignored (evaluated on the GPU from v); a 10th
+    positional operator argument is read as indx, as in MATLAB.  Returns the
+    1-element list [datau] (the reference returns the cell {datau}, :166)."""
+    ops = list(ops)
+    if len(ops) == 10 and indx is None:
+        indx = ops.pop()
+    if indx is None or (np.size(indx) == 0):
+        indx = 0
+    v = np.asarray(v, dtype=np.float64)
+    nz, nx = v.shape
+    w = np.asarray(w, dtype=np.float64).ravel()
+    if w.size != int(nt):
+        raise MnadError(ERR_ARG, f"length(w) = {w.size} differs from nt = {int(nt)}")
+    g = Grid(nz, nx, int(d_pml), float(dx), float(dz), float(dt))
+    xs_, zs_ = int(_as_index(xs, "xs")[0]), int(_as_index(zs, "zs")[0])
+    xr_, zr_ = _as_index(xr, "xr"), _as_index(zr, "zr")
+    r = default_context().fwd(g, v, w, zs_ - 1, xs_ - 1, zr_ - 1, xr_ - 1, ss=int(ss),
+                              indx=int(indx), snap_fmt=snap_fmt, want_state=False)
+    if int(indx) == 1:
+        _last_illum[int(ss)] = r["illum"]
+    return [r["gather"]]
+
+
+_last_illum: dict = {}
+
+
+def illumination(ss):
+    """T*sum(d2u^2) of the last fwd_PDE_exact_MNAD(..., ss, indx=1) call
+    (what the reference writes to ./Illuminations/illum_<ss>.mat, :162-163)."""
+    return _last_illum[int(ss)]
+
+
+def Born_adj_MNAD(dx, dz, nt, dt, v, xr, zr, data, d_pml, ss, *ops):
+    """Im = Born_adj_MNAD(dx,dz,nt,dt,v,xr,zr,data,d_pml,ss,Auu,...,Aww)
+
+    Mirrors SRC/migration/Born_adj_MNAD.m:1, reading U_r as U (the variable
+    is undefined in the reference, :105-106) and accepting `data` either as
+    the [nt x nr] matrix the driver passes (main_Acoustic_RTM.m:225,252) or
+    as the cell/list {datau, ...} the docstring describes (:46-50).
+    Returns Im as a column vector over the interior (column-major)."""
+    if isinstance(data, (list, tuple)):
+        data = data[0]
+    v = np.asarray(v, dtype=np.float64)
+    nz, nx = v.shape
+    data = np.asarray(data, dtype=np.float64)
+    if data.shape[0] != int(nt):
+        raise MnadError(ERR_ARG, f"data has {data.shape[0]} rows, nt = {int(nt)}")
+    g = Grid(nz, nx, int(d_pml), float(dx), float(dz), float(dt))
+    xr_, zr_ = _as_index(xr, "xr"), _as_index(zr, "zr")
+    return default_context().born_adj(g, v, data, zr_ - 1, xr_ - 1, ss=int(ss))

@@ -1,0 +1,435 @@
+// mnad_capi.hip -- the extern "C" entry points of include/mnad.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "mnad.h"
+#include "mnad_internal.h"
+
+using namespace mnad;
+
+namespace mnad {
+extern "C" hipError_t mnad_launch_make_radj(const float* dobs, const float* dsyn,
+                                            const float* arec, int nt, int nrec, float* radj,
+                                            hipStream_t st);
+extern "C" hipError_t mnad_launch_stack(const float* img, const float* imgc, const float* illum,
+                                        float T, long n, float* isum, float* isumc, float* lsum,
+                                        float* lsumc, hipStream_t st);
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? MNAD_ERR_NOMEM : MNAD_ERR_HIP,     \
+                        "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                     \
+    } while (0)
+
+#define CHK(x)              \
+    do {                    \
+        int rc_ = (x);      \
+        if (rc_ != MNAD_OK) \
+            return rc_;     \
+    } while (0)
+
+#define ENTER(ctx)                                                   \
+    if (!(ctx)) return fail(MNAD_ERR_ARG, "context is NULL");        \
+    std::lock_guard<std::recursive_mutex> lock_((ctx)->mu);          \
+    HIPCHK(hipSetDevice((ctx)->device))
+
+extern "C" {
+
+const char* mnad_version(void) { return "libmnad 0.1.0 (gfx950)"; }
+
+const char* mnad_last_error(void) { return last_error(); }
+
+int mnad_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int mnad_ctx_create(int device, mnad_ctx** out)
+{
+    if (!out) return fail(MNAD_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int n = mnad_device_count();
+    if (n <= 0) return fail(MNAD_ERR_NOGPU, "no HIP device visible: libmnad needs a GPU");
+    if (device < 0 || device >= n)
+        return fail(MNAD_ERR_ARG, "device %d out of range (%d visible)", device, n);
+    HIPCHK(hipSetDevice(device));
+    mnad_ctx* c = new mnad_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MNAD_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    if (const char* s = getenv("MNAD_GRAPH_CHUNK")) c->graph_chunk = std::max(1, atoi(s));
+    *out = c;
+    return MNAD_OK;
+}
+
+void mnad_ctx_destroy(mnad_ctx* ctx)
+{
+    if (!ctx) return;
+    {
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->kept.clear();
+        ctx->stacks.clear();
+        ctx->pool.trim();
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int mnad_snap_schedule(int nt, double vmin, double vmax, int* nf, int* T, int* K, int* ncol)
+{
+    if (!(vmin > 0) || !(vmax >= vmin)) return fail(MNAD_ERR_ARG, "bad velocity range");
+    return schedule(nt, vmin, vmax, nf, T, K, ncol);
+}
+
+int mnad_timing(mnad_ctx* ctx, int reset, double* ms, int64_t* launches, double* bytes)
+{
+    ENTER(ctx);
+    if (ms) *ms = ctx->t_ms;
+    if (launches) *launches = ctx->t_launches;
+    if (bytes) *bytes = ctx->t_bytes;
+    if (reset) {
+        ctx->t_ms = 0.0;
+        ctx->t_launches = 0;
+        ctx->t_bytes = 0.0;
+    }
+    return MNAD_OK;
+}
+
+int mnad_mem_info(mnad_ctx* ctx, uint64_t* in_use, uint64_t* cached)
+{
+    if (!ctx) return fail(MNAD_ERR_ARG, "context is NULL");
+    std::lock_guard<std::mutex> lk(ctx->pool.mu);
+    if (in_use) *in_use = ctx->pool.in_use;
+    if (cached) *cached = ctx->pool.cached;
+    return MNAD_OK;
+}
+
+// fwd_PDE_exact_MNAD.m:1
+int mnad_fwd(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, const double* w,
+             int zs, int xs, int nr, const int* zr, const int* xr, int ss, int indx,
+             int snap_fmt, int flags, double* datau, double* U, double* V, double* W,
+             double* illum)
+{
+    ENTER(ctx);
+    if (indx != 0 && indx != 1) return fail(MNAD_ERR_ARG, "indx must be 0 or 1");
+    if (snap_fmt < MNAD_SNAP_F32 || snap_fmt > MNAD_SNAP_BF16)
+        return fail(MNAD_ERR_ARG, "unknown snapshot format %d", snap_fmt);
+    if (nr < 0 || (nr > 0 && (!zr || !xr))) return fail(MNAD_ERR_ARG, "bad receiver arrays");
+    Model m;
+    CHK(make_model(ctx, g, v, m));
+    ShotSpec sp;
+    sp.mode = MODE_FWD;
+    sp.nt = nt;
+    sp.w = w;
+    sp.zs = zs;
+    sp.xs = xs;
+    sp.nr = nr;
+    sp.zr = zr;
+    sp.xr = xr;
+    if (indx == 1) {
+        CHK(schedule(nt, m.vmin, m.vmax, nullptr, &sp.T, &sp.K, &sp.ncol));
+        sp.snap_fmt = snap_fmt;
+    }
+    Shot s;
+    CHK(make_shot(ctx, m, sp, s));
+    std::vector<Shot*> b{&s};
+    CHK(run_batch(ctx, b, flags));
+    CHK(download_gather(ctx, s, nt, nr, datau));
+    CHK(download_field(ctx, m, s, 0, 1.0, U));
+    CHK(download_field(ctx, m, s, 1, 1.0 / g->dx, V));
+    CHK(download_field(ctx, m, s, 2, 1.0 / g->dz, W));
+    if (indx == 1) {
+        if (illum) CHK(download_interior(ctx, *g, s.illum.as<float>(), nullptr, (double)sp.T, illum));
+        Kept k;
+        k.snaps = std::move(s.snaps);
+        k.fmt = snap_fmt;
+        k.sscale = s.d.sscale;
+        k.nz = g->nz;
+        k.nx = g->nx;
+        k.npml = g->npml;
+        k.nt = nt;
+        k.T = sp.T;
+        k.K = sp.K;
+        k.ncol = sp.ncol;
+        ctx->kept[ss] = std::move(k);
+    }
+    return MNAD_OK;
+}
+
+// Born_adj_MNAD.m:1 (U_r := U)
+int mnad_born_adj(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, int nr,
+                  const int* zr, const int* xr, const double* data, int ss, int flags, double* Im)
+{
+    ENTER(ctx);
+    if (nr < 0 || (nr > 0 && (!zr || !xr || !data))) return fail(MNAD_ERR_ARG, "bad receiver arrays");
+    auto it = ctx->kept.find(ss);
+    if (it == ctx->kept.end())
+        return fail(MNAD_ERR_STATE,
+                    "no source wavefield for shot %d: run mnad_fwd(..., ss=%d, indx=1) first "
+                    "(Born_adj_MNAD.m:33 loads ./Wavefields/UU_<ss>)",
+                    ss, ss);
+    Model m;
+    CHK(make_model(ctx, g, v, m));
+    ShotSpec sp;
+    sp.mode = MODE_ADJ;
+    sp.nt = nt;
+    sp.nr = nr;
+    sp.zr = zr;
+    sp.xr = xr;
+    CHK(schedule(nt, m.vmin, m.vmax, nullptr, &sp.T, &sp.K, &sp.ncol));
+    Kept& k = it->second;
+    if (k.nz != g->nz || k.nx != g->nx || k.npml != g->npml || k.nt != nt || k.T != sp.T ||
+        k.K != sp.K || k.ncol != sp.ncol)
+        return fail(MNAD_ERR_STATE,
+                    "shot %d: kept snapshots (grid %dx%d, npml %d, nt %d, T %d, K %d) do not "
+                    "match this call (grid %dx%d, npml %d, nt %d, T %d, K %d)",
+                    ss, k.nz, k.nx, k.npml, k.nt, k.T, k.K, g->nz, g->nx, g->npml, nt, sp.T, sp.K);
+    Shot s;
+    CHK(make_shot(ctx, m, sp, s));
+    attach_snaps(s, std::move(k.snaps), k.fmt, k.sscale);
+    ctx->kept.erase(it);
+    CHK(fill_radj_host(ctx, s, nt, nr, data));
+    std::vector<Shot*> b{&s};
+    CHK(run_batch(ctx, b, flags));
+    CHK(download_interior(ctx, *g, s.img.as<float>(), s.imgc.as<float>(), (double)sp.T, Im));
+    return MNAD_OK;
+}
+
+int mnad_release(mnad_ctx* ctx, int ss)
+{
+    ENTER(ctx);
+    ctx->kept.erase(ss);
+    return MNAD_OK;
+}
+
+int mnad_stack_create(mnad_ctx* ctx, const mnad_grid* g, int* stack_id)
+{
+    ENTER(ctx);
+    if (!g || !stack_id) return fail(MNAD_ERR_ARG, "NULL argument");
+    Stack st;
+    st.g = *g;
+    const size_t ni = (size_t)(g->nz - 2 * g->npml) * (g->nx - 2 * g->npml);
+    if (g->nz <= 2 * g->npml || g->nx <= 2 * g->npml) return fail(MNAD_ERR_ARG, "empty interior");
+    CHK(dalloc(ctx, st.isum, ni * sizeof(float), true));
+    CHK(dalloc(ctx, st.isumc, ni * sizeof(float), true));
+    CHK(dalloc(ctx, st.lsum, ni * sizeof(float), true));
+    CHK(dalloc(ctx, st.lsumc, ni * sizeof(float), true));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int id = ctx->next_stack++;
+    ctx->stacks.emplace(id, std::move(st));
+    *stack_id = id;
+    return MNAD_OK;
+}
+
+int mnad_stack_destroy(mnad_ctx* ctx, int stack_id)
+{
+    ENTER(ctx);
+    ctx->stacks.erase(stack_id);
+    return MNAD_OK;
+}
+
+// main_Acoustic_RTM.m:243,256-262: img = mean_s(Im_s) ./ mean_s(illum_s).
+int mnad_stack_image(mnad_ctx* ctx, int stack_id, double* img, double* illum, int* nshots)
+{
+    ENTER(ctx);
+    auto it = ctx->stacks.find(stack_id);
+    if (it == ctx->stacks.end()) return fail(MNAD_ERR_STATE, "unknown stack %d", stack_id);
+    Stack& st = it->second;
+    const mnad_grid& g = st.g;
+    const size_t ni = (size_t)(g.nz - 2 * g.npml) * (g.nx - 2 * g.npml);
+    std::vector<double> isum(ni), lsum(ni);
+    CHK(download_interior(ctx, g, st.isum.as<float>(), st.isumc.as<float>(), 1.0, isum.data()));
+    CHK(download_interior(ctx, g, st.lsum.as<float>(), st.lsumc.as<float>(), 1.0, lsum.data()));
+    const double ns = st.nshots > 0 ? st.nshots : 1;
+    for (size_t q = 0; q < ni; ++q) {
+        if (img) img[q] = (isum[q] / ns) / (lsum[q] / ns);
+        if (illum) illum[q] = lsum[q] / ns;
+    }
+    if (nshots) *nshots = st.nshots;
+    return MNAD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- batched RTM
+namespace {
+
+size_t shot_bytes(const Model& m, int nt, int nu)
+{
+    const mnad_grid& g = m.g;
+    const size_t np = g.npml;
+    return 6 * m.elems * 4 + 9 * (g.nz * 2 * np + 2 * np * g.nx) * 4 + (size_t)nt * nu * 4 + 4096;
+}
+
+int pick_batch(mnad_ctx* ctx, size_t per_shot, int nshots, int batch)
+{
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+        (void)hipGetLastError();
+        freeb = 0;
+    }
+    size_t budget = ctx->budget ? ctx->budget : (size_t)(0.85 * (double)(freeb + ctx->pool.cached));
+    int b = (int)std::max<size_t>(1, budget / std::max<size_t>(per_shot, 1));
+    if (batch > 0) b = std::min(b, batch);
+    return std::max(1, std::min(b, nshots));
+}
+
+int check_shots(const mnad_shots* sh)
+{
+    if (!sh || sh->nshots < 1 || !sh->zs || !sh->xs || !sh->rec_off)
+        return fail(MNAD_ERR_ARG, "bad shot list");
+    if (sh->rec_off[0] != 0) return fail(MNAD_ERR_ARG, "rec_off[0] must be 0");
+    for (int k = 0; k < sh->nshots; ++k)
+        if (sh->rec_off[k + 1] < sh->rec_off[k]) return fail(MNAD_ERR_ARG, "rec_off not monotone");
+    if (sh->rec_off[sh->nshots] > 0 && (!sh->zr || !sh->xr))
+        return fail(MNAD_ERR_ARG, "receiver arrays are NULL");
+    return MNAD_OK;
+}
+
+}  // namespace
+
+extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
+                              const double* v_true, const double* v_smooth, int nt,
+                              const double* w, const mnad_shots* sh, int snap_fmt, int batch,
+                              int flags, double* dobs, double* dsyn)
+{
+    ENTER(ctx);
+    CHK(check_shots(sh));
+    if (snap_fmt < MNAD_SNAP_F32 || snap_fmt > MNAD_SNAP_BF16)
+        return fail(MNAD_ERR_ARG, "unknown snapshot format %d", snap_fmt);
+    auto sit = ctx->stacks.find(stack_id);
+    if (sit == ctx->stacks.end()) return fail(MNAD_ERR_STATE, "unknown stack %d", stack_id);
+    Stack& st = sit->second;
+    if (st.g.nz != g->nz || st.g.nx != g->nx || st.g.npml != g->npml)
+        return fail(MNAD_ERR_ARG, "stack grid does not match");
+    Model mt, ms;
+    CHK(make_model(ctx, g, v_true, mt));
+    CHK(make_model(ctx, g, v_smooth, ms));
+    int T, K, ncol;
+    CHK(schedule(nt, ms.vmin, ms.vmax, nullptr, &T, &K, &ncol));   // from vs, as :224
+    const size_t ni = (size_t)(g->nz - 2 * g->npml) * (g->nx - 2 * g->npml);
+    int maxnr = 0;
+    for (int k = 0; k < sh->nshots; ++k) maxnr = std::max(maxnr, sh->rec_off[k + 1] - sh->rec_off[k]);
+    const size_t esz = snap_fmt == MNAD_SNAP_F32 ? 4 : 2;
+    const size_t per = 3 * shot_bytes(mt, nt, maxnr) + (size_t)K * ni * esz + 4 * ni * 4;
+    const int B = pick_batch(ctx, per, sh->nshots, batch);
+    for (int k0 = 0; k0 < sh->nshots; k0 += B) {
+        const int k1 = std::min(sh->nshots, k0 + B), nb = k1 - k0;
+        std::vector<Shot> fw(2 * nb);
+        std::vector<Shot*> run;
+        for (int b = 0; b < nb; ++b) {
+            const int k = k0 + b, r0 = sh->rec_off[k], nr = sh->rec_off[k + 1] - r0;
+            ShotSpec sp;
+            sp.mode = MODE_FWD;
+            sp.nt = nt;
+            sp.w = w;
+            sp.zs = sh->zs[k];
+            sp.xs = sh->xs[k];
+            sp.nr = nr;
+            sp.zr = sh->zr + r0;
+            sp.xr = sh->xr + r0;
+            CHK(make_shot(ctx, mt, sp, fw[2 * b]));          // observed: true model
+            sp.T = T;
+            sp.K = K;
+            sp.ncol = ncol;
+            sp.snap_fmt = snap_fmt;
+            CHK(make_shot(ctx, ms, sp, fw[2 * b + 1]));      // background + snapshots
+            run.push_back(&fw[2 * b]);
+            run.push_back(&fw[2 * b + 1]);
+        }
+        CHK(run_batch(ctx, run, flags));
+        std::vector<Shot> ad(nb);
+        run.clear();
+        for (int b = 0; b < nb; ++b) {
+            const int k = k0 + b, r0 = sh->rec_off[k], nr = sh->rec_off[k + 1] - r0;
+            if (dobs) CHK(download_gather(ctx, fw[2 * b], nt, nr, dobs + (size_t)r0 * nt));
+            if (dsyn) CHK(download_gather(ctx, fw[2 * b + 1], nt, nr, dsyn + (size_t)r0 * nt));
+            fw[2 * b].release_state();
+            fw[2 * b + 1].release_state();
+            ShotSpec sp;
+            sp.mode = MODE_ADJ;
+            sp.nt = nt;
+            sp.nr = nr;
+            sp.zr = sh->zr + r0;
+            sp.xr = sh->xr + r0;
+            sp.T = T;
+            sp.K = K;
+            sp.ncol = ncol;
+            CHK(make_shot(ctx, ms, sp, ad[b]));
+            if (ad[b].nu > 0)
+                HIPCHK(mnad_launch_make_radj(fw[2 * b].gather.as<float>(),
+                                             fw[2 * b + 1].gather.as<float>(),
+                                             ad[b].arec.as<float>(), nt, ad[b].nu,
+                                             ad[b].radj.as<float>(), ctx->stream));
+            attach_snaps(ad[b], std::move(fw[2 * b + 1].snaps), snap_fmt, fw[2 * b + 1].d.sscale);
+            run.push_back(&ad[b]);
+        }
+        CHK(run_batch(ctx, run, flags));
+        for (int b = 0; b < nb; ++b)
+            HIPCHK(mnad_launch_stack(ad[b].img.as<float>(), ad[b].imgc.as<float>(),
+                                     fw[2 * b + 1].illum.as<float>(), (float)T, (long)ni,
+                                     st.isum.as<float>(), st.isumc.as<float>(),
+                                     st.lsum.as<float>(), st.lsumc.as<float>(), ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        st.nshots += nb;
+    }
+    return MNAD_OK;
+}
+
+extern "C" int mnad_fwd_shots(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt,
+                              const double* w, const mnad_shots* sh, int batch, int flags,
+                              double* datau)
+{
+    ENTER(ctx);
+    CHK(check_shots(sh));
+    Model m;
+    CHK(make_model(ctx, g, v, m));
+    int maxnr = 0;
+    for (int k = 0; k < sh->nshots; ++k) maxnr = std::max(maxnr, sh->rec_off[k + 1] - sh->rec_off[k]);
+    const int B = pick_batch(ctx, shot_bytes(m, nt, maxnr), sh->nshots, batch);
+    for (int k0 = 0; k0 < sh->nshots; k0 += B) {
+        const int k1 = std::min(sh->nshots, k0 + B), nb = k1 - k0;
+        std::vector<Shot> fw(nb);
+        std::vector<Shot*> run;
+        for (int b = 0; b < nb; ++b) {
+            const int k = k0 + b, r0 = sh->rec_off[k];
+            ShotSpec sp;
+            sp.mode = MODE_FWD;
+            sp.nt = nt;
+            sp.w = w;
+            sp.zs = sh->zs[k];
+            sp.xs = sh->xs[k];
+            sp.nr = sh->rec_off[k + 1] - r0;
+            sp.zr = sh->zr + r0;
+            sp.xr = sh->xr + r0;
+            CHK(make_shot(ctx, m, sp, fw[b]));
+            run.push_back(&fw[b]);
+        }
+        CHK(run_batch(ctx, run, flags));
+        for (int b = 0; b < nb; ++b) {
+            const int k = k0 + b, r0 = sh->rec_off[k];
+            if (datau)
+                CHK(download_gather(ctx, fw[b], nt, sh->rec_off[k + 1] - r0, datau + (size_t)r0 * nt));
+        }
+    }
+    return MNAD_OK;
+}

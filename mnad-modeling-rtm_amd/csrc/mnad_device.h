@@ -1,0 +1,64 @@
+// mnad_device.h -- device-side descriptors shared by the HIP kernels and the
+// host runtime (plain structs, no torch types).
+//
+// Grid storage (per field, per shot): rows of `pitch` floats; grid point
+// (j, i) (j = z row, i = x column, 0-based, padded grid incl. CPML) lives at
+//     (j + MNAD_ZO) * pitch + (i + MNAD_XO).
+// Two zero rows above/below and at least two zero columns left/right are
+// never written, so the +-2 stencil reads zeros beyond the grid, which is what
+// the *_gen.m index ranges do (neighbours beyond the padded grid omitted).
+#pragma once
+
+#include <stdint.h>
+
+#define MNAD_ZO 2    // ghost rows above the grid
+#define MNAD_XO 32   // column offset of x = 0 (row starts 128-byte aligned)
+
+namespace mnad {
+
+enum SnapFmt : int { SNAP_F32 = 0, SNAP_F16 = 1, SNAP_BF16 = 2 };
+enum Mode : int { MODE_FWD = 0, MODE_ADJ = 1 };
+
+// Everything one shot needs inside the step kernel.  Lives in device memory;
+// one entry per shot of a batch.
+struct ShotDev {
+    int nz, nx, pitch, npml;        // padded grid
+    int nt;                         // number of steps
+    int mode;                       // MODE_FWD / MODE_ADJ
+    int t0;                         // global step index of this shot's step 0
+    int ntx;                        // tiles along x (for tile -> (tj, ti))
+    float kap;                      // rz / rx = dx^2 / dz^2
+    float inv_dt2;                  // 1 / dt^2 (snapshot d2u)
+    const float* rx;                // a / dx^2 per point, grid storage
+    float* f[2][3];                 // state [parity][U, Vs, Ws], grid storage
+    // CPML, strip-compact: x strips = 2*npml columns x nz rows,
+    // z strips = 2*npml rows x nx columns.
+    float* psx;                     // [9][nz * 2 npml]   (field*3 + level)
+    float* psz;                     // [9][2 npml * nx]
+    const float* cfx;               // [3][nz * 2 npml]   c1, h*c2 (signed), c3
+    const float* cfz;               // [3][2 npml * nx]
+    // source (forward)
+    int src_j, src_i;               // -1 if none
+    const float* src_amp;           // [nt + 1]: a_src * w(s + 1); [nt] = 0
+    // receivers (unique points)
+    int nrec;
+    const int* rrow;                // [nz]: offset into rcol, or -1
+    const int* rcol;                // [nrows_with_receivers * nx]: u or -1
+    float* gather;                  // forward: [nt][nrec]
+    const float* radj;              // adjoint: [nt][nrec] a_u * d(tt + 1)
+    // snapshots (forward) / imaging (adjoint)
+    int T, K, shift;                // T = 0: off; shift = ncol - K
+    int sfmt;                       // SnapFmt
+    float sscale, sinv;             // f16 scale and its inverse
+    void* snaps;                    // [K][nzi * nxi]
+    float* illum;                   // [nzi * nxi]
+    float* img;                     // [nzi * nxi] Kahan sum
+    float* imgc;                    // [nzi * nxi] Kahan compensation
+};
+
+struct TileRef {
+    int shot;   // -1: padding entry, the block exits
+    int tile;   // tile index within the shot: tj * ntx + ti
+};
+
+}  // namespace mnad

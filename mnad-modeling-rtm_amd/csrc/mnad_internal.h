@@ -1,0 +1,142 @@
+// mnad_internal.h -- host-side runtime types shared by the runtime sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mnad.h"
+#include "mnad_device.h"
+
+namespace mnad {
+
+int fail(int code, const char* fmt, ...);
+const char* last_error();
+
+// Caching device allocator: freed blocks are kept by exact size and reused
+// (repeated shots of one configuration allocate once).
+struct Pool {
+    std::mutex mu;
+    std::multimap<size_t, void*> cache;
+    size_t in_use = 0, cached = 0;
+    void* get(size_t n, int* rc);
+    void put(void* p, size_t n);
+    void trim();
+};
+
+// Owning handle of one pool block.
+struct DevBuf {
+    Pool* pool = nullptr;
+    void* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : pool(o.pool), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; o.pool = nullptr; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            reset();
+            pool = o.pool; p = o.p; n = o.n;
+            o.p = nullptr; o.n = 0; o.pool = nullptr;
+        }
+        return *this;
+    }
+    ~DevBuf() { reset(); }
+    void reset();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Source wavefield kept between mnad_fwd(indx = 1) and mnad_born_adj.
+struct Kept {
+    DevBuf snaps;
+    int fmt = 0;
+    float sscale = 1.0f;
+    int nz = 0, nx = 0, npml = 0, nt = 0, T = 0, K = 0, ncol = 0;
+};
+
+// Image / illumination stack (compensated sums over shots).
+struct Stack {
+    mnad_grid g{};
+    DevBuf isum, isumc, lsum, lsumc;
+    int nshots = 0;
+};
+
+}  // namespace mnad
+
+struct mnad_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    mnad::Pool pool;
+    std::map<int, mnad::Kept> kept;
+    std::map<int, mnad::Stack> stacks;
+    int next_stack = 1;
+    int graph_chunk = 64;
+    size_t budget = 0;          // bytes usable for batches (0 = from free memory)
+    // step-kernel accounting (mnad_timing)
+    double t_ms = 0.0;
+    long long t_launches = 0;
+    double t_bytes = 0.0;
+};
+
+namespace mnad {
+
+// Velocity-dependent device data of one model.
+struct Model {
+    mnad_grid g{};
+    int pitch = 0, rows = 0;
+    size_t elems = 0;              // floats per field (rows * pitch)
+    double vmin = 0, vmax = 0;
+    const double* vhost = nullptr;  // caller's column-major v, valid during the call
+    DevBuf rx, cfx, cfz;
+};
+
+struct ShotSpec {
+    int mode = MODE_FWD;
+    int nt = 0;
+    const double* w = nullptr;   // forward wavelet [nt]
+    int zs = -1, xs = -1;
+    int nr = 0;
+    const int* zr = nullptr;
+    const int* xr = nullptr;
+    int T = 0, K = 0, ncol = 0;  // snapshots (forward) / imaging (adjoint); T = 0: off
+    int snap_fmt = MNAD_SNAP_F32;
+};
+
+struct Shot {
+    ShotDev d{};
+    DevBuf state[6], psx, psz, amp, rrow, rcol, gather, radj, arec, snaps, illum, img, imgc;
+    std::vector<int> uid;                       // receiver r -> unique point u
+    std::vector<std::pair<int, int>> upts;      // unique points (j, i)
+    std::vector<double> arec_h;                 // (v dt)^2 at unique points
+    int nu = 0;
+    void release_state()
+    {
+        for (auto& b : state) b.reset();
+        psx.reset();
+        psz.reset();
+    }
+};
+
+int dalloc(mnad_ctx* c, DevBuf& b, size_t bytes, bool zero);
+int upload(mnad_ctx* c, DevBuf& b, const void* src, size_t bytes);
+int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m);
+int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s);
+void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
+int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);
+int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags);
+int download_gather(mnad_ctx* c, const Shot& s, int nt, int nr, double* out);
+int download_field(mnad_ctx* c, const Model& m, const Shot& s, int field, double scale,
+                   double* out);
+int download_interior(mnad_ctx* c, const mnad_grid& g, const float* a, const float* b,
+                      double scale, double* out);
+int schedule(int nt, double vmin, double vmax, int* nf, int* T, int* K, int* ncol);
+
+}  // namespace mnad

@@ -1,0 +1,386 @@
+// mnad_kernels.hip -- gfx950 kernels of the MNAD hot path.
+//
+//   k_setup_model  FP64 set-up: a/dx^2 per point and the CPML coefficients of
+//                  C_cof.m, written as FP32 strip-compact arrays.
+//   k_step         one time step of one or many shots (forward or adjoint):
+//                  the coupled (U, Vs, Ws) update of fwd_PDE_exact_MNAD.m:114-116
+//                  with the CPML memories of Ux123_gen.m/Uz123_gen.m folded in on
+//                  the strips, source injection, receiver sampling
+//                  (fwd_PDE_exact_MNAD.m:111,119-123), receiver injection
+//                  (Born_adj_MNAD.m:109-111), d2u snapshots + illumination
+//                  (fwd_PDE_exact_MNAD.m:126-143) and the zero-lag imaging
+//                  condition with compensated FP32 sums (Born_adj_MNAD.m:127-134).
+//   k_advance      bumps the device-side step counter (end of a graph chunk).
+//   k_make_radj    a_u * (d_obs - d_syn) per adjoint step (main_Acoustic_RTM.m:225
+//                  and Born_adj_MNAD.m:110), from two device gathers.
+//   k_stack        image / illumination stack of main_Acoustic_RTM.m:237-262.
+//
+// Injection bookkeeping.  The reference injects the source into the CURRENT
+// U before the update (fwd_PDE_exact_MNAD.m:111) and the receiver data into
+// the current U after it (Born_adj_MNAD.m:109-111).  Both are folded into the
+// output of the step before, so no extra launch and no read/write race on U:
+//   forward: step s writes  U_new(src) += a_src w(s+2)  (the next step's
+//            injection); the CPML memories, which see the pre-injection U in the
+//            reference, subtract it back when they read U at the source point.
+//   adjoint: the injected value only survives as the next step's -U_prev, so
+//            step s writes U_new(rec) -= a_rec d(tt+1).
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include "mnad_device.h"
+#include "mnad_stencil.h"
+
+namespace mnad {
+
+constexpr int BX = 64;   // tile width  (x), one wavefront per tile row
+constexpr int BZ = 4;    // tile height (z)
+
+// ------------------------------------------------------------------ set-up
+// v: column-major [nz x nx] FP64 as MATLAB passes it (v(j,i) = v[j + i*nz]).
+// a = v^2 dt^2 (fwd_PDE_exact_MNAD.m:39; *_gen.m use v(j,i).^2*dt^2).
+// CPML (C_cof.m:4-65): sigma = 3 ln(1/R)/(2 L^3) v x^2, sigma' = 2 (...) v x,
+// den = 1 + sigma dt/2, c1 = (1 - sigma dt/2)/den, c2 = sigma' dt/den,
+// c3 = sigma dt/den; x = distance into the layer (h ... L), L = npml h.
+__global__ void k_setup_model(const double* __restrict__ v, int nz, int nx,
+                              int pitch, int npml, double dx, double dz,
+                              double dt, double R, float* __restrict__ rx,
+                              float* __restrict__ cfx, float* __restrict__ cfz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= nx || j >= nz) return;
+    const double vv = v[(size_t)j + (size_t)i * nz];
+    const double a = vv * vv * (dt * dt);
+    rx[(size_t)(j + MNAD_ZO) * pitch + (i + MNAD_XO)] = (float)(a / (dx * dx));
+    if (npml <= 0) return;
+    const int np2 = 2 * npml;
+    if (i < npml || i >= nx - npml) {
+        const double L = npml * dx;
+        const double k = (3.0 * log(1.0 / R)) / (2.0 * L * L * L);
+        const bool left = i < npml;
+        const double x = left ? (double)(npml - i) * dx : (double)(i - (nx - npml) + 1) * dx;
+        const double sig0 = k * vv;
+        const double sig = sig0 * (x * x);
+        const double dsig = 2.0 * sig0 * x;
+        const double den = 1.0 + sig * (dt / 2.0);
+        const int ix = left ? i : i - (nx - np2);
+        const size_t q = (size_t)j * np2 + ix, ns = (size_t)nz * np2;
+        cfx[q] = (float)((1.0 - sig * (dt / 2.0)) / den);
+        cfx[ns + q] = (float)((left ? dx : -dx) * ((dsig * dt) / den));
+        cfx[2 * ns + q] = (float)((sig * dt) / den);
+    }
+    if (j < npml || j >= nz - npml) {
+        const double L = npml * dz;
+        const double k = (3.0 * log(1.0 / R)) / (2.0 * L * L * L);
+        const bool top = j < npml;
+        const double x = top ? (double)(npml - j) * dz : (double)(j - (nz - npml) + 1) * dz;
+        const double sig0 = k * vv;
+        const double sig = sig0 * (x * x);
+        const double dsig = 2.0 * sig0 * x;
+        const double den = 1.0 + sig * (dt / 2.0);
+        const int jz = top ? j : j - (nz - np2);
+        const size_t q = (size_t)jz * nx + i, ns = (size_t)np2 * nx;
+        cfz[q] = (float)((1.0 - sig * (dt / 2.0)) / den);
+        cfz[ns + q] = (float)((top ? dz : -dz) * ((dsig * dt) / den));
+        cfz[2 * ns + q] = (float)((sig * dt) / den);
+    }
+}
+
+// ------------------------------------------------------------ step kernel
+__device__ __forceinline__ void load9(const float* __restrict__ F, long p, int P, Nbh& o)
+{
+    o.nw = F[p - P - 1]; o.n = F[p - P]; o.ne = F[p - P + 1];
+    o.w = F[p - 1];      o.c = F[p];     o.e = F[p + 1];
+    o.sw = F[p + P - 1]; o.s = F[p + P]; o.se = F[p + P + 1];
+}
+
+__device__ __forceinline__ void store_snap(const ShotDev& d, long idx, float x)
+{
+    if (d.sfmt == SNAP_F32) {
+        static_cast<float*>(d.snaps)[idx] = x;
+    } else if (d.sfmt == SNAP_F16) {
+        static_cast<__half*>(d.snaps)[idx] = __float2half(x * d.sscale);
+    } else {
+        static_cast<__hip_bfloat16*>(d.snaps)[idx] = __float2bfloat16(x);
+    }
+}
+
+__device__ __forceinline__ float load_snap(const ShotDev& d, long idx)
+{
+    if (d.sfmt == SNAP_F32) return static_cast<const float*>(d.snaps)[idx];
+    if (d.sfmt == SNAP_F16) return __half2float(static_cast<const __half*>(d.snaps)[idx]) * d.sinv;
+    return __bfloat162float(static_cast<const __hip_bfloat16*>(d.snaps)[idx]);
+}
+
+// Three CPML memory triples (U, Vs, Ws) of one strip point along one axis.
+// ps: 9 planes of `ns` floats (field*3 + level), cf: 3 planes (c1, h c2, c3).
+__device__ __forceinline__ void psi3(float* __restrict__ ps, const float* __restrict__ cf,
+                                     long ns, long q, float r,
+                                     float ud, float uc, float ua,
+                                     float vd, float vc, float va,
+                                     float wd, float wc, float wa,
+                                     float& du, float& dv, float& dw)
+{
+    const float c1 = cf[q], hc2 = cf[ns + q], c3 = cf[2 * ns + q];
+    float p1, p2, p3;
+    p1 = ps[q]; p2 = ps[ns + q]; p3 = ps[2 * ns + q];
+    du -= r * mnad_psi(p1, p2, p3, ud, uc, ua, c1, hc2, c3);
+    ps[q] = p1; ps[ns + q] = p2; ps[2 * ns + q] = p3;
+    p1 = ps[3 * ns + q]; p2 = ps[4 * ns + q]; p3 = ps[5 * ns + q];
+    dv -= r * mnad_psi(p1, p2, p3, vd, vc, va, c1, hc2, c3);
+    ps[3 * ns + q] = p1; ps[4 * ns + q] = p2; ps[5 * ns + q] = p3;
+    p1 = ps[6 * ns + q]; p2 = ps[7 * ns + q]; p3 = ps[8 * ns + q];
+    dw -= r * mnad_psi(p1, p2, p3, wd, wc, wa, c1, hc2, c3);
+    ps[6 * ns + q] = p1; ps[7 * ns + q] = p2; ps[8 * ns + q] = p3;
+}
+
+// One step for every tile in `tiles`.  Block = BX x BZ threads = one tile.
+// Local step of a shot: s = *base + rel - t0 (0-based); tiles of shots outside
+// [0, nt) exit at once, so a graph chunk may run past a shot's end.
+__global__ __launch_bounds__(BX * BZ) void k_step(const ShotDev* __restrict__ shots,
+                                                  const TileRef* __restrict__ tiles,
+                                                  const int* __restrict__ base, int rel)
+{
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int i = ti * BX + threadIdx.x;
+    const int j = tj * BZ + threadIdx.y;
+    if (i >= d.nx || j >= d.nz) return;
+
+    const int cb = s & 1;
+    const float* __restrict__ U = d.f[cb][0];
+    const float* __restrict__ V = d.f[cb][1];
+    const float* __restrict__ W = d.f[cb][2];
+    float* __restrict__ PU = d.f[cb ^ 1][0];
+    float* __restrict__ PV = d.f[cb ^ 1][1];
+    float* __restrict__ PW = d.f[cb ^ 1][2];
+    const int P = d.pitch;
+    const long p = (long)(j + MNAD_ZO) * P + (i + MNAD_XO);
+
+    Nbh u, v, w;
+    load9(U, p, P, u);
+    load9(V, p, P, v);
+    load9(W, p, P, w);
+    const float vww = V[p - 2], vee = V[p + 2];
+    const float wnn = W[p - 2 * P], wss = W[p + 2 * P];
+    const float rx = d.rx[p];
+    const float rz = rx * d.kap;
+    const float pu = PU[p], pv = PV[p], pw = PW[p];
+
+    float du, dv, dw;
+    mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+
+    const int np = d.npml;
+    const bool inx = (i < np) || (i >= d.nx - np);
+    const bool inz = (j < np) || (j >= d.nz - np);
+    if (inx || inz) {
+        // The CPML memories see the pre-injection U (fwd_PDE_exact_MNAD.m:95-111).
+        float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
+        if (d.mode == MODE_FWD && d.src_j >= 0) {
+            const float a0 = d.src_amp[s];
+            if (j == d.src_j) {
+                if (i == d.src_i) uc -= a0;
+                if (i - 1 == d.src_i) uw -= a0;
+                if (i + 1 == d.src_i) ue -= a0;
+            }
+            if (i == d.src_i) {
+                if (j - 1 == d.src_j) un -= a0;
+                if (j + 1 == d.src_j) us -= a0;
+            }
+        }
+        if (inx) {
+            const long ns = (long)d.nz * 2 * np;
+            const int ix = (i < np) ? i : i - (d.nx - 2 * np);
+            const long q = (long)j * 2 * np + ix;
+            psi3(d.psx, d.cfx, ns, q, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+        }
+        if (inz) {
+            const long ns = (long)2 * np * d.nx;
+            const int jz = (j < np) ? j : j - (d.nz - 2 * np);
+            const long q = (long)jz * d.nx + i;
+            psi3(d.psz, d.cfz, ns, q, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+        }
+    }
+
+    float nu = (2.0f * u.c + du) - pu;
+    const float nv = (2.0f * v.c + dv) - pv;
+    const float nw = (2.0f * w.c + dw) - pw;
+
+    int r = -1;
+    if (d.nrec > 0) {
+        const int off = d.rrow[j];
+        if (off >= 0) r = d.rcol[off + i];
+    }
+    if (d.mode == MODE_FWD) {
+        if (j == d.src_j && i == d.src_i) nu += d.src_amp[s + 1];
+        if (r >= 0) d.gather[(long)s * d.nrec + r] = u.c;
+    } else if (r >= 0) {
+        nu -= d.radj[(long)s * d.nrec + r];
+    }
+    PU[p] = nu;
+    PV[p] = nv;
+    PW[p] = nw;
+
+    if (d.T > 0 && !inx && !inz) {
+        const int nxi = d.nx - 2 * np;
+        const long q = (long)(j - np) * nxi + (i - np);
+        const long ni = (long)(d.nz - 2 * np) * nxi;
+        if (d.mode == MODE_FWD) {
+            if ((s + 1) % d.T == 0) {
+                // d2u = (nU - 2U + pU)/dt^2 = du/dt^2; snapshot k = (s+1)/T - 1.
+                const int k = (s + 1) / d.T - 1;
+                const float d2u = du * d.inv_dt2;
+                store_snap(d, (long)k * ni + q, d2u);
+                d.illum[q] += d2u * d2u;
+            }
+        } else {
+            const int tt = d.nt - s;
+            if (tt % d.T == 0) {
+                // UU(:, ncol - ttt + 1) = snapshot column shift + tt/T (1-based).
+                const int col = d.shift + tt / d.T;
+                if (col <= d.K) {
+                    const float x = load_snap(d, (long)(col - 1) * ni + q) * nu;
+                    const float sum = d.img[q], c = d.imgc[q];
+                    const float y = x - c;
+                    const float t = sum + y;
+                    d.imgc[q] = (t - sum) - y;
+                    d.img[q] = t;
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_advance(int* base, int n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *base += n;
+}
+
+// Initial state: U at the source holds the first injection a_src w(1).
+__global__ void k_init_source(ShotDev* shots, int nshots)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nshots) return;
+    const ShotDev& d = shots[k];
+    if (d.mode != MODE_FWD || d.src_j < 0) return;
+    d.f[0][0][(long)(d.src_j + MNAD_ZO) * d.pitch + (d.src_i + MNAD_XO)] = d.src_amp[0];
+}
+
+// radj[s][u] = a_u * (dobs[nt-s][u] - dsyn[nt-s][u]) for s >= 1, 0 for s = 0.
+__global__ void k_make_radj(const float* __restrict__ dobs, const float* __restrict__ dsyn,
+                            const float* __restrict__ arec, int nt, int nrec,
+                            float* __restrict__ radj)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)nt * nrec) return;
+    const int s = (int)(k / nrec);
+    const int u = (int)(k - (long)s * nrec);
+    float val = 0.0f;
+    if (s >= 1) {
+        const long src = (long)(nt - s) * nrec + u;
+        val = arec[u] * (dobs[src] - (dsyn ? dsyn[src] : 0.0f));
+    }
+    radj[k] = val;
+}
+
+// Compensated accumulation of one shot's image and illumination into the
+// stack: img_sum += T*(img - imgc), ill_sum += illum (illum already holds the
+// un-scaled sum; the T factor is applied here, fwd_PDE_exact_MNAD.m:162).
+__global__ void k_stack(const float* __restrict__ img, const float* __restrict__ imgc,
+                        const float* __restrict__ illum, float T, long n,
+                        float* __restrict__ isum, float* __restrict__ isumc,
+                        float* __restrict__ lsum, float* __restrict__ lsumc)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    {
+        const float x = T * (img[q] - imgc[q]);
+        const float s = isum[q], c = isumc[q];
+        const float y = x - c, t = s + y;
+        isumc[q] = (t - s) - y;
+        isum[q] = t;
+    }
+    if (illum) {
+        const float x = T * illum[q];
+        const float s = lsum[q], c = lsumc[q];
+        const float y = x - c, t = s + y;
+        lsumc[q] = (t - s) - y;
+        lsum[q] = t;
+    }
+}
+
+__global__ void k_fill(float* __restrict__ p, long n, float val)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = val;
+}
+
+// ------------------------------------------------------------ host launchers
+extern "C" {
+
+hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, int npml,
+                                   double dx, double dz, double dt, float* rx, float* cfx,
+                                   float* cfz, hipStream_t st)
+{
+    dim3 blk(64, 4);
+    dim3 grd((nx + 63) / 64, (nz + 3) / 4);
+    hipLaunchKernelGGL(k_setup_model, grd, blk, 0, st, v, nz, nx, pitch, npml, dx, dz, dt,
+                       1e-4, rx, cfx, cfz);
+    return hipGetLastError();
+}
+
+hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
+                            const int* base, int rel, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step, dim3(ntiles), dim3(BX, BZ), 0, st, shots, tiles, base, rel);
+    return hipGetLastError();
+}
+
+hipError_t mnad_launch_advance(int* base, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, base, n);
+    return hipGetLastError();
+}
+
+hipError_t mnad_launch_init_source(ShotDev* shots, int nshots, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init_source, dim3((nshots + 63) / 64), dim3(64), 0, st, shots, nshots);
+    return hipGetLastError();
+}
+
+hipError_t mnad_launch_make_radj(const float* dobs, const float* dsyn, const float* arec,
+                                 int nt, int nrec, float* radj, hipStream_t st)
+{
+    const long n = (long)nt * nrec;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_make_radj, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       dobs, dsyn, arec, nt, nrec, radj);
+    return hipGetLastError();
+}
+
+hipError_t mnad_launch_stack(const float* img, const float* imgc, const float* illum, float T,
+                             long n, float* isum, float* isumc, float* lsum, float* lsumc,
+                             hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, img, imgc,
+                       illum, T, n, isum, isumc, lsum, lsumc);
+    return hipGetLastError();
+}
+
+hipError_t mnad_launch_fill(float* p, long n, float val, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, val);
+    return hipGetLastError();
+}
+
+}  // extern "C"
+
+}  // namespace mnad

@@ -1,0 +1,563 @@
+// mnad_runtime.hip -- host runtime behind include/mnad.h.
+//
+// Owns the device context (stream, caching memory pool, kept source
+// wavefields, image stacks), builds per-shot device descriptors, and drives
+// the step kernel over batches of shots: one launch per time step covers
+// every tile of every shot in the batch (the reference's parfor over shots,
+// main_Acoustic_Data.m:169-171, main_Acoustic_RTM.m:223-226,250-254, becomes
+// the batch axis), replayed in hipGraph chunks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mnad.h"
+#include "mnad_device.h"
+#include "mnad_internal.h"
+
+namespace mnad {
+
+extern "C" {
+hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, int npml,
+                                   double dx, double dz, double dt, float* rx, float* cfx,
+                                   float* cfz, hipStream_t st);
+hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
+                            const int* base, int rel, hipStream_t st);
+hipError_t mnad_launch_advance(int* base, int n, hipStream_t st);
+hipError_t mnad_launch_init_source(ShotDev* shots, int nshots, hipStream_t st);
+hipError_t mnad_launch_make_radj(const float* dobs, const float* dsyn, const float* arec,
+                                 int nt, int nrec, float* radj, hipStream_t st);
+hipError_t mnad_launch_stack(const float* img, const float* imgc, const float* illum, float T,
+                             long n, float* isum, float* isumc, float* lsum, float* lsumc,
+                             hipStream_t st);
+hipError_t mnad_launch_fill(float* p, long n, float val, hipStream_t st);
+}
+
+static thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? MNAD_ERR_NOMEM : MNAD_ERR_HIP,     \
+                        "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                     \
+    } while (0)
+
+#define CHK(x)                  \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_ != MNAD_OK)     \
+            return rc_;         \
+    } while (0)
+
+// ------------------------------------------------------------------ pool
+void* Pool::get(size_t n, int* rc)
+{
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(n);
+    if (it != cache.end()) {
+        void* p = it->second;
+        cache.erase(it);
+        cached -= n;
+        in_use += n;
+        *rc = MNAD_OK;
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        // Free everything cached and retry once.
+        for (auto& kv : cache) (void)hipFree(kv.second);
+        cache.clear();
+        cached = 0;
+        e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *rc = fail(MNAD_ERR_NOMEM, "device allocation of %zu bytes failed (%zu in use)", n,
+                       in_use);
+            return nullptr;
+        }
+    }
+    in_use += n;
+    *rc = MNAD_OK;
+    return p;
+}
+
+void Pool::put(void* p, size_t n)
+{
+    std::lock_guard<std::mutex> lk(mu);
+    cache.emplace(n, p);
+    in_use -= n;
+    cached += n;
+}
+
+void Pool::trim()
+{
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& kv : cache) (void)hipFree(kv.second);
+    cache.clear();
+    cached = 0;
+}
+
+void DevBuf::reset()
+{
+    if (p && pool) pool->put(p, n);
+    p = nullptr;
+    n = 0;
+    pool = nullptr;
+}
+
+int dalloc(mnad_ctx* c, DevBuf& b, size_t bytes, bool zero)
+{
+    b.reset();
+    if (bytes == 0) return MNAD_OK;
+    bytes = (bytes + 255) & ~size_t(255);
+    int rc;
+    void* p = c->pool.get(bytes, &rc);
+    if (!p) return rc;
+    b.p = p;
+    b.n = bytes;
+    b.pool = &c->pool;
+    if (zero) HIPCHK(hipMemsetAsync(p, 0, bytes, c->stream));
+    return MNAD_OK;
+}
+
+int upload(mnad_ctx* c, DevBuf& b, const void* src, size_t bytes)
+{
+    CHK(dalloc(c, b, bytes, false));
+    if (bytes) {
+        // The caller's host buffer may be freed on return: finish the copy.
+        HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return MNAD_OK;
+}
+
+// ----------------------------------------------------------------- model
+static int check_grid(const mnad_grid* g)
+{
+    if (!g) return fail(MNAD_ERR_ARG, "grid is NULL");
+    if (g->nz < 3 || g->nx < 3 || g->npml < 0)
+        return fail(MNAD_ERR_ARG, "grid %dx%d (npml %d) too small", g->nz, g->nx, g->npml);
+    if (g->nx < 2 * g->npml + 1 || g->nz < 2 * g->npml + 1)
+        return fail(MNAD_ERR_ARG,
+                    "grid %dx%d cannot hold CPML strips of %d cells on each side "
+                    "(C_cof.m:6,38 needs nx, nz >= 2*d_pml)",
+                    g->nz, g->nx, g->npml);
+    if (!(g->dx > 0) || !(g->dz > 0) || !(g->dt > 0) || !std::isfinite(g->dx) ||
+        !std::isfinite(g->dz) || !std::isfinite(g->dt))
+        return fail(MNAD_ERR_ARG, "dx, dz, dt must be positive and finite");
+    if ((long)g->nz * (long)g->nx > (1L << 30))
+        return fail(MNAD_ERR_ARG, "grid of %ld points exceeds the 2^30 limit",
+                    (long)g->nz * g->nx);
+    return MNAD_OK;
+}
+
+int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m)
+{
+    CHK(check_grid(g));
+    if (!v) return fail(MNAD_ERR_ARG, "velocity is NULL");
+    const size_t n = (size_t)g->nz * g->nx;
+    double vmin = v[0], vmax = v[0];
+    for (size_t k = 0; k < n; ++k) {
+        const double x = v[k];
+        if (!std::isfinite(x) || !(x > 0))
+            return fail(MNAD_ERR_ARG, "velocity must be finite and positive (v[%zu] = %g)", k, x);
+        vmin = std::min(vmin, x);
+        vmax = std::max(vmax, x);
+    }
+    const double courant = vmax * g->dt / std::min(g->dx, g->dz);
+    if (courant > 1.0)
+        return fail(MNAD_ERR_ARG,
+                    "vmax*dt/min(dx,dz) = %.3f > 1: unstable (the reference picks "
+                    "dt = 0.8*0.65*dx/vmax, main_Acoustic_Data.m:71,77)",
+                    courant);
+    m.g = *g;
+    m.vmin = vmin;
+    m.vmax = vmax;
+    m.vhost = v;
+    m.pitch = ((MNAD_XO + g->nx + 2) + 31) / 32 * 32;
+    m.rows = g->nz + 2 * MNAD_ZO;
+    m.elems = (size_t)m.rows * m.pitch;
+    DevBuf dv;
+    CHK(upload(c, dv, v, n * sizeof(double)));
+    CHK(dalloc(c, m.rx, m.elems * sizeof(float), true));
+    const int np = g->npml;
+    if (np > 0) {
+        CHK(dalloc(c, m.cfx, 3 * (size_t)g->nz * 2 * np * sizeof(float), true));
+        CHK(dalloc(c, m.cfz, 3 * (size_t)2 * np * g->nx * sizeof(float), true));
+    }
+    HIPCHK(mnad_launch_setup_model(dv.as<double>(), g->nz, g->nx, m.pitch, np, g->dx, g->dz,
+                                   g->dt, m.rx.as<float>(), m.cfx.as<float>(),
+                                   m.cfz.as<float>(), c->stream));
+    // dv goes back to the pool when this returns; the stream order keeps the
+    // set-up kernel ahead of any later reuse of that memory.
+    return MNAD_OK;
+}
+
+// ------------------------------------------------------------------ shot
+static size_t snap_elem(int fmt) { return fmt == MNAD_SNAP_F32 ? 4 : 2; }
+
+// Unique receiver points and the row/column lookup tables of the kernel.
+static int build_receivers(mnad_ctx* c, const Model& m, int nr, const int* zr, const int* xr,
+                           Shot& s)
+{
+    const mnad_grid& g = m.g;
+    s.uid.assign(nr, -1);
+    s.upts.clear();
+    std::map<std::pair<int, int>, int> seen;
+    for (int r = 0; r < nr; ++r) {
+        if (zr[r] < 0 || zr[r] >= g.nz || xr[r] < 0 || xr[r] >= g.nx)
+            return fail(MNAD_ERR_ARG, "receiver %d at (%d,%d) outside the %dx%d grid", r, zr[r],
+                        xr[r], g.nz, g.nx);
+        auto key = std::make_pair(zr[r], xr[r]);
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+            int u = (int)s.upts.size();
+            seen.emplace(key, u);
+            s.upts.push_back(key);
+            s.uid[r] = u;
+        } else {
+            s.uid[r] = it->second;
+        }
+    }
+    s.nu = (int)s.upts.size();
+    s.d.nrec = s.nu;
+    if (s.nu == 0) return MNAD_OK;
+    std::vector<int> rrow(g.nz, -1);
+    std::vector<int> rows;
+    for (auto& pt : s.upts)
+        if (rrow[pt.first] < 0) {
+            rrow[pt.first] = (int)rows.size() * g.nx;
+            rows.push_back(pt.first);
+        }
+    std::vector<int> rcol(rows.size() * (size_t)g.nx, -1);
+    for (int u = 0; u < s.nu; ++u) rcol[rrow[s.upts[u].first] + s.upts[u].second] = u;
+    CHK(upload(c, s.rrow, rrow.data(), rrow.size() * sizeof(int)));
+    CHK(upload(c, s.rcol, rcol.data(), rcol.size() * sizeof(int)));
+    s.d.rrow = s.rrow.as<int>();
+    s.d.rcol = s.rcol.as<int>();
+    // a = (v dt)^2 at each unique receiver point (Born_adj_MNAD.m:36,110).
+    s.arec_h.resize(s.nu);
+    for (int u = 0; u < s.nu; ++u) {
+        const double vv = m.vhost[(size_t)s.upts[u].first + (size_t)s.upts[u].second * g.nz];
+        s.arec_h[u] = (vv * g.dt) * (vv * g.dt);
+    }
+    return MNAD_OK;
+}
+
+int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
+{
+    const mnad_grid& g = m.g;
+    if (sp.nt < 1) return fail(MNAD_ERR_ARG, "nt must be >= 1");
+    s.d = ShotDev{};
+    s.d.nz = g.nz;
+    s.d.nx = g.nx;
+    s.d.pitch = m.pitch;
+    s.d.npml = g.npml;
+    s.d.nt = sp.nt;
+    s.d.mode = sp.mode;
+    s.d.kap = (float)((g.dx * g.dx) / (g.dz * g.dz));
+    s.d.inv_dt2 = (float)(1.0 / (g.dt * g.dt));
+    s.d.rx = m.rx.as<float>();
+    s.d.ntx = (g.nx + 63) / 64;
+    s.d.src_j = s.d.src_i = -1;
+    for (int k = 0; k < 6; ++k) {
+        CHK(dalloc(c, s.state[k], m.elems * sizeof(float), true));
+        s.d.f[k / 3][k % 3] = s.state[k].as<float>();
+    }
+    const int np = g.npml;
+    if (np > 0) {
+        CHK(dalloc(c, s.psx, 9 * (size_t)g.nz * 2 * np * sizeof(float), true));
+        CHK(dalloc(c, s.psz, 9 * (size_t)2 * np * g.nx * sizeof(float), true));
+        s.d.psx = s.psx.as<float>();
+        s.d.psz = s.psz.as<float>();
+        s.d.cfx = m.cfx.as<float>();
+        s.d.cfz = m.cfz.as<float>();
+    }
+    CHK(build_receivers(c, m, sp.nr, sp.zr, sp.xr, s));
+    const size_t ni = (size_t)(g.nz - 2 * np) * (g.nx - 2 * np);
+    if (sp.mode == MODE_FWD) {
+        if (sp.zs < 0 || sp.zs >= g.nz || sp.xs < 0 || sp.xs >= g.nx)
+            return fail(MNAD_ERR_ARG, "source (%d,%d) outside the %dx%d grid", sp.zs, sp.xs,
+                        g.nz, g.nx);
+        if (!sp.w) return fail(MNAD_ERR_ARG, "wavelet is NULL");
+        s.d.src_j = sp.zs;
+        s.d.src_i = sp.xs;
+        // a_src * w(tt): fwd_PDE_exact_MNAD.m:39,111 (FP64 on the host).
+        const double vs = m.vhost[(size_t)sp.zs + (size_t)sp.xs * g.nz];
+        const double asrc = vs * vs * (g.dt * g.dt);
+        std::vector<float> amp(sp.nt + 1, 0.0f);
+        double wmax = 0;
+        for (int t = 0; t < sp.nt; ++t) {
+            if (!std::isfinite(sp.w[t])) return fail(MNAD_ERR_ARG, "wavelet sample %d is not finite", t);
+            amp[t] = (float)(asrc * sp.w[t]);
+            wmax = std::max(wmax, std::fabs(sp.w[t]));
+        }
+        CHK(upload(c, s.amp, amp.data(), amp.size() * sizeof(float)));
+        s.d.src_amp = s.amp.as<float>();
+        if (s.nu > 0) {
+            CHK(dalloc(c, s.gather, (size_t)sp.nt * s.nu * sizeof(float), true));
+            s.d.gather = s.gather.as<float>();
+        }
+        if (sp.T > 0) {
+            s.d.T = sp.T;
+            s.d.K = sp.K;
+            s.d.shift = sp.ncol - sp.K;
+            s.d.sfmt = sp.snap_fmt;
+            // FP16: power-of-two scale mapping the largest expected d2u (the
+            // source point, ~4 a_src max|w| / dt^2) to 2^12, far below 65504.
+            double est = 4.0 * asrc * std::max(wmax, 1e-30) / (g.dt * g.dt);
+            double sc = std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / est)));
+            s.d.sscale = (float)sc;
+            s.d.sinv = (float)(1.0 / sc);
+            CHK(dalloc(c, s.snaps, (size_t)sp.K * ni * snap_elem(sp.snap_fmt), false));
+            CHK(dalloc(c, s.illum, ni * sizeof(float), true));
+            s.d.snaps = s.snaps.p;
+            s.d.illum = s.illum.as<float>();
+        }
+    } else {
+        if (s.nu > 0) {
+            CHK(dalloc(c, s.radj, (size_t)sp.nt * s.nu * sizeof(float), true));
+            s.d.radj = s.radj.as<float>();
+            std::vector<float> ah(s.arec_h.begin(), s.arec_h.end());
+            CHK(upload(c, s.arec, ah.data(), ah.size() * sizeof(float)));
+        }
+        if (sp.T > 0) {
+            s.d.T = sp.T;
+            s.d.K = sp.K;
+            s.d.shift = sp.ncol - sp.K;
+            CHK(dalloc(c, s.img, ni * sizeof(float), true));
+            CHK(dalloc(c, s.imgc, ni * sizeof(float), true));
+            s.d.img = s.img.as<float>();
+            s.d.imgc = s.imgc.as<float>();
+        }
+    }
+    return MNAD_OK;
+}
+
+// Attach snapshots (from a forward shot or a kept handle) to an adjoint shot.
+void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale)
+{
+    adj.snaps = std::move(snaps);
+    adj.d.snaps = adj.snaps.p;
+    adj.d.sfmt = fmt;
+    adj.d.sscale = sscale;
+    adj.d.sinv = 1.0f / sscale;
+}
+
+// Host-side radj for an adjoint from a [nt x nr] column-major data matrix:
+// radj[s][u] = a_u * sum_{r -> u} d(nt - s + 1, r) (1-based time), s >= 1.
+int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data)
+{
+    if (s.nu == 0) return MNAD_OK;
+    std::vector<double> acc((size_t)nt * s.nu, 0.0);
+    for (int r = 0; r < nr; ++r) {
+        const int u = s.uid[r];
+        for (int t = 0; t < nt; ++t) acc[(size_t)t * s.nu + u] += data[(size_t)t + (size_t)r * nt];
+    }
+    std::vector<float> radj((size_t)nt * s.nu, 0.0f);
+    for (int st = 1; st < nt; ++st)
+        for (int u = 0; u < s.nu; ++u)
+            radj[(size_t)st * s.nu + u] = (float)(s.arec_h[u] * acc[(size_t)(nt - st) * s.nu + u]);
+    for (double x : acc)
+        if (!std::isfinite(x)) return fail(MNAD_ERR_ARG, "data contains NaN/Inf");
+    HIPCHK(hipMemcpyAsync(s.radj.p, radj.data(), radj.size() * sizeof(float),
+                          hipMemcpyHostToDevice, c->stream));
+    // keep the host buffer alive until the copy is done
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MNAD_OK;
+}
+
+// ------------------------------------------------------------ batch runner
+int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
+{
+    const int n = (int)shots.size();
+    if (n == 0) return MNAD_OK;
+    std::vector<ShotDev> h(n);
+    int nsteps = 0;
+    std::vector<TileRef> list;
+    for (int k = 0; k < n; ++k) {
+        h[k] = shots[k]->d;
+        h[k].t0 = 0;
+        nsteps = std::max(nsteps, h[k].nt);
+        const int ntx = (h[k].nx + 63) / 64, ntz = (h[k].nz + 3) / 4;
+        h[k].ntx = ntx;
+        for (int t = 0; t < ntx * ntz; ++t) list.push_back(TileRef{k, t});
+    }
+    // XCD-aware order: workgroup b is dispatched to XCD b % 8, so XCD x gets
+    // the contiguous run [x*L/8, (x+1)*L/8) of the spatial tile list and
+    // vertically adjacent tiles share halo rows in the same L2.
+    const int nx8 = 8;
+    const size_t L = (list.size() + nx8 - 1) / nx8 * nx8, per = L / nx8;
+    std::vector<TileRef> table(L, TileRef{-1, 0});
+    for (size_t b = 0; b < L; ++b) {
+        const size_t src = (b % nx8) * per + b / nx8;
+        if (src < list.size()) table[b] = list[src];
+    }
+    DevBuf dsh, dtab, dbase;
+    CHK(upload(c, dsh, h.data(), h.size() * sizeof(ShotDev)));
+    CHK(upload(c, dtab, table.data(), table.size() * sizeof(TileRef)));
+    CHK(dalloc(c, dbase, sizeof(int), true));
+    HIPCHK(mnad_launch_init_source(dsh.as<ShotDev>(), n, c->stream));
+    // Algorithmic HBM bytes of the whole sequence (see mnad_timing).
+    double bytes = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const ShotDev& d = h[k];
+        const double N = (double)d.nz * d.nx, np2 = 2.0 * d.npml;
+        const double strips = np2 * d.nz + np2 * d.nx;
+        const double ni = (double)(d.nz - 2 * d.npml) * (d.nx - 2 * d.npml);
+        const double esz = d.sfmt == SNAP_F32 ? 4.0 : 2.0;
+        bytes += (double)d.nt * (40.0 * N + 72.0 * strips + 4.0 * d.nrec);
+        if (d.T > 0) bytes += (double)d.K * ni * (esz + (d.mode == MODE_FWD ? 8.0 : 16.0));
+    }
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } evg{ev0, ev1};
+    HIPCHK(hipEventRecord(ev0, c->stream));
+    const int ntiles = (int)table.size();
+    const ShotDev* ds = dsh.as<ShotDev>();
+    const TileRef* dt = dtab.as<TileRef>();
+    int* base = dbase.as<int>();
+    int chunk = c->graph_chunk;
+    if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nsteps < 2 * chunk) {
+        for (int s = 0; s < nsteps; ++s) HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s, c->stream));
+    } else {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = hipSuccess;
+        for (int r = 0; r < chunk && ce == hipSuccess; ++r)
+            ce = mnad_launch_step(ds, dt, ntiles, base, r, c->stream);
+        if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk, c->stream);
+        hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        HIPCHK(ce);
+        HIPCHK(ee);
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            HIPCHK(ie);
+        }
+        const int full = nsteps / chunk, rem = nsteps - full * chunk;
+        hipError_t le = hipSuccess;
+        for (int k = 0; k < full && le == hipSuccess; ++k) le = hipGraphLaunch(exec, c->stream);
+        for (int r = 0; r < rem && le == hipSuccess; ++r)
+            le = mnad_launch_step(ds, dt, ntiles, base, r, c->stream);
+        hipError_t se = hipStreamSynchronize(c->stream);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(le);
+        HIPCHK(se);
+    }
+    HIPCHK(hipEventRecord(ev1, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    c->t_ms += ms;
+    c->t_launches += nsteps;
+    c->t_bytes += bytes;
+    return MNAD_OK;
+}
+
+// Gathers of a forward shot -> column-major [nt x nr] doubles (duplicated
+// receivers read the same unique point).
+int download_gather(mnad_ctx* c, const Shot& s, int nt, int nr, double* out)
+{
+    if (!out || nr == 0) return MNAD_OK;
+    std::vector<float> g((size_t)nt * s.nu);
+    HIPCHK(hipMemcpyAsync(g.data(), s.gather.p, g.size() * sizeof(float), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < nr; ++r) {
+        const int u = s.uid[r];
+        for (int t = 0; t < nt; ++t) out[(size_t)t + (size_t)r * nt] = g[(size_t)t * s.nu + u];
+    }
+    return MNAD_OK;
+}
+
+// Final level of a field (parity nt & 1) -> column-major doubles, times scale.
+int download_field(mnad_ctx* c, const Model& m, const Shot& s, int field, double scale,
+                   double* out)
+{
+    if (!out) return MNAD_OK;
+    const mnad_grid& g = m.g;
+    std::vector<float> buf(m.elems);
+    const int par = s.d.nt & 1;
+    HIPCHK(hipMemcpyAsync(buf.data(), s.d.f[par][field], m.elems * sizeof(float),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < g.nx; ++i)
+        for (int j = 0; j < g.nz; ++j)
+            out[(size_t)j + (size_t)i * g.nz] =
+                scale * buf[(size_t)(j + MNAD_ZO) * m.pitch + (i + MNAD_XO)];
+    return MNAD_OK;
+}
+
+// Interior row-major float array -> column-major doubles, out = scale*(a - b).
+int download_interior(mnad_ctx* c, const mnad_grid& g, const float* a, const float* b,
+                      double scale, double* out)
+{
+    if (!out) return MNAD_OK;
+    const int nzi = g.nz - 2 * g.npml, nxi = g.nx - 2 * g.npml;
+    const size_t ni = (size_t)nzi * nxi;
+    std::vector<float> ha(ni), hb(b ? ni : 0);
+    HIPCHK(hipMemcpyAsync(ha.data(), a, ni * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (b) HIPCHK(hipMemcpyAsync(hb.data(), b, ni * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nxi; ++i)
+        for (int j = 0; j < nzi; ++j) {
+            const size_t q = (size_t)j * nxi + i;
+            out[(size_t)j + (size_t)i * nzi] = scale * ((double)ha[q] - (b ? (double)hb[q] : 0.0));
+        }
+    return MNAD_OK;
+}
+
+int schedule(int nt, double vmin, double vmax, int* nf, int* T, int* K, int* ncol)
+{
+    // fwd_PDE_exact_MNAD.m:46-47,53; Born_adj_MNAD.m:41-42.
+    const double nfd = std::floor((double)nt / (2.0 * 3.0 * vmax / vmin));
+    if (!(nfd >= 1.0))
+        return fail(MNAD_ERR_ARG,
+                    "nt = %d is too short for the snapshot schedule: nf = floor(nt/(6 vmax/vmin)) "
+                    "= 0 gives T = Inf and a NaN image in the reference",
+                    nt);
+    const int nf_ = (int)nfd;
+    const int T_ = (int)std::floor((double)nt / (double)nf_ / 2.0);
+    const int K_ = nt / T_;
+    if (nf) *nf = nf_;
+    if (T) *T = T_;
+    if (K) *K = K_;
+    if (ncol) *ncol = std::max(2 * nf_ + 1, K_);
+    return MNAD_OK;
+}
+
+}  // namespace mnad
