@@ -1,0 +1,156 @@
+"""Benchmark of the MNAD hot path on one MI355X.
+
+Workload (default, the configuration the headline metric is quoted on):
+single-shot FP32 modelling plus one-shot RTM on a 1001 x 501 grid
+(dx = dz = 10 m, 30 CPML cells -> 561 x 1061 padded), seeded layered model
+1500-4000 m/s with 5 % Gaussian-correlated perturbations (50 m), migration
+model = true model smoothed with a 100 m Gaussian, Ricker 20 Hz at the centre
+of the first physical row, 1001 receivers on the second, dt = 0.8 ms,
+nt = 4000.  One step = one shot: a forward on the true model (observed
+gather) and a forward on the migration model with d2u snapshots (both in one
+batched launch sequence), the scattered data, and the adjoint with the
+imaging condition.  Useful updates per step = 3 * N * nt (N padded points).
+
+Prints one JSON line (metric, value in Gpt-updates/s, roofline of the step
+kernel from HIP events on the library's stream, CPU baseline of the FP64
+oracle on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gpt-updates/s & RTM shots/hr vs plain FP32 at ≤1e-4 rel err; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_workload(nt):
+    from __graft_entry__ import load_package
+    load_package()
+    import importlib
+    synth = importlib.import_module("mnad_amd.synth")
+    npml, dx, dt, fc = 30, 10.0, 8e-4, 20.0
+    V, Vs = synth.model_c2(seed=2024)
+    v, vs = synth.pad_edges(V, npml), synth.pad_edges(Vs, npml)
+    w = synth.ricker(fc, dt, nt)
+    nxp = V.shape[1]
+    zs, xs = npml, npml + nxp // 2
+    xr = npml + np.arange(nxp)
+    zr = np.full(nxp, npml + 1)
+    return dict(v=v, vs=vs, w=w, npml=npml, dx=dx, dt=dt, zs=zs, xs=xs, zr=zr, xr=xr)
+
+
+def cpu_baseline(wl, steps):
+    """FP64 oracle (one thread) on the first `steps` forward steps of the
+    true-model shot of the same workload."""
+    from oracle import oracle as O
+    v = wl["v"]
+    g = O.grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    t0 = time.perf_counter()
+    O.forward(g, v, wl["w"][:steps], wl["zs"], wl["xs"], wl["zr"], wl["xr"], want_state=False)
+    el = time.perf_counter() - t0
+    return dict(value=v.size * steps / el / 1e9, unit="Gpt-updates/s", cores=1, kind="port",
+                sample=f"FP64 oracle forward, true model, first {steps} of {wl['w'].size} steps, "
+                       f"1 thread ({el:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--snap", default="f16", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by "
+              "shots with one image all-reduce, which this single-GPU build does not run.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from __graft_entry__ import load_package
+    mnad = load_package()
+    wl = c2_workload(args.nt)
+    v = wl["v"]
+    g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    fmt = mnad.SNAP_FORMATS[args.snap]
+    recs = [(wl["zr"], wl["xr"])]
+    N = v.shape[0] * v.shape[1]
+    nt = args.nt
+    print(f"# C2 {v.shape[0]}x{v.shape[1]} nt={nt} snap={args.snap}", file=sys.stderr, flush=True)
+
+    with mnad.Context(0) as ctx:
+        sid = ctx.stack_create(g)
+
+        def shot(flags=0):
+            ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], [wl["zs"]], [wl["xs"]], recs,
+                          snap_fmt=fmt, batch=1, flags=flags)
+
+        for k in range(args.warmup):
+            shot()
+            print(f"# warmup {k + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        ctx.timing(reset=True)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            shot()
+            print(f"# step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
+        el = time.perf_counter() - t0
+        k_ms, k_launches, k_bytes = ctx.timing(reset=True)
+        plain_ms = None
+        if not args.no_plain:
+            t1 = time.perf_counter()
+            shot(flags=mnad.FLAG_NO_GRAPH)
+            plain_ms = (time.perf_counter() - t1) * 1e3
+            print("# plain-launch shot done", file=sys.stderr, flush=True)
+        ctx.stack_destroy(sid)
+
+    ms_per_step = el / args.steps * 1e3
+    updates = 3.0 * N * nt * args.steps
+    value = updates / el / 1e9
+    avg_launch_us = k_ms / max(k_launches, 1) * 1e3
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gpt-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered + correlated-random model)",
+        "config": {"workload": "C2: 1001x501 (+30 CPML) dx=10 m, nt=4000, dt=0.8 ms, Ricker 20 Hz, "
+                               "1001 receivers; modelling + 1-shot RTM per step",
+                   "grid_padded": [int(v.shape[0]), int(v.shape[1])], "nt": nt,
+                   "snapshots": args.snap, "updates_per_step": 3 * N * nt},
+        "shots_per_hour": round(3600.0 / (ms_per_step / 1e3), 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_step", "avg_launch_us": round(avg_launch_us, 3),
+                     "launches": int(k_launches)},
+    }
+    if plain_ms is not None:
+        out["plain_ms_per_step"] = round(plain_ms, 3)
+        out["speedup_vs_plain"] = round(plain_ms / ms_per_step, 3)
+    if not args.no_cpu and args.cpu_steps > 0:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(wl, min(args.cpu_steps, nt))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
