@@ -149,6 +149,18 @@ MNAD_API int mnad_fwd_shots(mnad_ctx *ctx, const mnad_grid *g, const double *v, 
 MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches,
                          double *bytes);
 
+/* Launch-shape knobs of the step kernel (chosen by the autotuner):
+ * rows_per_thread 1, 2 or 4 (a tile is 64 x 4*rows_per_thread points),
+ * graph_chunk = step launches per captured hipGraph (<= 1: no graph),
+ * strip_first != 0 dispatches the CPML-strip tiles of each XCD first. */
+typedef struct {
+    int rows_per_thread;
+    int graph_chunk;
+    int strip_first;
+} mnad_tuning;
+MNAD_API int mnad_get_tuning(mnad_ctx *ctx, mnad_tuning *t);
+MNAD_API int mnad_set_tuning(mnad_ctx *ctx, const mnad_tuning *t);
+
 /* Device memory in use / cached by the context's pool, in bytes. */
 MNAD_API int mnad_mem_info(mnad_ctx *ctx, uint64_t *in_use, uint64_t *cached);
 

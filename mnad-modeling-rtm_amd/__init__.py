@@ -41,7 +41,7 @@ EXPORTS = (
     "mnad_ctx_destroy", "mnad_fwd", "mnad_born_adj", "mnad_release",
     "mnad_snap_schedule", "mnad_stack_create", "mnad_stack_destroy",
     "mnad_stack_image", "mnad_rtm_shots", "mnad_fwd_shots", "mnad_mem_info",
-    "mnad_timing",
+    "mnad_timing", "mnad_get_tuning", "mnad_set_tuning",
 )
 
 
@@ -54,6 +54,11 @@ class MnadError(RuntimeError):
 class Grid(C.Structure):
     _fields_ = [("nz", C.c_int), ("nx", C.c_int), ("npml", C.c_int),
                 ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("rows_per_thread", C.c_int), ("graph_chunk", C.c_int),
+                ("strip_first", C.c_int)]
 
 
 class Shots(C.Structure):
@@ -97,6 +102,8 @@ def lib():
             L.mnad_fwd_shots.argtypes = [vp, gp, dp, C.c_int, dp, C.POINTER(Shots), C.c_int,
                                          C.c_int, dp]
             L.mnad_mem_info.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+            L.mnad_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+            L.mnad_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
             L.mnad_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int64), C.POINTER(C.c_double)]
             for name in EXPORTS:
@@ -290,6 +297,19 @@ class Context:
         ms, n, b = C.c_double(), C.c_int64(), C.c_double()
         _check(lib().mnad_timing(self._h, int(bool(reset)), C.byref(ms), C.byref(n), C.byref(b)))
         return ms.value, n.value, b.value
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().mnad_get_tuning(self._h, C.byref(t)))
+        return dict(rows_per_thread=t.rows_per_thread, graph_chunk=t.graph_chunk,
+                    strip_first=t.strip_first)
+
+    def set_tuning(self, rows_per_thread=None, graph_chunk=None, strip_first=None):
+        cur = self.tuning()
+        t = Tuning(int(cur["rows_per_thread"] if rows_per_thread is None else rows_per_thread),
+                   int(cur["graph_chunk"] if graph_chunk is None else graph_chunk),
+                   int(cur["strip_first"] if strip_first is None else strip_first))
+        _check(lib().mnad_set_tuning(self._h, C.byref(t)))
 
     def mem_info(self):
         a, b = C.c_uint64(), C.c_uint64()

@@ -75,6 +75,11 @@ int mnad_ctx_create(int device, mnad_ctx** out)
         return fail(MNAD_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
     if (const char* s = getenv("MNAD_GRAPH_CHUNK")) c->graph_chunk = std::max(1, atoi(s));
+    if (const char* s = getenv("MNAD_RZ")) {
+        const int v = atoi(s);
+        if (v == 1 || v == 2 || v == 4) c->rz = v;
+    }
+    if (const char* s = getenv("MNAD_STRIP_FIRST")) c->strip_first = atoi(s) != 0;
     *out = c;
     return MNAD_OK;
 }
@@ -111,6 +116,28 @@ int mnad_timing(mnad_ctx* ctx, int reset, double* ms, int64_t* launches, double*
         ctx->t_launches = 0;
         ctx->t_bytes = 0.0;
     }
+    return MNAD_OK;
+}
+
+int mnad_get_tuning(mnad_ctx* ctx, mnad_tuning* t)
+{
+    ENTER(ctx);
+    if (!t) return fail(MNAD_ERR_ARG, "tuning is NULL");
+    t->rows_per_thread = ctx->rz;
+    t->graph_chunk = ctx->graph_chunk;
+    t->strip_first = ctx->strip_first;
+    return MNAD_OK;
+}
+
+int mnad_set_tuning(mnad_ctx* ctx, const mnad_tuning* t)
+{
+    ENTER(ctx);
+    if (!t) return fail(MNAD_ERR_ARG, "tuning is NULL");
+    if (t->rows_per_thread != 1 && t->rows_per_thread != 2 && t->rows_per_thread != 4)
+        return fail(MNAD_ERR_ARG, "rows_per_thread must be 1, 2 or 4 (got %d)", t->rows_per_thread);
+    ctx->rz = t->rows_per_thread;
+    ctx->graph_chunk = std::max(1, t->graph_chunk);
+    ctx->strip_first = t->strip_first != 0;
     return MNAD_OK;
 }
 

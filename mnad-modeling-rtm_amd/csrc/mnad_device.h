@@ -4,8 +4,8 @@
 // Grid storage (per field, per shot): rows of `pitch` floats; grid point
 // (j, i) (j = z row, i = x column, 0-based, padded grid incl. CPML) lives at
 //     (j + MNAD_ZO) * pitch + (i + MNAD_XO).
-// Two zero rows above/below and at least two zero columns left/right are
-// never written, so the +-2 stencil reads zeros beyond the grid, which is what
+// Two zero rows above, 2 + MNAD_ZPAD below and at least two zero columns
+// left/right are never written, so the +-2 stencil reads zeros beyond the grid, which is what
 // the *_gen.m index ranges do (neighbours beyond the padded grid omitted).
 #pragma once
 
@@ -13,6 +13,7 @@
 
 #define MNAD_ZO 2    // ghost rows above the grid
 #define MNAD_XO 32   // column offset of x = 0 (row starts 128-byte aligned)
+#define MNAD_ZPAD 8  // extra zero rows below the grid (rows a partial tile may read)
 
 namespace mnad {
 

@@ -79,6 +79,8 @@ struct mnad_ctx {
     std::map<int, mnad::Stack> stacks;
     int next_stack = 1;
     int graph_chunk = 64;
+    int rz = 1;                 // rows per thread of the step kernel (1, 2, 4)
+    int strip_first = 1;        // dispatch CPML-strip tiles first
     size_t budget = 0;          // bytes usable for batches (0 = from free memory)
     // step-kernel accounting (mnad_timing)
     double t_ms = 0.0;

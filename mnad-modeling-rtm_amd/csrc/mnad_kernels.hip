@@ -135,47 +135,15 @@ __device__ __forceinline__ void psi3(float* __restrict__ ps, const float* __rest
     ps[6 * ns + q] = p1; ps[7 * ns + q] = p2; ps[8 * ns + q] = p3;
 }
 
-// One step for every tile in `tiles`.  Block = BX x BZ threads = one tile.
-// Local step of a shot: s = *base + rel - t0 (0-based); tiles of shots outside
-// [0, nt) exit at once, so a graph chunk may run past a shot's end.
-__global__ __launch_bounds__(BX * BZ) void k_step(const ShotDev* __restrict__ shots,
-                                                  const TileRef* __restrict__ tiles,
-                                                  const int* __restrict__ base, int rel)
+// Everything after the increments for one point: CPML memories on the
+// strips, the new level, injections, gathers, snapshots and imaging.
+__device__ __forceinline__ void finish_point(const ShotDev& d, int s, int j, int i, long p,
+                                             const Nbh& u, const Nbh& v, const Nbh& w,
+                                             float du, float dv, float dw, float rx, float rz,
+                                             float* __restrict__ PU, float* __restrict__ PV,
+                                             float* __restrict__ PW)
 {
-    const TileRef tr = tiles[blockIdx.x];
-    if (tr.shot < 0) return;
-    const ShotDev& d = shots[tr.shot];
-    const int s = *base + rel - d.t0;
-    if (s < 0 || s >= d.nt) return;
-    const int tj = tr.tile / d.ntx;
-    const int ti = tr.tile - tj * d.ntx;
-    const int i = ti * BX + threadIdx.x;
-    const int j = tj * BZ + threadIdx.y;
-    if (i >= d.nx || j >= d.nz) return;
-
-    const int cb = s & 1;
-    const float* __restrict__ U = d.f[cb][0];
-    const float* __restrict__ V = d.f[cb][1];
-    const float* __restrict__ W = d.f[cb][2];
-    float* __restrict__ PU = d.f[cb ^ 1][0];
-    float* __restrict__ PV = d.f[cb ^ 1][1];
-    float* __restrict__ PW = d.f[cb ^ 1][2];
-    const int P = d.pitch;
-    const long p = (long)(j + MNAD_ZO) * P + (i + MNAD_XO);
-
-    Nbh u, v, w;
-    load9(U, p, P, u);
-    load9(V, p, P, v);
-    load9(W, p, P, w);
-    const float vww = V[p - 2], vee = V[p + 2];
-    const float wnn = W[p - 2 * P], wss = W[p + 2 * P];
-    const float rx = d.rx[p];
-    const float rz = rx * d.kap;
     const float pu = PU[p], pv = PV[p], pw = PW[p];
-
-    float du, dv, dw;
-    mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
-
     const int np = d.npml;
     const bool inx = (i < np) || (i >= d.nx - np);
     const bool inz = (j < np) || (j >= d.nz - np);
@@ -254,6 +222,80 @@ __global__ __launch_bounds__(BX * BZ) void k_step(const ShotDev* __restrict__ sh
                 }
             }
         }
+    }
+}
+
+// One step for every tile in `tiles`.  Block = BX x BZ threads; each thread
+// updates RZ consecutive rows of one column, so a tile is BX x (BZ*RZ)
+// points and the column values shared by those rows are loaded once.
+// Local step of a shot: s = *base + rel - t0 (0-based); tiles of shots outside
+// [0, nt) exit at once, so a graph chunk may run past a shot's end.
+// Storage keeps MNAD_ZO + MNAD_ZPAD zero rows below the grid, so the loads of
+// a partial bottom tile stay inside the allocation (RZ <= MNAD_ZPAD).
+template <int RZ>
+__global__ __launch_bounds__(BX * BZ) void k_step(const ShotDev* __restrict__ shots,
+                                                  const TileRef* __restrict__ tiles,
+                                                  const int* __restrict__ base, int rel)
+{
+    static_assert(RZ >= 1 && RZ <= MNAD_ZPAD, "rows per thread");
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int i = ti * BX + threadIdx.x;
+    const int j0 = (tj * BZ + threadIdx.y) * RZ;
+    if (i >= d.nx || j0 >= d.nz) return;
+
+    const int cb = s & 1;
+    const float* __restrict__ U = d.f[cb][0];
+    const float* __restrict__ V = d.f[cb][1];
+    const float* __restrict__ W = d.f[cb][2];
+    float* __restrict__ PU = d.f[cb ^ 1][0];
+    float* __restrict__ PV = d.f[cb ^ 1][1];
+    float* __restrict__ PW = d.f[cb ^ 1][2];
+    const int P = d.pitch;
+    const long p0 = (long)(j0 + MNAD_ZO) * P + (i + MNAD_XO);
+
+    // Column windows: rows j0-1 .. j0+RZ (U, V, W sides), j0-2 .. j0+RZ+1 (W centre).
+    float ua[RZ + 2][3], va[RZ + 2][3], vf[RZ][2], wc[RZ + 4], ws[RZ + 2][2];
+#pragma unroll
+    for (int k = 0; k < RZ + 2; ++k) {
+        const long q = p0 + (long)(k - 1) * P;
+        ua[k][0] = U[q - 1]; ua[k][1] = U[q]; ua[k][2] = U[q + 1];
+        va[k][0] = V[q - 1]; va[k][1] = V[q]; va[k][2] = V[q + 1];
+        ws[k][0] = W[q - 1]; ws[k][1] = W[q + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < RZ + 4; ++k) wc[k] = W[p0 + (long)(k - 2) * P];
+#pragma unroll
+    for (int k = 0; k < RZ; ++k) {
+        vf[k][0] = V[p0 + (long)k * P - 2];
+        vf[k][1] = V[p0 + (long)k * P + 2];
+    }
+
+#pragma unroll
+    for (int r = 0; r < RZ; ++r) {
+        const int j = j0 + r;
+        if (j >= d.nz) break;
+        const long p = p0 + (long)r * P;
+        Nbh u, v, w;
+        u.nw = ua[r][0];     u.n = ua[r][1];     u.ne = ua[r][2];
+        u.w = ua[r + 1][0];  u.c = ua[r + 1][1]; u.e = ua[r + 1][2];
+        u.sw = ua[r + 2][0]; u.s = ua[r + 2][1]; u.se = ua[r + 2][2];
+        v.nw = va[r][0];     v.n = va[r][1];     v.ne = va[r][2];
+        v.w = va[r + 1][0];  v.c = va[r + 1][1]; v.e = va[r + 1][2];
+        v.sw = va[r + 2][0]; v.s = va[r + 2][1]; v.se = va[r + 2][2];
+        w.nw = ws[r][0];     w.n = wc[r + 1];    w.ne = ws[r][1];
+        w.w = ws[r + 1][0];  w.c = wc[r + 2];    w.e = ws[r + 1][1];
+        w.sw = ws[r + 2][0]; w.s = wc[r + 3];    w.se = ws[r + 2][1];
+        const float rx = d.rx[p];
+        const float rz = rx * d.kap;
+        float du, dv, dw;
+        mnad_incr(rx, rz, u, v, vf[r][0], vf[r][1], w, wc[r], wc[r + 4], du, dv, dw);
+        finish_point(d, s, j, i, p, u, v, w, du, dv, dw, rx, rz, PU, PV, PW);
     }
 }
 
@@ -336,9 +378,15 @@ hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, i
 }
 
 hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
-                            const int* base, int rel, hipStream_t st)
+                            const int* base, int rel, int rz, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_step, dim3(ntiles), dim3(BX, BZ), 0, st, shots, tiles, base, rel);
+    const dim3 g(ntiles), b(BX, BZ);
+    switch (rz) {
+    case 1: hipLaunchKernelGGL(k_step<1>, g, b, 0, st, shots, tiles, base, rel); break;
+    case 2: hipLaunchKernelGGL(k_step<2>, g, b, 0, st, shots, tiles, base, rel); break;
+    case 4: hipLaunchKernelGGL(k_step<4>, g, b, 0, st, shots, tiles, base, rel); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -32,7 +32,7 @@ hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, i
                                    double dx, double dz, double dt, float* rx, float* cfx,
                                    float* cfz, hipStream_t st);
 hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
-                            const int* base, int rel, hipStream_t st);
+                            const int* base, int rel, int rz, hipStream_t st);
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st);
 hipError_t mnad_launch_init_source(ShotDev* shots, int nshots, hipStream_t st);
 hipError_t mnad_launch_make_radj(const float* dobs, const float* dsyn, const float* arec,
@@ -202,7 +202,7 @@ int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m)
     m.vmax = vmax;
     m.vhost = v;
     m.pitch = ((MNAD_XO + g->nx + 2) + 31) / 32 * 32;
-    m.rows = g->nz + 2 * MNAD_ZO;
+    m.rows = g->nz + 2 * MNAD_ZO + MNAD_ZPAD;
     m.elems = (size_t)m.rows * m.pitch;
     DevBuf dv;
     CHK(upload(c, dv, v, n * sizeof(double)));
@@ -402,23 +402,35 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     std::vector<ShotDev> h(n);
     int nsteps = 0;
     std::vector<TileRef> list;
+    std::vector<char> strip;
+    const int rz = c->rz, th = 4 * rz;   // tile = 64 columns x (4 * rz) rows
     for (int k = 0; k < n; ++k) {
         h[k] = shots[k]->d;
         h[k].t0 = 0;
         nsteps = std::max(nsteps, h[k].nt);
-        const int ntx = (h[k].nx + 63) / 64, ntz = (h[k].nz + 3) / 4;
+        const int ntx = (h[k].nx + 63) / 64, ntz = (h[k].nz + th - 1) / th;
+        const int np = h[k].npml;
         h[k].ntx = ntx;
-        for (int t = 0; t < ntx * ntz; ++t) list.push_back(TileRef{k, t});
+        for (int t = 0; t < ntx * ntz; ++t) {
+            list.push_back(TileRef{k, t});
+            const int tj = t / ntx, ti = t - tj * ntx;
+            strip.push_back(ti * 64 < np || ti * 64 + 63 >= h[k].nx - np || tj * th < np ||
+                            (tj + 1) * th - 1 >= h[k].nz - np);
+        }
     }
     // XCD-aware order: workgroup b is dispatched to XCD b % 8, so XCD x gets
     // the contiguous run [x*L/8, (x+1)*L/8) of the spatial tile list and
-    // vertically adjacent tiles share halo rows in the same L2.
+    // vertically adjacent tiles share halo rows in the same L2.  Within each
+    // run the CPML-strip tiles (about twice the memory traffic) go first.
     const int nx8 = 8;
     const size_t L = (list.size() + nx8 - 1) / nx8 * nx8, per = L / nx8;
     std::vector<TileRef> table(L, TileRef{-1, 0});
-    for (size_t b = 0; b < L; ++b) {
-        const size_t src = (b % nx8) * per + b / nx8;
-        if (src < list.size()) table[b] = list[src];
+    for (int x = 0; x < nx8; ++x) {
+        std::vector<size_t> run;
+        for (size_t q = x * per; q < std::min(list.size(), (x + 1) * per); ++q) run.push_back(q);
+        if (c->strip_first)
+            std::stable_partition(run.begin(), run.end(), [&](size_t q) { return strip[q] != 0; });
+        for (size_t m = 0; m < run.size(); ++m) table[m * nx8 + x] = list[run[m]];
     }
     DevBuf dsh, dtab, dbase;
     CHK(upload(c, dsh, h.data(), h.size() * sizeof(ShotDev)));
@@ -450,14 +462,14 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     int* base = dbase.as<int>();
     int chunk = c->graph_chunk;
     if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nsteps < 2 * chunk) {
-        for (int s = 0; s < nsteps; ++s) HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s, c->stream));
+        for (int s = 0; s < nsteps; ++s) HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s, rz, c->stream));
     } else {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t ce = hipSuccess;
         for (int r = 0; r < chunk && ce == hipSuccess; ++r)
-            ce = mnad_launch_step(ds, dt, ntiles, base, r, c->stream);
+            ce = mnad_launch_step(ds, dt, ntiles, base, r, rz, c->stream);
         if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk, c->stream);
         hipError_t ee = hipStreamEndCapture(c->stream, &graph);
         HIPCHK(ce);
@@ -471,7 +483,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         hipError_t le = hipSuccess;
         for (int k = 0; k < full && le == hipSuccess; ++k) le = hipGraphLaunch(exec, c->stream);
         for (int r = 0; r < rem && le == hipSuccess; ++r)
-            le = mnad_launch_step(ds, dt, ntiles, base, r, c->stream);
+            le = mnad_launch_step(ds, dt, ntiles, base, r, rz, c->stream);
         hipError_t se = hipStreamSynchronize(c->stream);
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);

@@ -179,3 +179,23 @@ def test_invalid_inputs_raise(mnad, ctx):
     with pytest.raises(mnad.MnadError):
         mnad.fwd_PDE_exact_MNAD(10.0, 3.5, [2], 10.0, 3, [2], v, 1e-3, f["w"].size, f["w"],
                                 int(f["npml"]), 1)           # non-integer index
+
+
+@pytest.mark.parametrize("rz", [1, 2, 4])
+@pytest.mark.parametrize("strip_first", [0, 1])
+def test_launch_shapes_agree(mnad, ctx, oracle, rz, strip_first):
+    """Every rows-per-thread / dispatch-order variant of the step kernel meets
+    the oracle and agrees with the default shape to rounding."""
+    _say(f"launch shape rz={rz} strip_first={strip_first}")
+    f = golden("fwd_rect")
+    g = _grid(mnad, f)
+    base = ctx.fwd(g, f["v"], f["w"], int(f["zs"]), int(f["xs"]), f["zr"], f["xr"])
+    old = ctx.tuning()
+    try:
+        ctx.set_tuning(rows_per_thread=rz, strip_first=strip_first)
+        r = ctx.fwd(g, f["v"], f["w"], int(f["zs"]), int(f["xs"]), f["zr"], f["xr"])
+    finally:
+        ctx.set_tuning(**old)
+    for k in ("gather", "U", "V", "W"):
+        assert rel_l2(r[k], f[k]) < GATHER_TOL, k
+        assert rel_l2(r[k], base[k]) < 1e-6, k
