@@ -48,6 +48,97 @@ def c2_workload(nt):
     return dict(v=v, vs=vs, w=w, npml=npml, dx=dx, dt=dt, zs=zs, xs=xs, zr=zr, xr=xr)
 
 
+def c3_workload(nt, nshots):
+    """C3: 3401 x 751 (+30 CPML), dx = 10 m, dipping layers, three normal
+    faults, a 4500 m/s body and a 200 m water layer (seeded); migration
+    model = 100 m Gaussian smoothing; `nshots` sources evenly spaced on the
+    first physical row, split-spread receivers +-200 cells (2 km) every cell
+    on the second row, clipped at the model edges; Ricker 25 Hz; dt = 0.8 ms."""
+    from __graft_entry__ import load_package
+    load_package()
+    import importlib
+    synth = importlib.import_module("mnad_amd.synth")
+    npml, dx, dt, fc = 30, 10.0, 8e-4, 25.0
+    V, Vs = synth.model_c3(seed=3)
+    v, vs = synth.pad_edges(V, npml), synth.pad_edges(Vs, npml)
+    w = synth.ricker(fc, dt, nt)
+    nxp = V.shape[1]
+    xs_phys = np.rint(np.linspace(0, nxp - 1, nshots)).astype(int)
+    recs = []
+    for x in xs_phys:
+        xr = np.arange(max(0, x - 200), min(nxp, x + 201))
+        recs.append((np.full(xr.size, npml + 1), npml + xr))
+    return dict(v=v, vs=vs, w=w, npml=npml, dx=dx, dt=dt, zs=[npml] * nshots,
+                xs=list(npml + xs_phys), recs=recs)
+
+
+def c2_multi_workload(nt, nshots):
+    """C2 model with `nshots` sources evenly spaced on the first physical row,
+    each with the full 1001-receiver line."""
+    wl = c2_workload(nt)
+    npml, nxp = wl["npml"], wl["v"].shape[1] - 2 * wl["npml"]
+    xs_phys = np.rint(np.linspace(50, nxp - 51, nshots)).astype(int)
+    wl.update(zs=[npml] * nshots, xs=list(npml + xs_phys),
+              recs=[(wl["zr"], wl["xr"])] * nshots)
+    return wl
+
+
+def run_multi(args, mnad, wl, label):
+    """Many-shot RTM: the batched form (autotuned kernel variant and shots per
+    launch, hipGraph chunks) against the plain form (one shot and one
+    propagation per launch sequence, plain launches) on the same workload."""
+    v = wl["v"]
+    g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    fmt = mnad.SNAP_FORMATS[args.snap]
+    N = v.shape[0] * v.shape[1]
+    nshots = len(wl["xs"])
+    plain_flags = mnad.FLAG_SERIAL | mnad.FLAG_NO_GRAPH
+    with mnad.Context(0) as ctx:
+        t0 = time.perf_counter()
+        tuning, per_launch, us = ctx.autotune(g, v, max_shots=32)
+        tune_s = time.perf_counter() - t0
+        print(f"# autotune {tune_s:.2f} s: {tuning} shots/launch {per_launch} ({us:.2f} us/shot-step)",
+              file=sys.stderr, flush=True)
+        sid = ctx.stack_create(g)
+        t0 = time.perf_counter()
+        for k in range(args.plain_shots):
+            ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], [wl["zs"][k]], [wl["xs"][k]],
+                          [wl["recs"][k]], snap_fmt=fmt, batch=1, flags=plain_flags)
+        plain_s = (time.perf_counter() - t0) / max(args.plain_shots, 1)
+        print(f"# plain path: {plain_s:.3f} s/shot", file=sys.stderr, flush=True)
+        ctx.stack_destroy(sid)
+        sid = ctx.stack_create(g)
+        ctx.timing(reset=True)
+        t0 = time.perf_counter()
+        ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], wl["zs"], wl["xs"], wl["recs"], snap_fmt=fmt,
+                      batch=args.batch)
+        el = time.perf_counter() - t0
+        k_ms, k_launches, k_bytes = ctx.timing(reset=True)
+        img, ill, ns = ctx.stack_image(sid, g)
+        ctx.stack_destroy(sid)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    nt = wl["w"].size
+    out = {
+        "metric": METRIC, "value": round(3.0 * N * nt * nshots / el / 1e9, 3),
+        "unit": "Gpt-updates/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(el * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded model)",
+        "config": {"workload": label, "grid_padded": [int(v.shape[0]), int(v.shape[1])],
+                   "shots": nshots, "nt": nt, "snapshots": args.snap,
+                   "tuning": tuning, "shots_per_launch": per_launch},
+        "shots_per_hour": round(3600.0 * nshots / el, 1),
+        "plain_shots_per_hour": round(3600.0 / plain_s, 1),
+        "speedup_vs_plain": round(plain_s * nshots / el, 3),
+        "autotune_s": round(tune_s, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_step", "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 3),
+                     "launches": int(k_launches)},
+        "image_finite": bool(np.isfinite(img).all()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(wl, steps):
     """FP64 oracle (one thread) on the first `steps` forward steps of the
     true-model shot of the same workload."""
@@ -65,6 +156,12 @@ def cpu_baseline(wl, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--shots", type=int, default=128, help="C3: number of shots")
+    ap.add_argument("--batch", type=int, default=0, help="C3: shots per launch sequence (0 = by memory)")
+    ap.add_argument("--plain-shots", type=int, default=2, help="many-shot runs: shots timed on the plain path")
+    ap.add_argument("--shots-c2", type=int, default=1,
+                    help="C2 grid with this many shots, batched vs plain (default: the 1-shot bench)")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=4000)
@@ -94,6 +191,15 @@ def main():
 
     from __graft_entry__ import load_package
     mnad = load_package()
+    if args.workload == "c3":
+        nt = 6000 if args.nt == 4000 else args.nt
+        run_multi(args, mnad, c3_workload(nt, args.shots),
+                  f"C3: 3401x751 (+30 CPML), {args.shots} shots batched RTM, nt={nt}")
+        return
+    if args.shots_c2 > 1:
+        run_multi(args, mnad, c2_multi_workload(args.nt, args.shots_c2),
+                  f"C2 grid: 1001x501 (+30 CPML), {args.shots_c2} shots batched RTM, nt={args.nt}")
+        return
     wl = c2_workload(args.nt)
     v = wl["v"]
     g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
@@ -123,9 +229,10 @@ def main():
         plain_ms = None
         if not args.no_plain:
             t1 = time.perf_counter()
-            shot(flags=mnad.FLAG_NO_GRAPH)
+            shot(flags=mnad.FLAG_SERIAL | mnad.FLAG_NO_GRAPH)
             plain_ms = (time.perf_counter() - t1) * 1e3
-            print("# plain-launch shot done", file=sys.stderr, flush=True)
+            print("# plain shot (one propagation per launch sequence, no graph) done",
+                  file=sys.stderr, flush=True)
         ctx.stack_destroy(sid)
 
     ms_per_step = el / args.steps * 1e3

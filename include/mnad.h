@@ -53,6 +53,8 @@ extern "C" {
 
 /* Flags */
 #define MNAD_FLAG_NO_GRAPH 1   /* plain launches, one step per launch */
+#define MNAD_FLAG_SERIAL 2     /* one shot and one propagation per launch sequence */
+#define MNAD_FLAG_AUTOTUNE 4   /* batched calls: autotune an unseen grid shape first */
 
 typedef struct {
     int nz, nx;       /* padded grid (rows = z, columns = x) */
@@ -150,16 +152,31 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
                          double *bytes);
 
 /* Launch-shape knobs of the step kernel (chosen by the autotuner):
- * rows_per_thread 1, 2 or 4 (a tile is 64 x 4*rows_per_thread points),
- * graph_chunk = step launches per captured hipGraph (<= 1: no graph),
+ * a tile is 64 x (block_rows * rows_per_thread) points, one block per tile;
+ * kernel 0 = neighbourhoods loaded through the vector cache: block_rows 4
+ *   with rows_per_thread 1, 2 or 4, or block_rows 8 or 16 with 1 row;
+ * kernel 1 = block_rows 4, 1 row, every per-point input issued first;
+ * kernel 2 = block_rows 4, tiles + halos staged in LDS, rows_per_thread 1 or 2;
+ * graph_chunk = step launches per captured hipGraph (<= 1: no graph);
  * strip_first != 0 dispatches the CPML-strip tiles of each XCD first. */
 typedef struct {
+    int kernel;
     int rows_per_thread;
+    int block_rows;
     int graph_chunk;
     int strip_first;
 } mnad_tuning;
 MNAD_API int mnad_get_tuning(mnad_ctx *ctx, mnad_tuning *t);
 MNAD_API int mnad_set_tuning(mnad_ctx *ctx, const mnad_tuning *t);
+
+/* Autotuner: times the step-kernel variants and 1, 2, 4, ... up to max_shots
+ * shots per launch with short forward probes on this grid and velocity, keeps
+ * the fastest per (nz, nx, npml) in the context and returns it.  Batched calls
+ * with batch = 0 then use the tuned variant and shots per launch (an RTM
+ * group of B shots puts 2B forward shots in one launch). */
+MNAD_API int mnad_autotune(mnad_ctx *ctx, const mnad_grid *g, const double *v,
+                           int max_shots, mnad_tuning *best, int *best_shots,
+                           double *us_per_shot_step);
 
 /* Device memory in use / cached by the context's pool, in bytes. */
 MNAD_API int mnad_mem_info(mnad_ctx *ctx, uint64_t *in_use, uint64_t *cached);

@@ -33,6 +33,8 @@ ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE = -1, -2, -3, -4
 ERR_CANCELLED, ERR_BUDGET, ERR_NOGPU, ERR_TIMEOUT = -5, -6, -7, -8
 SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
 FLAG_NO_GRAPH = 1
+FLAG_SERIAL = 2
+FLAG_AUTOTUNE = 4
 SNAP_FORMATS = {"f32": SNAP_F32, "f16": SNAP_F16, "bf16": SNAP_BF16}
 
 # Every symbol include/mnad.h declares (checked by the CPU test suite).
@@ -41,7 +43,7 @@ EXPORTS = (
     "mnad_ctx_destroy", "mnad_fwd", "mnad_born_adj", "mnad_release",
     "mnad_snap_schedule", "mnad_stack_create", "mnad_stack_destroy",
     "mnad_stack_image", "mnad_rtm_shots", "mnad_fwd_shots", "mnad_mem_info",
-    "mnad_timing", "mnad_get_tuning", "mnad_set_tuning",
+    "mnad_timing", "mnad_get_tuning", "mnad_set_tuning", "mnad_autotune",
 )
 
 
@@ -57,8 +59,8 @@ class Grid(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("rows_per_thread", C.c_int), ("graph_chunk", C.c_int),
-                ("strip_first", C.c_int)]
+    _fields_ = [("kernel", C.c_int), ("rows_per_thread", C.c_int), ("block_rows", C.c_int),
+                ("graph_chunk", C.c_int), ("strip_first", C.c_int)]
 
 
 class Shots(C.Structure):
@@ -102,6 +104,8 @@ def lib():
             L.mnad_fwd_shots.argtypes = [vp, gp, dp, C.c_int, dp, C.POINTER(Shots), C.c_int,
                                          C.c_int, dp]
             L.mnad_mem_info.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+            L.mnad_autotune.argtypes = [vp, gp, dp, C.c_int, C.POINTER(Tuning),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_double)]
             L.mnad_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
             L.mnad_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
             L.mnad_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
@@ -301,15 +305,29 @@ class Context:
     def tuning(self):
         t = Tuning()
         _check(lib().mnad_get_tuning(self._h, C.byref(t)))
-        return dict(rows_per_thread=t.rows_per_thread, graph_chunk=t.graph_chunk,
+        return dict(kernel=t.kernel, rows_per_thread=t.rows_per_thread,
+                    block_rows=t.block_rows, graph_chunk=t.graph_chunk,
                     strip_first=t.strip_first)
 
-    def set_tuning(self, rows_per_thread=None, graph_chunk=None, strip_first=None):
+    def set_tuning(self, **kw):
         cur = self.tuning()
-        t = Tuning(int(cur["rows_per_thread"] if rows_per_thread is None else rows_per_thread),
-                   int(cur["graph_chunk"] if graph_chunk is None else graph_chunk),
-                   int(cur["strip_first"] if strip_first is None else strip_first))
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise MnadError(ERR_ARG, f"unknown tuning keys {sorted(unknown)}")
+        cur.update({k: int(v) for k, v in kw.items()})
+        t = Tuning(cur["kernel"], cur["rows_per_thread"], cur["block_rows"], cur["graph_chunk"],
+                   cur["strip_first"])
         _check(lib().mnad_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, grid: Grid, v, max_shots=32):
+        """Time kernel variants x shots per launch on this grid; returns
+        (tuning dict, shots per launch, microseconds per shot-step)."""
+        v = _f64(v)
+        t, b, us = Tuning(), C.c_int(), C.c_double()
+        _check(lib().mnad_autotune(self._h, C.byref(grid), _dp(v), int(max_shots), C.byref(t),
+                                   C.byref(b), C.byref(us)))
+        return (dict(kernel=t.kernel, rows_per_thread=t.rows_per_thread, block_rows=t.block_rows,
+                     graph_chunk=t.graph_chunk, strip_first=t.strip_first), b.value, us.value)
 
     def mem_info(self):
         a, b = C.c_uint64(), C.c_uint64()

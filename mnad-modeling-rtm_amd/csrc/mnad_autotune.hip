@@ -1,0 +1,143 @@
+// mnad_autotune.hip -- picks the step-kernel variant and the number of shots
+// per launch for one grid shape by timing short forward probes on the device.
+//
+// Why measure: on MI355X the best batch depends on whether a launch's
+// working set (about 40 B per point per shot and step) fits the 256 MB
+// Infinity Cache; a single 3401 x 751 shot (130 MB per step) already runs
+// from it, while 595 k-point shots need several per launch to fill the
+// 256 CUs.  The kernel variant (rows per thread, LDS staging) trades
+// parallelism against load instructions the same way.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mnad.h"
+#include "mnad_internal.h"
+
+namespace mnad {
+
+namespace {
+
+struct Variant {
+    int kernel, rz, brows;
+};
+
+const Variant kVariants[] = {{0, 1, 4}, {0, 2, 4}, {0, 4, 4}, {2, 1, 4}};
+
+// Time `steps` steps of `b` forward shots (source at the centre, no
+// receivers, no snapshots) with the context's current tuning; returns
+// microseconds per shot-step of step-kernel time.
+int probe(mnad_ctx* c, const Model& m, int b, int steps, double* us)
+{
+    std::vector<double> w(steps, 0.0);
+    for (int t = 0; t < steps; ++t) w[t] = std::sin(0.3 * t) * std::exp(-0.01 * t);
+    std::vector<Shot> shots(b);
+    std::vector<Shot*> run;
+    for (int k = 0; k < b; ++k) {
+        ShotSpec sp;
+        sp.mode = MODE_FWD;
+        sp.nt = steps;
+        sp.w = w.data();
+        sp.zs = m.g.nz / 2;
+        sp.xs = m.g.nx / 2;
+        int rc = make_shot(c, m, sp, shots[k]);
+        if (rc != MNAD_OK) return rc;
+        run.push_back(&shots[k]);
+    }
+    const double ms0 = c->t_ms;
+    const long long n0 = c->t_launches;
+    const double b0 = c->t_bytes;
+    int rc = run_batch(c, run, 0);
+    // probes do not count as work in the context's accounting
+    const double ms = c->t_ms - ms0;
+    c->t_ms = ms0;
+    c->t_launches = n0;
+    c->t_bytes = b0;
+    if (rc != MNAD_OK) return rc;
+    *us = ms * 1e3 / ((double)b * steps);
+    return MNAD_OK;
+}
+
+}  // namespace
+
+int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
+{
+    const mnad_grid& g = m.g;
+    // probe length: enough launches to amortise the graph chunking, fewer on big grids
+    const double pts = (double)g.nz * g.nx;
+    const int steps = pts > 4e6 ? 2 * c->graph_chunk : 4 * c->graph_chunk;
+    // shots per launch: powers of two up to max_shots, bounded by memory
+    const size_t per_shot = 6 * m.elems * 4 + 9 * ((size_t)g.nz * 2 * g.npml + (size_t)2 * g.npml * g.nx) * 4;
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+        (void)hipGetLastError();
+        freeb = per_shot;
+    }
+    const int mem_cap = (int)std::max<size_t>(1, (size_t)(0.5 * (double)(freeb + c->pool.cached)) / per_shot);
+    max_shots = std::max(1, std::min(max_shots, mem_cap));
+
+    const int sk = c->kernel, sr = c->rz, sb = c->brows;
+    TunePick best{0, 1, 4, 1, 1e30};
+    for (int b = 1; b <= max_shots; b *= 2) {
+        double best_b = 1e30;
+        for (const Variant& v : kVariants) {
+            c->kernel = v.kernel;
+            c->rz = v.rz;
+            c->brows = v.brows;
+            double us = 0;
+            int rc = probe(c, m, b, steps, &us);   // warm-up: first touch, code load
+            if (rc == MNAD_OK) rc = probe(c, m, b, steps, &us);
+            if (rc != MNAD_OK) {
+                c->kernel = sk;
+                c->rz = sr;
+                c->brows = sb;
+                return rc;
+            }
+            best_b = std::min(best_b, us);
+            if (us < best.us) best = TunePick{v.kernel, v.rz, v.brows, b, us};
+        }
+        // stop growing the batch once it no longer helps (working set past the cache)
+        if (b > 1 && best_b > 1.05 * best.us) break;
+    }
+    c->kernel = sk;
+    c->rz = sr;
+    c->brows = sb;
+    *out = best;
+    c->tuned[std::make_tuple(g.nz, g.nx, g.npml)] = best;
+    return MNAD_OK;
+}
+
+const TunePick* find_tuned(mnad_ctx* c, const mnad_grid& g)
+{
+    auto it = c->tuned.find(std::make_tuple(g.nz, g.nx, g.npml));
+    return it == c->tuned.end() ? nullptr : &it->second;
+}
+
+}  // namespace mnad
+
+extern "C" int mnad_autotune(mnad_ctx* ctx, const mnad_grid* g, const double* v, int max_shots,
+                             mnad_tuning* best, int* best_shots, double* us_per_shot_step)
+{
+    using namespace mnad;
+    if (!ctx) return fail(MNAD_ERR_ARG, "context is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(MNAD_ERR_HIP, "hipSetDevice failed");
+    Model m;
+    int rc = make_model(ctx, g, v, m);
+    if (rc != MNAD_OK) return rc;
+    TunePick p;
+    rc = autotune(ctx, m, std::max(1, max_shots), &p);
+    if (rc != MNAD_OK) return rc;
+    if (best) {
+        best->kernel = p.kernel;
+        best->rows_per_thread = p.rz;
+        best->block_rows = p.brows;
+        best->graph_chunk = ctx->graph_chunk;
+        best->strip_first = ctx->strip_first;
+    }
+    if (best_shots) *best_shots = p.batch;
+    if (us_per_shot_step) *us_per_shot_step = p.us;
+    return MNAD_OK;
+}

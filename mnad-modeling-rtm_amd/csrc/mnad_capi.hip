@@ -42,6 +42,15 @@ extern "C" hipError_t mnad_launch_stack(const float* img, const float* imgc, con
     std::lock_guard<std::recursive_mutex> lock_((ctx)->mu);          \
     HIPCHK(hipSetDevice((ctx)->device))
 
+static bool tuning_ok(int kernel, int rows, int brows)
+{
+    if (kernel == 0 && brows == 4) return rows == 1 || rows == 2 || rows == 4;
+    if (kernel == 0 && (brows == 8 || brows == 16)) return rows == 1;
+    if (kernel == 1) return brows == 4 && rows == 1;
+    if (kernel == 2) return brows == 4 && (rows == 1 || rows == 2);
+    return false;
+}
+
 extern "C" {
 
 const char* mnad_version(void) { return "libmnad 0.1.0 (gfx950)"; }
@@ -75,11 +84,16 @@ int mnad_ctx_create(int device, mnad_ctx** out)
         return fail(MNAD_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
     if (const char* s = getenv("MNAD_GRAPH_CHUNK")) c->graph_chunk = std::max(1, atoi(s));
-    if (const char* s = getenv("MNAD_RZ")) {
-        const int v = atoi(s);
-        if (v == 1 || v == 2 || v == 4) c->rz = v;
+    if (const char* s = getenv("MNAD_KERNEL")) c->kernel = atoi(s);
+    if (const char* s = getenv("MNAD_RZ")) c->rz = atoi(s);
+    if (const char* s = getenv("MNAD_BROWS")) c->brows = atoi(s);
+    if (!tuning_ok(c->kernel, c->rz, c->brows)) {
+        c->kernel = 0;
+        c->rz = 1;
+        c->brows = 4;
     }
     if (const char* s = getenv("MNAD_STRIP_FIRST")) c->strip_first = atoi(s) != 0;
+    if (const char* s = getenv("MNAD_AUTOTUNE")) c->autotune_on_use = atoi(s) != 0;
     *out = c;
     return MNAD_OK;
 }
@@ -123,7 +137,9 @@ int mnad_get_tuning(mnad_ctx* ctx, mnad_tuning* t)
 {
     ENTER(ctx);
     if (!t) return fail(MNAD_ERR_ARG, "tuning is NULL");
+    t->kernel = ctx->kernel;
     t->rows_per_thread = ctx->rz;
+    t->block_rows = ctx->brows;
     t->graph_chunk = ctx->graph_chunk;
     t->strip_first = ctx->strip_first;
     return MNAD_OK;
@@ -133,9 +149,13 @@ int mnad_set_tuning(mnad_ctx* ctx, const mnad_tuning* t)
 {
     ENTER(ctx);
     if (!t) return fail(MNAD_ERR_ARG, "tuning is NULL");
-    if (t->rows_per_thread != 1 && t->rows_per_thread != 2 && t->rows_per_thread != 4)
-        return fail(MNAD_ERR_ARG, "rows_per_thread must be 1, 2 or 4 (got %d)", t->rows_per_thread);
+    if (!tuning_ok(t->kernel, t->rows_per_thread, t->block_rows))
+        return fail(MNAD_ERR_ARG,
+                    "unsupported step kernel %d with %d rows per thread and %d block rows",
+                    t->kernel, t->rows_per_thread, t->block_rows);
+    ctx->kernel = t->kernel;
     ctx->rz = t->rows_per_thread;
+    ctx->brows = t->block_rows;
     ctx->graph_chunk = std::max(1, t->graph_chunk);
     ctx->strip_first = t->strip_first != 0;
     return MNAD_OK;
@@ -317,7 +337,9 @@ int pick_batch(mnad_ctx* ctx, size_t per_shot, int nshots, int batch)
     size_t budget = ctx->budget ? ctx->budget : (size_t)(0.85 * (double)(freeb + ctx->pool.cached));
     int b = (int)std::max<size_t>(1, budget / std::max<size_t>(per_shot, 1));
     if (batch > 0) b = std::min(b, batch);
-    return std::max(1, std::min(b, nshots));
+    b = std::max(1, std::min(b, nshots));
+    const int groups = (nshots + b - 1) / b;     // even groups: no small last batch
+    return (nshots + groups - 1) / groups;
 }
 
 int check_shots(const mnad_shots* sh)
@@ -358,6 +380,18 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
     for (int k = 0; k < sh->nshots; ++k) maxnr = std::max(maxnr, sh->rec_off[k + 1] - sh->rec_off[k]);
     const size_t esz = snap_fmt == MNAD_SNAP_F32 ? 4 : 2;
     const size_t per = 3 * shot_bytes(mt, nt, maxnr) + (size_t)K * ni * esz + 4 * ni * 4;
+    const TunePick* tp = nullptr;
+    if (!(flags & MNAD_FLAG_SERIAL)) {
+        tp = find_tuned(ctx, *g);
+        if (!tp && batch == 0 && ((flags & MNAD_FLAG_AUTOTUNE) || ctx->autotune_on_use)) {
+            TunePick pick;
+            CHK(autotune(ctx, ms, 32, &pick));
+            tp = find_tuned(ctx, *g);
+        }
+    }
+    TuningGuard tg(ctx, tp);
+    if (batch == 0 && tp) batch = std::max(1, tp->batch / 2);
+    if (flags & MNAD_FLAG_SERIAL) batch = 1;
     const int B = pick_batch(ctx, per, sh->nshots, batch);
     for (int k0 = 0; k0 < sh->nshots; k0 += B) {
         const int k1 = std::min(sh->nshots, k0 + B), nb = k1 - k0;
@@ -383,7 +417,11 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
             run.push_back(&fw[2 * b]);
             run.push_back(&fw[2 * b + 1]);
         }
-        CHK(run_batch(ctx, run, flags));
+        if (flags & MNAD_FLAG_SERIAL) {
+            for (Shot* sp : run) CHK(run_batch(ctx, std::vector<Shot*>{sp}, flags));
+        } else {
+            CHK(run_batch(ctx, run, flags));
+        }
         std::vector<Shot> ad(nb);
         run.clear();
         for (int b = 0; b < nb; ++b) {
@@ -410,7 +448,11 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
             attach_snaps(ad[b], std::move(fw[2 * b + 1].snaps), snap_fmt, fw[2 * b + 1].d.sscale);
             run.push_back(&ad[b]);
         }
-        CHK(run_batch(ctx, run, flags));
+        if (flags & MNAD_FLAG_SERIAL) {
+            for (Shot* sp : run) CHK(run_batch(ctx, std::vector<Shot*>{sp}, flags));
+        } else {
+            CHK(run_batch(ctx, run, flags));
+        }
         for (int b = 0; b < nb; ++b)
             HIPCHK(mnad_launch_stack(ad[b].img.as<float>(), ad[b].imgc.as<float>(),
                                      fw[2 * b + 1].illum.as<float>(), (float)T, (long)ni,
@@ -432,6 +474,18 @@ extern "C" int mnad_fwd_shots(mnad_ctx* ctx, const mnad_grid* g, const double* v
     CHK(make_model(ctx, g, v, m));
     int maxnr = 0;
     for (int k = 0; k < sh->nshots; ++k) maxnr = std::max(maxnr, sh->rec_off[k + 1] - sh->rec_off[k]);
+    const TunePick* tp = nullptr;
+    if (!(flags & MNAD_FLAG_SERIAL)) {
+        tp = find_tuned(ctx, *g);
+        if (!tp && batch == 0 && ((flags & MNAD_FLAG_AUTOTUNE) || ctx->autotune_on_use)) {
+            TunePick pick;
+            CHK(autotune(ctx, m, 32, &pick));
+            tp = find_tuned(ctx, *g);
+        }
+    }
+    TuningGuard tg(ctx, tp);
+    if (batch == 0 && tp) batch = tp->batch;
+    if (flags & MNAD_FLAG_SERIAL) batch = 1;
     const int B = pick_batch(ctx, shot_bytes(m, nt, maxnr), sh->nshots, batch);
     for (int k0 = 0; k0 < sh->nshots; k0 += B) {
         const int k1 = std::min(sh->nshots, k0 + B), nb = k1 - k0;

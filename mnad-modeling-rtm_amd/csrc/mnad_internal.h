@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -61,6 +62,13 @@ struct Kept {
     int nz = 0, nx = 0, npml = 0, nt = 0, T = 0, K = 0, ncol = 0;
 };
 
+// Autotuner result for one grid shape.
+struct TunePick {
+    int kernel, rz, brows;   // step-kernel variant
+    int batch;               // shots per launch
+    double us;               // step-kernel microseconds per shot-step
+};
+
 // Image / illumination stack (compensated sums over shots).
 struct Stack {
     mnad_grid g{};
@@ -79,9 +87,13 @@ struct mnad_ctx {
     std::map<int, mnad::Stack> stacks;
     int next_stack = 1;
     int graph_chunk = 64;
-    int rz = 1;                 // rows per thread of the step kernel (1, 2, 4)
+    int kernel = 0;             // step kernel family: 0 registers, 1 prefetch, 2 LDS
+    int rz = 1;                 // rows per thread
+    int brows = 4;              // thread rows per block (tile = 64 x brows*rz)
     int strip_first = 1;        // dispatch CPML-strip tiles first
     size_t budget = 0;          // bytes usable for batches (0 = from free memory)
+    int autotune_on_use = 0;    // batched calls tune an unseen grid shape first
+    std::map<std::tuple<int, int, int>, mnad::TunePick> tuned;   // (nz, nx, npml)
     // step-kernel accounting (mnad_timing)
     double t_ms = 0.0;
     long long t_launches = 0;
@@ -140,5 +152,27 @@ int download_field(mnad_ctx* c, const Model& m, const Shot& s, int field, double
 int download_interior(mnad_ctx* c, const mnad_grid& g, const float* a, const float* b,
                       double scale, double* out);
 int schedule(int nt, double vmin, double vmax, int* nf, int* T, int* K, int* ncol);
+int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out);
+const TunePick* find_tuned(mnad_ctx* c, const mnad_grid& g);
+
+// Applies a tuned step-kernel variant for the lifetime of the guard.
+struct TuningGuard {
+    mnad_ctx* c;
+    int k, r, b;
+    TuningGuard(mnad_ctx* c_, const TunePick* p) : c(c_), k(c_->kernel), r(c_->rz), b(c_->brows)
+    {
+        if (p) {
+            c->kernel = p->kernel;
+            c->rz = p->rz;
+            c->brows = p->brows;
+        }
+    }
+    ~TuningGuard()
+    {
+        c->kernel = k;
+        c->rz = r;
+        c->brows = b;
+    }
+};
 
 }  // namespace mnad

@@ -135,6 +135,9 @@ __device__ __forceinline__ void psi3(float* __restrict__ ps, const float* __rest
     ps[6 * ns + q] = p1; ps[7 * ns + q] = p2; ps[8 * ns + q] = p3;
 }
 
+__device__ __forceinline__ void snap_or_image(const ShotDev& d, int s, int j, int i, float du,
+                                              float nu);
+
 // Everything after the increments for one point: CPML memories on the
 // strips, the new level, injections, gathers, snapshots and imaging.
 __device__ __forceinline__ void finish_point(const ShotDev& d, int s, int j, int i, long p,
@@ -194,8 +197,16 @@ __device__ __forceinline__ void finish_point(const ShotDev& d, int s, int j, int
     PU[p] = nu;
     PV[p] = nv;
     PW[p] = nw;
+    if (d.T > 0 && !inx && !inz) snap_or_image(d, s, j, i, du, nu);
+}
 
-    if (d.T > 0 && !inx && !inz) {
+// d2u snapshot + illumination (forward) or imaging (adjoint) at an interior
+// point on the steps where the schedule takes them.
+__device__ __forceinline__ void snap_or_image(const ShotDev& d, int s, int j, int i, float du,
+                                              float nu)
+{
+    const int np = d.npml;
+    {
         const int nxi = d.nx - 2 * np;
         const long q = (long)(j - np) * nxi + (i - np);
         const long ni = (long)(d.nz - 2 * np) * nxi;
@@ -225,15 +236,15 @@ __device__ __forceinline__ void finish_point(const ShotDev& d, int s, int j, int
     }
 }
 
-// One step for every tile in `tiles`.  Block = BX x BZ threads; each thread
-// updates RZ consecutive rows of one column, so a tile is BX x (BZ*RZ)
+// One step for every tile in `tiles`.  Block = BX x BZR threads; each thread
+// updates RZ consecutive rows of one column, so a tile is BX x (BZR*RZ)
 // points and the column values shared by those rows are loaded once.
 // Local step of a shot: s = *base + rel - t0 (0-based); tiles of shots outside
 // [0, nt) exit at once, so a graph chunk may run past a shot's end.
 // Storage keeps MNAD_ZO + MNAD_ZPAD zero rows below the grid, so the loads of
 // a partial bottom tile stay inside the allocation (RZ <= MNAD_ZPAD).
-template <int RZ>
-__global__ __launch_bounds__(BX * BZ) void k_step(const ShotDev* __restrict__ shots,
+template <int RZ, int BZR = BZ>
+__global__ __launch_bounds__(BX * BZR) void k_step(const ShotDev* __restrict__ shots,
                                                   const TileRef* __restrict__ tiles,
                                                   const int* __restrict__ base, int rel)
 {
@@ -246,7 +257,7 @@ __global__ __launch_bounds__(BX * BZ) void k_step(const ShotDev* __restrict__ sh
     const int tj = tr.tile / d.ntx;
     const int ti = tr.tile - tj * d.ntx;
     const int i = ti * BX + threadIdx.x;
-    const int j0 = (tj * BZ + threadIdx.y) * RZ;
+    const int j0 = (tj * BZR + threadIdx.y) * RZ;
     if (i >= d.nx || j0 >= d.nz) return;
 
     const int cb = s & 1;
@@ -297,6 +308,209 @@ __global__ __launch_bounds__(BX * BZ) void k_step(const ShotDev* __restrict__ sh
         mnad_incr(rx, rz, u, v, vf[r][0], vf[r][1], w, wc[r], wc[r + 4], du, dv, dw);
         finish_point(d, s, j, i, p, u, v, w, du, dv, dw, rx, rz, PU, PV, PW);
     }
+}
+
+// LDS-staged variant: the block first copies its tile of U, Vs, Ws plus halos
+// (U: rows -1..TZ, cols -1..64; Vs: rows -1..TZ, cols -2..65; Ws: rows
+// -2..TZ+1, cols -1..64) into LDS with row-contiguous loads, then every
+// thread reads its neighbourhood from LDS (consecutive lanes, consecutive
+// banks).  About 5 global loads per point instead of 35 through the texture
+// path.  TZ rows per tile, TZ/BZ rows per thread.
+template <int TZ>
+__global__ __launch_bounds__(BX * BZ) void k_step_lds(const ShotDev* __restrict__ shots,
+                                                      const TileRef* __restrict__ tiles,
+                                                      const int* __restrict__ base, int rel)
+{
+    static_assert(TZ % BZ == 0 && TZ <= MNAD_ZPAD + 2, "tile height");
+    constexpr int UW = BX + 2, VW = BX + 4, WW = BX + 2;
+    __shared__ float su[(TZ + 2) * UW];
+    __shared__ float sv[(TZ + 2) * VW];
+    __shared__ float sw[(TZ + 4) * WW];
+
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int i0 = ti * BX, j0 = tj * TZ;
+
+    const int cb = s & 1;
+    const float* __restrict__ U = d.f[cb][0];
+    const float* __restrict__ V = d.f[cb][1];
+    const float* __restrict__ W = d.f[cb][2];
+    float* __restrict__ PU = d.f[cb ^ 1][0];
+    float* __restrict__ PV = d.f[cb ^ 1][1];
+    float* __restrict__ PW = d.f[cb ^ 1][2];
+    const int P = d.pitch, nx = d.nx, nz = d.nz;
+    const long org = (long)(j0 + MNAD_ZO) * P + (i0 + MNAD_XO);   // storage of (j0, i0)
+    const int tid = threadIdx.y * BX + threadIdx.x;
+
+    // Columns >= nx and rows >= nz are zero in storage; skipping them also
+    // keeps the loads of the right-most / bottom tiles inside the allocation.
+    for (int e = tid; e < (TZ + 2) * UW; e += BX * BZ) {
+        const int rr = e / UW, cc = e - rr * UW;
+        const int gj = j0 + rr - 1, gi = i0 + cc - 1;
+        su[e] = (gj < nz && gi < nx) ? U[org + (long)(rr - 1) * P + (cc - 1)] : 0.0f;
+    }
+    for (int e = tid; e < (TZ + 2) * VW; e += BX * BZ) {
+        const int rr = e / VW, cc = e - rr * VW;
+        const int gj = j0 + rr - 1, gi = i0 + cc - 2;
+        sv[e] = (gj < nz && gi < nx) ? V[org + (long)(rr - 1) * P + (cc - 2)] : 0.0f;
+    }
+    for (int e = tid; e < (TZ + 4) * WW; e += BX * BZ) {
+        const int rr = e / WW, cc = e - rr * WW;
+        const int gj = j0 + rr - 2, gi = i0 + cc - 1;
+        sw[e] = (gj < nz && gi < nx) ? W[org + (long)(rr - 2) * P + (cc - 1)] : 0.0f;
+    }
+    __syncthreads();
+
+    const int tx = threadIdx.x, i = i0 + tx;
+    if (i >= nx) return;
+#pragma unroll
+    for (int rr = 0; rr < TZ / BZ; ++rr) {
+        const int jl = threadIdx.y * (TZ / BZ) + rr;
+        const int j = j0 + jl;
+        if (j >= nz) break;
+        const long p = org + (long)jl * P + tx;
+        const float* a = su + (jl + 1) * UW + (tx + 1);
+        const float* b = sv + (jl + 1) * VW + (tx + 2);
+        const float* c = sw + (jl + 2) * WW + (tx + 1);
+        Nbh u, v, w;
+        u.nw = a[-UW - 1]; u.n = a[-UW]; u.ne = a[-UW + 1];
+        u.w = a[-1];       u.c = a[0];   u.e = a[1];
+        u.sw = a[UW - 1];  u.s = a[UW];  u.se = a[UW + 1];
+        v.nw = b[-VW - 1]; v.n = b[-VW]; v.ne = b[-VW + 1];
+        v.w = b[-1];       v.c = b[0];   v.e = b[1];
+        v.sw = b[VW - 1];  v.s = b[VW];  v.se = b[VW + 1];
+        w.nw = c[-WW - 1]; w.n = c[-WW]; w.ne = c[-WW + 1];
+        w.w = c[-1];       w.c = c[0];   w.e = c[1];
+        w.sw = c[WW - 1];  w.s = c[WW];  w.se = c[WW + 1];
+        const float rx = d.rx[p];
+        const float rz = rx * d.kap;
+        float du, dv, dw;
+        mnad_incr(rx, rz, u, v, b[-2], b[2], w, c[-2 * WW], c[2 * WW], du, dv, dw);
+        finish_point(d, s, j, i, p, u, v, w, du, dv, dw, rx, rz, PU, PV, PW);
+    }
+}
+
+// Same update as k_step<1>, with every per-point input issued before the
+// arithmetic: the receiver lookup (a chain of dependent loads), the previous
+// level, and on CPML strips the nine memories and three coefficients of each
+// axis, so a strip point costs one memory round trip instead of two.
+__global__ __launch_bounds__(BX * BZ) void k_step_pf(const ShotDev* __restrict__ shots,
+                                                     const TileRef* __restrict__ tiles,
+                                                     const int* __restrict__ base, int rel)
+{
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int i = ti * BX + threadIdx.x;
+    const int j = tj * BZ + threadIdx.y;
+    if (i >= d.nx || j >= d.nz) return;
+
+    const int cb = s & 1;
+    const float* __restrict__ U = d.f[cb][0];
+    const float* __restrict__ V = d.f[cb][1];
+    const float* __restrict__ W = d.f[cb][2];
+    float* __restrict__ PU = d.f[cb ^ 1][0];
+    float* __restrict__ PV = d.f[cb ^ 1][1];
+    float* __restrict__ PW = d.f[cb ^ 1][2];
+    const int P = d.pitch;
+    const long p = (long)(j + MNAD_ZO) * P + (i + MNAD_XO);
+    const int np = d.npml;
+    const bool inx = (i < np) || (i >= d.nx - np);
+    const bool inz = (j < np) || (j >= d.nz - np);
+
+    int r = -1;
+    if (d.nrec > 0) {
+        const int off = d.rrow[j];
+        if (off >= 0) r = d.rcol[off + i];
+    }
+    Nbh u, v, w;
+    load9(U, p, P, u);
+    load9(V, p, P, v);
+    load9(W, p, P, w);
+    const float vww = V[p - 2], vee = V[p + 2];
+    const float wnn = W[p - 2 * P], wss = W[p + 2 * P];
+    const float rx = d.rx[p];
+    const float pu = PU[p], pv = PV[p], pw = PW[p];
+
+    float cx[3] = {0.f, 0.f, 0.f}, px[9];
+    float cz[3] = {0.f, 0.f, 0.f}, pz[9];
+    long qx = 0, nsx = 0, qz = 0, nsz = 0;
+    if (inx) {
+        nsx = (long)d.nz * 2 * np;
+        qx = (long)j * 2 * np + ((i < np) ? i : i - (d.nx - 2 * np));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) cx[k] = d.cfx[k * nsx + qx];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) px[k] = d.psx[k * nsx + qx];
+    }
+    if (inz) {
+        nsz = (long)2 * np * d.nx;
+        qz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.nx + i;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) cz[k] = d.cfz[k * nsz + qz];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) pz[k] = d.psz[k * nsz + qz];
+    }
+    float radj = 0.0f;
+    if (d.mode == MODE_ADJ && r >= 0) radj = d.radj[(long)s * d.nrec + r];
+
+    const float rz = rx * d.kap;
+    float du, dv, dw;
+    mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+
+    if (inx || inz) {
+        // The CPML memories see the pre-injection U (fwd_PDE_exact_MNAD.m:95-111).
+        float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
+        if (d.mode == MODE_FWD && d.src_j >= 0) {
+            const float a0 = d.src_amp[s];
+            if (j == d.src_j) {
+                if (i == d.src_i) uc -= a0;
+                if (i - 1 == d.src_i) uw -= a0;
+                if (i + 1 == d.src_i) ue -= a0;
+            }
+            if (i == d.src_i) {
+                if (j - 1 == d.src_j) un -= a0;
+                if (j + 1 == d.src_j) us -= a0;
+            }
+        }
+        if (inx) {
+            du -= rx * mnad_psi(px[0], px[1], px[2], uw, uc, ue, cx[0], cx[1], cx[2]);
+            dv -= rx * mnad_psi(px[3], px[4], px[5], v.w, v.c, v.e, cx[0], cx[1], cx[2]);
+            dw -= rx * mnad_psi(px[6], px[7], px[8], w.w, w.c, w.e, cx[0], cx[1], cx[2]);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) d.psx[k * nsx + qx] = px[k];
+        }
+        if (inz) {
+            du -= rz * mnad_psi(pz[0], pz[1], pz[2], un, uc, us, cz[0], cz[1], cz[2]);
+            dv -= rz * mnad_psi(pz[3], pz[4], pz[5], v.n, v.c, v.s, cz[0], cz[1], cz[2]);
+            dw -= rz * mnad_psi(pz[6], pz[7], pz[8], w.n, w.c, w.s, cz[0], cz[1], cz[2]);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) d.psz[k * nsz + qz] = pz[k];
+        }
+    }
+
+    float nu = (2.0f * u.c + du) - pu;
+    const float nv = (2.0f * v.c + dv) - pv;
+    const float nw = (2.0f * w.c + dw) - pw;
+    if (d.mode == MODE_FWD) {
+        if (j == d.src_j && i == d.src_i) nu += d.src_amp[s + 1];
+        if (r >= 0) d.gather[(long)s * d.nrec + r] = u.c;
+    } else {
+        nu -= radj;
+    }
+    PU[p] = nu;
+    PV[p] = nv;
+    PW[p] = nw;
+    if (d.T > 0 && !inx && !inz) snap_or_image(d, s, j, i, du, nu);
 }
 
 __global__ void k_advance(int* base, int n)
@@ -378,15 +592,28 @@ hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, i
 }
 
 hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
-                            const int* base, int rel, int rz, hipStream_t st)
+                            const int* base, int rel, int kernel, int rows, int brows,
+                            hipStream_t st)
 {
     const dim3 g(ntiles), b(BX, BZ);
-    switch (rz) {
-    case 1: hipLaunchKernelGGL(k_step<1>, g, b, 0, st, shots, tiles, base, rel); break;
-    case 2: hipLaunchKernelGGL(k_step<2>, g, b, 0, st, shots, tiles, base, rel); break;
-    case 4: hipLaunchKernelGGL(k_step<4>, g, b, 0, st, shots, tiles, base, rel); break;
-    default: return hipErrorInvalidValue;
-    }
+    if (kernel == 0 && brows == 4 && rows == 1)
+        hipLaunchKernelGGL((k_step<1, 4>), g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 0 && brows == 4 && rows == 2)
+        hipLaunchKernelGGL((k_step<2, 4>), g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 0 && brows == 4 && rows == 4)
+        hipLaunchKernelGGL((k_step<4, 4>), g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 0 && brows == 8 && rows == 1)
+        hipLaunchKernelGGL((k_step<1, 8>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
+    else if (kernel == 0 && brows == 16 && rows == 1)
+        hipLaunchKernelGGL((k_step<1, 16>), g, dim3(BX, 16), 0, st, shots, tiles, base, rel);
+    else if (kernel == 1 && brows == 4 && rows == 1)
+        hipLaunchKernelGGL(k_step_pf, g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 2 && brows == 4 && rows == 1)
+        hipLaunchKernelGGL(k_step_lds<4>, g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 2 && brows == 4 && rows == 2)
+        hipLaunchKernelGGL(k_step_lds<8>, g, b, 0, st, shots, tiles, base, rel);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

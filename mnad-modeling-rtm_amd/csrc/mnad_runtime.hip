@@ -32,7 +32,8 @@ hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, i
                                    double dx, double dz, double dt, float* rx, float* cfx,
                                    float* cfz, hipStream_t st);
 hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
-                            const int* base, int rel, int rz, hipStream_t st);
+                            const int* base, int rel, int kernel, int rows, int brows,
+                            hipStream_t st);
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st);
 hipError_t mnad_launch_init_source(ShotDev* shots, int nshots, hipStream_t st);
 hipError_t mnad_launch_make_radj(const float* dobs, const float* dsyn, const float* arec,
@@ -403,7 +404,8 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     int nsteps = 0;
     std::vector<TileRef> list;
     std::vector<char> strip;
-    const int rz = c->rz, th = 4 * rz;   // tile = 64 columns x (4 * rz) rows
+    // tile = 64 columns x (block_rows * rows_per_thread) rows
+    const int kern = c->kernel, rz = c->rz, br = c->brows, th = br * rz;
     for (int k = 0; k < n; ++k) {
         h[k] = shots[k]->d;
         h[k].t0 = 0;
@@ -455,21 +457,21 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         hipEvent_t a, b;
         ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
     } evg{ev0, ev1};
-    HIPCHK(hipEventRecord(ev0, c->stream));
     const int ntiles = (int)table.size();
     const ShotDev* ds = dsh.as<ShotDev>();
     const TileRef* dt = dtab.as<TileRef>();
     int* base = dbase.as<int>();
     int chunk = c->graph_chunk;
     if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nsteps < 2 * chunk) {
-        for (int s = 0; s < nsteps; ++s) HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s, rz, c->stream));
+        HIPCHK(hipEventRecord(ev0, c->stream));
+        for (int s = 0; s < nsteps; ++s) HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s, kern, rz, br, c->stream));
     } else {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t ce = hipSuccess;
         for (int r = 0; r < chunk && ce == hipSuccess; ++r)
-            ce = mnad_launch_step(ds, dt, ntiles, base, r, rz, c->stream);
+            ce = mnad_launch_step(ds, dt, ntiles, base, r, kern, rz, br, c->stream);
         if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk, c->stream);
         hipError_t ee = hipStreamEndCapture(c->stream, &graph);
         HIPCHK(ce);
@@ -480,10 +482,10 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
             HIPCHK(ie);
         }
         const int full = nsteps / chunk, rem = nsteps - full * chunk;
-        hipError_t le = hipSuccess;
+        hipError_t le = hipEventRecord(ev0, c->stream);   // after instantiation
         for (int k = 0; k < full && le == hipSuccess; ++k) le = hipGraphLaunch(exec, c->stream);
         for (int r = 0; r < rem && le == hipSuccess; ++r)
-            le = mnad_launch_step(ds, dt, ntiles, base, r, rz, c->stream);
+            le = mnad_launch_step(ds, dt, ntiles, base, r, kern, rz, br, c->stream);
         hipError_t se = hipStreamSynchronize(c->stream);
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);

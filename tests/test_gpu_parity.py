@@ -181,21 +181,51 @@ def test_invalid_inputs_raise(mnad, ctx):
                                 int(f["npml"]), 1)           # non-integer index
 
 
-@pytest.mark.parametrize("rz", [1, 2, 4])
+@pytest.mark.parametrize("kernel,rz,br", [(0, 1, 4), (0, 2, 4), (0, 4, 4), (0, 1, 8), (0, 1, 16),
+                                          (1, 1, 4), (2, 1, 4), (2, 2, 4)])
 @pytest.mark.parametrize("strip_first", [0, 1])
-def test_launch_shapes_agree(mnad, ctx, oracle, rz, strip_first):
-    """Every rows-per-thread / dispatch-order variant of the step kernel meets
-    the oracle and agrees with the default shape to rounding."""
-    _say(f"launch shape rz={rz} strip_first={strip_first}")
+def test_launch_shapes_agree(mnad, ctx, oracle, kernel, rz, br, strip_first):
+    """Every kernel family / rows-per-thread / dispatch-order variant of the
+    step kernel meets the FP64 fixture and agrees with the default shape to
+    FP32 rounding (variants compile to different FMA contractions: ~1e-6
+    after 140 steps)."""
+    _say(f"launch shape kernel={kernel} rz={rz} br={br} strip_first={strip_first}")
     f = golden("fwd_rect")
     g = _grid(mnad, f)
     base = ctx.fwd(g, f["v"], f["w"], int(f["zs"]), int(f["xs"]), f["zr"], f["xr"])
     old = ctx.tuning()
     try:
-        ctx.set_tuning(rows_per_thread=rz, strip_first=strip_first)
+        ctx.set_tuning(kernel=kernel, rows_per_thread=rz, block_rows=br, strip_first=strip_first)
         r = ctx.fwd(g, f["v"], f["w"], int(f["zs"]), int(f["xs"]), f["zr"], f["xr"])
     finally:
         ctx.set_tuning(**old)
     for k in ("gather", "U", "V", "W"):
         assert rel_l2(r[k], f[k]) < GATHER_TOL, k
-        assert rel_l2(r[k], base[k]) < 1e-6, k
+        assert rel_l2(r[k], base[k]) < 1e-5, k
+
+
+def test_autotune_and_serial_paths(mnad, ctx):
+    """The autotuner picks a supported variant; RTM through the tuned batched
+    path and through the plain serial path (one propagation per launch
+    sequence, no graph) both reproduce the stacked fixture image."""
+    _say("autotune + serial")
+    f = golden("rtm_two_shot")
+    g = _grid(mnad, f, "vs")
+    tuning, per_launch, us = ctx.autotune(g, f["vs"], max_shots=4)
+    assert per_launch in (1, 2, 4) and us > 0
+    assert tuning["kernel"] in (0, 2)
+    n = int(f["nshots"])
+    zs = [int(f[f"zs{k}"]) for k in range(n)]
+    xs = [int(f[f"xs{k}"]) for k in range(n)]
+    recs = [(f[f"zr{k}"], f[f"xr{k}"]) for k in range(n)]
+    imgs = []
+    for flags in (0, mnad.FLAG_SERIAL | mnad.FLAG_NO_GRAPH):
+        sid = ctx.stack_create(g)
+        ctx.rtm_shots(sid, g, f["v"], f["vs"], f["w"], zs, xs, recs, snap_fmt=mnad.SNAP_F32,
+                      flags=flags)
+        img, ill, ns = ctx.stack_image(sid, g)
+        ctx.stack_destroy(sid)
+        assert ns == n
+        assert rel_l2(img, f["img"]) < IMAGE_TOL
+        imgs.append(img)
+    assert rel_l2(imgs[0], imgs[1]) < 1e-4
