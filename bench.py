@@ -139,6 +139,118 @@ def run_multi(args, mnad, wl, label):
     print(json.dumps(out), flush=True)
 
 
+def run_c4(args, mnad):
+    """Serving: Poisson arrivals at --qps of modelling (70 %) and RTM (30 %)
+    requests; physical grids U[256,2048] x U[256,1024] (dx = 10 m, 20 CPML
+    cells), layered + random models 1500-4500 m/s, Ricker U[10,30] Hz,
+    nt U[1000,8000], dt = 0.8*0.65*dx/vmax (main_Acoustic_Data.m:71,77).
+    Reports p50/p99 latency (submit -> result) and aggregate Gpt-updates/s."""
+    import threading
+    from __graft_entry__ import load_package
+    load_package()
+    import importlib
+    synth = importlib.import_module("mnad_amd.synth")
+    rng = np.random.default_rng(args.seed)
+    npml, dx = 20, 10.0
+    NZ, NX = 1024 + 2 * npml, 2048 + 2 * npml
+    noise = synth.correlated_noise((NZ, NX), 50.0, dx, rng)
+    noise_s = synth.gaussian_smooth(noise, 100.0, dx)
+    n = args.requests
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, n))
+    specs = []
+    for k in range(n):
+        nxp, nzp = int(rng.integers(256, 2049)), int(rng.integers(256, 1025))
+        specs.append(dict(nxp=nxp, nzp=nzp, fc=float(rng.uniform(10, 30)),
+                          nt=int(rng.integers(1000, 8001)), rtm=bool(rng.random() < 0.3),
+                          oz=int(rng.integers(0, NZ - nzp - 2 * npml + 1)),
+                          ox=int(rng.integers(0, NX - nxp - 2 * npml + 1)),
+                          xs=int(rng.integers(0, nxp))))
+
+    def build(sp):
+        nz, nx = sp["nzp"] + 2 * npml, sp["nxp"] + 2 * npml
+        z = np.arange(nz) * dx
+        base = 1500.0 + 2700.0 * (z / z[-1])[:, None]
+        crop = np.s_[sp["oz"]:sp["oz"] + nz, sp["ox"]:sp["ox"] + nx]
+        v = np.clip(base * (1 + 0.03 * noise[crop]), 1500.0, 4500.0)
+        vs = np.clip(base * (1 + 0.03 * noise_s[crop]), 1500.0, 4500.0) if sp["rtm"] else None
+        dt = 0.8 * 0.65 * dx / 4500.0
+        w = synth.ricker(sp["fc"], dt, sp["nt"])
+        xr = npml + np.arange(0, sp["nxp"], 4)
+        zr = np.full(xr.size, npml + 1)
+        g = mnad.Grid(nz, nx, npml, dx, dx, dt)
+        return g, v, vs, w, npml, npml + sp["xs"], zr, xr, nz * nx
+
+    lat, results, errors = {}, {}, []
+    tickets = []
+    lock = threading.Lock()
+    done_evt = threading.Event()
+    with mnad.Context(0) as ctx:
+        ctx.serve_start(hbm_budget=int(args.budget_gb * 2**30), max_active=args.max_active,
+                        chunk_steps=args.chunk)
+
+        def waiter():
+            k = 0
+            while k < n:
+                with lock:
+                    ready = k < len(tickets)
+                    item = tickets[k] if ready else None
+                if not ready:
+                    time.sleep(0.0005)
+                    continue
+                idx, t, ups = item
+                try:
+                    out, info = ctx.wait(t)
+                    lat[idx] = (info["done_s"] - info["submit_s"], info["admit_s"] - info["submit_s"])
+                    results[idx] = (info["submit_s"], info["done_s"], ups)
+                except Exception as e:   # noqa: BLE001 - reported in the JSON
+                    errors.append(f"{idx}: {e}")
+                k += 1
+                if k % 100 == 0:
+                    print(f"# {k}/{n} requests done", file=sys.stderr, flush=True)
+            done_evt.set()
+
+        th = threading.Thread(target=waiter, daemon=True)
+        th.start()
+        t0 = time.perf_counter()
+        for k, sp in enumerate(specs):
+            g, v, vs, w, zs, xs, zr, xr, N = build(sp)
+            delay = arrivals[k] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            kind = mnad.REQ_RTM if sp["rtm"] else mnad.REQ_MODEL
+            t = ctx.submit(kind, g, v, w, zs, xs, zr, xr, v_smooth=vs, snap_fmt=mnad.SNAP_F16,
+                           want_gather=not sp["rtm"])
+            ups = N * sp["nt"] * (3 if sp["rtm"] else 1)
+            with lock:
+                tickets.append((k, t, ups))
+        th.join()
+        ctx.serve_stop()
+    L = np.array([lat[k][0] for k in sorted(lat)]) * 1e3
+    Q = np.array([lat[k][1] for k in sorted(lat)]) * 1e3
+    first = min(r[0] for r in results.values())
+    last = max(r[1] for r in results.values())
+    updates = sum(r[2] for r in results.values())
+    out = {
+        "metric": METRIC, "value": round(updates / (last - first) / 1e9, 3),
+        "unit": "Gpt-updates/s", "n_gpus": 1, "steps": n, "warmup": 0,
+        "ms_per_step": round((last - first) * 1e3 / n, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded layered + random models)",
+        "config": {"workload": f"C4 serving: {n} requests, Poisson {args.qps}/s, 70% modelling / 30% RTM, "
+                               "grids 256x256..2048x1024 (+20 CPML), nt 1000..8000",
+                   "qps": args.qps, "requests": n, "chunk_steps": args.chunk,
+                   "budget_gb": args.budget_gb, "max_active": args.max_active},
+        "latency_ms": {"p50": round(float(np.percentile(L, 50)), 2),
+                       "p90": round(float(np.percentile(L, 90)), 2),
+                       "p99": round(float(np.percentile(L, 99)), 2),
+                       "max": round(float(L.max()), 2)},
+        "queue_wait_ms": {"p50": round(float(np.percentile(Q, 50)), 2),
+                          "p99": round(float(np.percentile(Q, 99)), 2)},
+        "makespan_s": round(last - first, 2), "errors": errors[:5], "n_errors": len(errors),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(wl, steps):
     """FP64 oracle (one thread) on the first `steps` forward steps of the
     true-model shot of the same workload."""
@@ -156,7 +268,13 @@ def cpu_baseline(wl, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--requests", type=int, default=1000, help="C4: number of requests")
+    ap.add_argument("--qps", type=float, default=10.0, help="C4: Poisson arrival rate")
+    ap.add_argument("--chunk", type=int, default=32, help="C4: steps between admissions")
+    ap.add_argument("--budget-gb", type=float, default=0.0, help="C4: HBM budget (0 = 80%% free)")
+    ap.add_argument("--max-active", type=int, default=64, help="C4: active requests at most")
+    ap.add_argument("--seed", type=int, default=4, help="C4: workload seed")
     ap.add_argument("--shots", type=int, default=128, help="C3: number of shots")
     ap.add_argument("--batch", type=int, default=0, help="C3: shots per launch sequence (0 = by memory)")
     ap.add_argument("--plain-shots", type=int, default=2, help="many-shot runs: shots timed on the plain path")
@@ -191,6 +309,9 @@ def main():
 
     from __graft_entry__ import load_package
     mnad = load_package()
+    if args.workload == "c4":
+        run_c4(args, mnad)
+        return
     if args.workload == "c3":
         nt = 6000 if args.nt == 4000 else args.nt
         run_multi(args, mnad, c3_workload(nt, args.shots),

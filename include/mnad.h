@@ -169,6 +169,52 @@ typedef struct {
 MNAD_API int mnad_get_tuning(mnad_ctx *ctx, mnad_tuning *t);
 MNAD_API int mnad_set_tuning(mnad_ctx *ctx, const mnad_tuning *t);
 
+/* ---- Serving: continuous batching of requests of any grid size ---------
+ * A server thread owns a private stream.  Requests are copied at submit and
+ * queued; at every chunk boundary (chunk_steps time steps) the loop admits
+ * queued requests while the HBM budget and max_active allow (in submission
+ * order, skipping any that do not fit yet), runs every active shot of every
+ * admitted request in one launch per step, and completes requests whose
+ * last step has run.  An RTM request runs its two forward propagations
+ * (v, v_smooth with snapshots), then its adjoint with imaging, in place.
+ * Results do not depend on what else is in the batch.  Output buffers must
+ * stay valid until mnad_wait returns for the ticket. */
+#define MNAD_REQ_MODEL 0   /* forward modelling on v: out_gather */
+#define MNAD_REQ_RTM 1     /* modelling on v + RTM on v_smooth: out_image */
+
+typedef struct {
+    int kind;                  /* MNAD_REQ_MODEL or MNAD_REQ_RTM */
+    mnad_grid grid;
+    const double *v;           /* [nz x nx] column-major */
+    const double *v_smooth;    /* RTM migration model, NULL for modelling */
+    int nt;
+    const double *w;           /* [nt] */
+    int zs, xs;                /* 0-based padded source */
+    int nr;
+    const int *zr, *xr;        /* 0-based padded receivers */
+    int snap_fmt;              /* RTM snapshot storage */
+    double *out_gather;        /* [nt x nr] observed gather (optional for RTM) */
+    double *out_image;         /* RTM: Im (Born_adj_MNAD.m:138), [nzi x nxi] */
+    double *out_illum;         /* RTM: T*sum(d2u^2), [nzi x nxi] (optional) */
+} mnad_request;
+
+typedef struct {
+    int status;                /* MNAD_OK or the request's error code */
+    double submit_s, admit_s, done_s;   /* server clock, seconds */
+    uint64_t bytes;            /* device bytes reserved for the request */
+} mnad_req_info;
+
+MNAD_API int mnad_serve_start(mnad_ctx *ctx, uint64_t hbm_budget, int max_active,
+                              int chunk_steps);
+MNAD_API int mnad_serve_stop(mnad_ctx *ctx);
+MNAD_API int mnad_submit(mnad_ctx *ctx, const mnad_request *req, int64_t *ticket);
+/* timeout_ms < 0 waits forever; MNAD_ERR_TIMEOUT leaves the ticket pending. */
+MNAD_API int mnad_wait(mnad_ctx *ctx, int64_t ticket, int timeout_ms, mnad_req_info *info);
+MNAD_API int mnad_cancel(mnad_ctx *ctx, int64_t ticket);
+/* queued / active requests, reserved and budget bytes, steps run so far */
+MNAD_API int mnad_serve_status(mnad_ctx *ctx, int *queued, int *active, uint64_t *reserved,
+                               uint64_t *budget, int64_t *steps);
+
 /* Autotuner: times the step-kernel variants and 1, 2, 4, ... up to max_shots
  * shots per launch with short forward probes on this grid and velocity, keeps
  * the fastest per (nz, nx, npml) in the context and returns it.  Batched calls

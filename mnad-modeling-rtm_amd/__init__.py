@@ -35,6 +35,7 @@ SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
 FLAG_NO_GRAPH = 1
 FLAG_SERIAL = 2
 FLAG_AUTOTUNE = 4
+REQ_MODEL, REQ_RTM = 0, 1
 SNAP_FORMATS = {"f32": SNAP_F32, "f16": SNAP_F16, "bf16": SNAP_BF16}
 
 # Every symbol include/mnad.h declares (checked by the CPU test suite).
@@ -44,6 +45,8 @@ EXPORTS = (
     "mnad_snap_schedule", "mnad_stack_create", "mnad_stack_destroy",
     "mnad_stack_image", "mnad_rtm_shots", "mnad_fwd_shots", "mnad_mem_info",
     "mnad_timing", "mnad_get_tuning", "mnad_set_tuning", "mnad_autotune",
+    "mnad_serve_start", "mnad_serve_stop", "mnad_submit", "mnad_wait", "mnad_cancel",
+    "mnad_serve_status",
 )
 
 
@@ -67,6 +70,20 @@ class Shots(C.Structure):
     _fields_ = [("nshots", C.c_int), ("zs", C.POINTER(C.c_int)), ("xs", C.POINTER(C.c_int)),
                 ("rec_off", C.POINTER(C.c_int)), ("zr", C.POINTER(C.c_int)),
                 ("xr", C.POINTER(C.c_int))]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("grid", Grid), ("v", C.POINTER(C.c_double)),
+                ("v_smooth", C.POINTER(C.c_double)), ("nt", C.c_int),
+                ("w", C.POINTER(C.c_double)), ("zs", C.c_int), ("xs", C.c_int), ("nr", C.c_int),
+                ("zr", C.POINTER(C.c_int)), ("xr", C.POINTER(C.c_int)), ("snap_fmt", C.c_int),
+                ("out_gather", C.POINTER(C.c_double)), ("out_image", C.POINTER(C.c_double)),
+                ("out_illum", C.POINTER(C.c_double))]
+
+
+class ReqInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double), ("bytes", C.c_uint64)]
 
 
 _lib = None
@@ -106,6 +123,14 @@ def lib():
             L.mnad_mem_info.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
             L.mnad_autotune.argtypes = [vp, gp, dp, C.c_int, C.POINTER(Tuning),
                                         C.POINTER(C.c_int), C.POINTER(C.c_double)]
+            L.mnad_serve_start.argtypes = [vp, C.c_uint64, C.c_int, C.c_int]
+            L.mnad_serve_stop.argtypes = [vp]
+            L.mnad_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
+            L.mnad_wait.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(ReqInfo)]
+            L.mnad_cancel.argtypes = [vp, C.c_int64]
+            L.mnad_serve_status.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_int64)]
             L.mnad_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
             L.mnad_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
             L.mnad_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
@@ -318,6 +343,59 @@ class Context:
         t = Tuning(cur["kernel"], cur["rows_per_thread"], cur["block_rows"], cur["graph_chunk"],
                    cur["strip_first"])
         _check(lib().mnad_set_tuning(self._h, C.byref(t)))
+
+    # -------------------------------------------------------------- serving
+    def serve_start(self, hbm_budget=0, max_active=64, chunk_steps=32):
+        """Start the continuous-batching server (budget 0: 80 % of free HBM)."""
+        self._reqs = {}
+        _check(lib().mnad_serve_start(self._h, int(hbm_budget), int(max_active),
+                                      int(chunk_steps)))
+
+    def serve_stop(self):
+        _check(lib().mnad_serve_stop(self._h))
+        self._reqs = {}
+
+    def submit(self, kind, grid: Grid, v, w, zs, xs, zr, xr, v_smooth=None,
+               snap_fmt=SNAP_F16, want_gather=True, want_illum=False):
+        """Queue a modelling (REQ_MODEL) or RTM (REQ_RTM) request (0-based
+        indices); returns a ticket for wait()."""
+        v = _f64(v)
+        vs = _f64(v_smooth) if v_smooth is not None else None
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.float64).ravel())
+        zr, xr = _i32(zr), _i32(xr)
+        nt, nr = w.size, zr.size
+        gather = np.zeros((nt, nr), order="F") if (want_gather or kind == REQ_MODEL) else None
+        shp = (grid.nz - 2 * grid.npml, grid.nx - 2 * grid.npml)
+        image = np.zeros(shp, order="F") if kind == REQ_RTM else None
+        illum = np.zeros(shp, order="F") if (kind == REQ_RTM and want_illum) else None
+        q = Request(int(kind), grid, _dp(v), _dp(vs), nt, _dp(w), int(zs), int(xs), nr, _ip(zr),
+                    _ip(xr), int(snap_fmt), _dp(gather), _dp(image), _dp(illum))
+        t = C.c_int64()
+        _check(lib().mnad_submit(self._h, C.byref(q), C.byref(t)))
+        self._reqs[t.value] = dict(gather=gather, image=image, illum=illum)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        """Block until the request completes; returns (outputs, info).
+        Raises MnadError with the request's status on failure/cancel."""
+        info = ReqInfo()
+        rc = lib().mnad_wait(self._h, int(ticket), int(timeout_ms), C.byref(info))
+        if rc == ERR_TIMEOUT:
+            raise MnadError(rc, lib().mnad_last_error().decode())
+        outs = self._reqs.pop(int(ticket), None)
+        _check(rc)
+        return outs, dict(status=info.status, submit_s=info.submit_s, admit_s=info.admit_s,
+                          done_s=info.done_s, bytes=info.bytes)
+
+    def cancel(self, ticket):
+        _check(lib().mnad_cancel(self._h, int(ticket)))
+
+    def serve_status(self):
+        q, a, r, b, s = C.c_int(), C.c_int(), C.c_uint64(), C.c_uint64(), C.c_int64()
+        _check(lib().mnad_serve_status(self._h, C.byref(q), C.byref(a), C.byref(r), C.byref(b),
+                                       C.byref(s)))
+        return dict(queued=q.value, active=a.value, reserved=r.value, budget=b.value,
+                    steps=s.value)
 
     def autotune(self, grid: Grid, v, max_shots=32):
         """Time kernel variants x shots per launch on this grid; returns

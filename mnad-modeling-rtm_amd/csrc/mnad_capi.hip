@@ -101,6 +101,7 @@ int mnad_ctx_create(int device, mnad_ctx** out)
 void mnad_ctx_destroy(mnad_ctx* ctx)
 {
     if (!ctx) return;
+    server_destroy(ctx);
     {
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         (void)hipSetDevice(ctx->device);

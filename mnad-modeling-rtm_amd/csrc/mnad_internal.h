@@ -78,8 +78,13 @@ struct Stack {
 
 }  // namespace mnad
 
+namespace mnad {
+struct Server;
+}
+
 struct mnad_ctx {
     int device = 0;
+    mnad::Server* server = nullptr;   // serving loop (mnad_serve_start)
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
     mnad::Pool pool;
@@ -126,6 +131,7 @@ struct ShotSpec {
 
 struct Shot {
     ShotDev d{};
+    long long t0 = 0;                           // server: global step of local step 0
     DevBuf state[6], psx, psz, amp, rrow, rcol, gather, radj, arec, snaps, illum, img, imgc;
     std::vector<int> uid;                       // receiver r -> unique point u
     std::vector<std::pair<int, int>> upts;      // unique points (j, i)
@@ -146,6 +152,8 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s);
 void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
 int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);
 int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags);
+void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table);
+double algorithmic_bytes(const ShotDev& d);
 int download_gather(mnad_ctx* c, const Shot& s, int nt, int nr, double* out);
 int download_field(mnad_ctx* c, const Model& m, const Shot& s, int field, double scale,
                    double* out);
@@ -153,6 +161,7 @@ int download_interior(mnad_ctx* c, const mnad_grid& g, const float* a, const flo
                       double scale, double* out);
 int schedule(int nt, double vmin, double vmax, int* nf, int* T, int* K, int* ncol);
 int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out);
+void server_destroy(mnad_ctx* c);
 const TunePick* find_tuned(mnad_ctx* c, const mnad_grid& g);
 
 // Applies a tuned step-kernel variant for the lifetime of the guard.

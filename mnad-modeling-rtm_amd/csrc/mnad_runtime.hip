@@ -396,20 +396,13 @@ int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data)
 }
 
 // ------------------------------------------------------------ batch runner
-int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
+void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table)
 {
-    const int n = (int)shots.size();
-    if (n == 0) return MNAD_OK;
-    std::vector<ShotDev> h(n);
-    int nsteps = 0;
     std::vector<TileRef> list;
     std::vector<char> strip;
     // tile = 64 columns x (block_rows * rows_per_thread) rows
-    const int kern = c->kernel, rz = c->rz, br = c->brows, th = br * rz;
-    for (int k = 0; k < n; ++k) {
-        h[k] = shots[k]->d;
-        h[k].t0 = 0;
-        nsteps = std::max(nsteps, h[k].nt);
+    const int th = c->brows * c->rz;
+    for (int k = 0; k < (int)h.size(); ++k) {
         const int ntx = (h[k].nx + 63) / 64, ntz = (h[k].nz + th - 1) / th;
         const int np = h[k].npml;
         h[k].ntx = ntx;
@@ -426,7 +419,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     // run the CPML-strip tiles (about twice the memory traffic) go first.
     const int nx8 = 8;
     const size_t L = (list.size() + nx8 - 1) / nx8 * nx8, per = L / nx8;
-    std::vector<TileRef> table(L, TileRef{-1, 0});
+    table.assign(L, TileRef{-1, 0});
     for (int x = 0; x < nx8; ++x) {
         std::vector<size_t> run;
         for (size_t q = x * per; q < std::min(list.size(), (x + 1) * per); ++q) run.push_back(q);
@@ -434,6 +427,33 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
             std::stable_partition(run.begin(), run.end(), [&](size_t q) { return strip[q] != 0; });
         for (size_t m = 0; m < run.size(); ++m) table[m * nx8 + x] = list[run[m]];
     }
+}
+
+double algorithmic_bytes(const ShotDev& d)
+{
+    const double N = (double)d.nz * d.nx, np2 = 2.0 * d.npml;
+    const double strips = np2 * d.nz + np2 * d.nx;
+    const double ni = (double)(d.nz - 2 * d.npml) * (d.nx - 2 * d.npml);
+    const double esz = d.sfmt == SNAP_F32 ? 4.0 : 2.0;
+    double b = (double)d.nt * (40.0 * N + 72.0 * strips + 4.0 * d.nrec);
+    if (d.T > 0) b += (double)d.K * ni * (esz + (d.mode == MODE_FWD ? 8.0 : 16.0));
+    return b;
+}
+
+int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
+{
+    const int n = (int)shots.size();
+    if (n == 0) return MNAD_OK;
+    std::vector<ShotDev> h(n);
+    int nsteps = 0;
+    const int kern = c->kernel, rz = c->rz, br = c->brows;
+    for (int k = 0; k < n; ++k) {
+        h[k] = shots[k]->d;
+        h[k].t0 = 0;
+        nsteps = std::max(nsteps, h[k].nt);
+    }
+    std::vector<TileRef> table;
+    build_tiles(c, h, table);
     DevBuf dsh, dtab, dbase;
     CHK(upload(c, dsh, h.data(), h.size() * sizeof(ShotDev)));
     CHK(upload(c, dtab, table.data(), table.size() * sizeof(TileRef)));
@@ -441,15 +461,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     HIPCHK(mnad_launch_init_source(dsh.as<ShotDev>(), n, c->stream));
     // Algorithmic HBM bytes of the whole sequence (see mnad_timing).
     double bytes = 0.0;
-    for (int k = 0; k < n; ++k) {
-        const ShotDev& d = h[k];
-        const double N = (double)d.nz * d.nx, np2 = 2.0 * d.npml;
-        const double strips = np2 * d.nz + np2 * d.nx;
-        const double ni = (double)(d.nz - 2 * d.npml) * (d.nx - 2 * d.npml);
-        const double esz = d.sfmt == SNAP_F32 ? 4.0 : 2.0;
-        bytes += (double)d.nt * (40.0 * N + 72.0 * strips + 4.0 * d.nrec);
-        if (d.T > 0) bytes += (double)d.K * ni * (esz + (d.mode == MODE_FWD ? 8.0 : 16.0));
-    }
+    for (int k = 0; k < n; ++k) bytes += algorithmic_bytes(h[k]);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));

@@ -1,0 +1,142 @@
+"""Serving loop (mnad_serve_* / mnad_submit / mnad_wait / mnad_cancel) on the
+GPU: continuous batching of modelling and RTM requests of different grid
+sizes.  Each result must equal the same request served alone (bitwise: a
+shot's arithmetic does not depend on its neighbours in a launch) and meet the
+FP64 fixtures at the stated tolerances (1e-4 gathers, 1e-3 f16 images)."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _grid(mnad, f, vkey="v"):
+    v = f[vkey]
+    return mnad.Grid(v.shape[0], v.shape[1], int(f["npml"]), float(f["dx"]), float(f["dz"]),
+                     float(f["dt"]))
+
+
+def _model_req(mnad, name):
+    f = golden(name)
+    return f, dict(kind=mnad.REQ_MODEL, grid=_grid(mnad, f), v=f["v"], w=f["w"],
+                   zs=int(f["zs"]), xs=int(f["xs"]), zr=f["zr"], xr=f["xr"])
+
+
+def _rtm_req(mnad, name, k=0, fmt=None):
+    f = golden(name)
+    return f, dict(kind=mnad.REQ_RTM, grid=_grid(mnad, f, "vs"), v=f["v"], v_smooth=f["vs"],
+                   w=f["w"], zs=int(f[f"zs{k}"]), xs=int(f[f"xs{k}"]), zr=f[f"zr{k}"],
+                   xr=f[f"xr{k}"], snap_fmt=mnad.SNAP_F16 if fmt is None else fmt)
+
+
+@pytest.fixture()
+def server(mnad):
+    ctx = mnad.Context(0)
+    ctx.serve_start(hbm_budget=0, max_active=64, chunk_steps=16)
+    yield ctx
+    ctx.serve_stop()
+    ctx.close()
+
+
+def _submit(ctx, r):
+    kw = dict(r)
+    return ctx.submit(kw.pop("kind"), kw.pop("grid"), kw.pop("v"), kw.pop("w"), kw.pop("zs"),
+                      kw.pop("xs"), kw.pop("zr"), kw.pop("xr"), **kw)
+
+
+def test_alone_equals_in_the_mix(mnad, server):
+    _say("serving: alone vs mix")
+    fm, rm = _model_req(mnad, "fwd_rect")
+    fs, rs = _model_req(mnad, "fwd_square")
+    fr, rr = _rtm_req(mnad, "rtm_two_shot", 0)
+    fq, rq = _rtm_req(mnad, "rtm_rect", 0, fmt=mnad.SNAP_F32)
+    alone = {}
+    for key, r in (("m", rm), ("s", rs), ("r", rr), ("q", rq)):
+        out, info = server.wait(_submit(server, r))
+        assert info["status"] == mnad.OK
+        alone[key] = out
+    # the mix: everything in flight at once, plus a second copy of each
+    tickets = {key: _submit(server, r) for key, r in (("m", rm), ("r", rr), ("s", rs), ("q", rq))}
+    extra = [_submit(server, r) for r in (rq, rm, rr)]
+    mixed = {key: server.wait(t)[0] for key, t in tickets.items()}
+    for t in extra:
+        server.wait(t)
+    assert np.array_equal(mixed["m"]["gather"], alone["m"]["gather"])
+    assert np.array_equal(mixed["s"]["gather"], alone["s"]["gather"])
+    assert np.array_equal(mixed["r"]["image"], alone["r"]["image"])
+    assert np.array_equal(mixed["q"]["image"], alone["q"]["image"])
+    assert rel_l2(mixed["m"]["gather"], fm["gather"]) < 1e-4
+    assert rel_l2(mixed["s"]["gather"], fs["gather"]) < 1e-4
+    assert rel_l2(mixed["r"]["gather"], fr["data0"]) < 1e-4
+    nzi, nxi = mixed["r"]["image"].shape
+    assert rel_l2(mixed["r"]["image"].ravel(order="F"), fr["Im0"]) < 1e-3
+    assert rel_l2(mixed["q"]["image"].ravel(order="F"), fq["Im0"]) < 1e-4
+
+
+def test_late_arrival_joins_running_batch(mnad, server):
+    """A short request submitted while a long one runs is admitted at the
+    next chunk boundary and finishes first."""
+    _say("serving: late arrival")
+    f, long_r = _model_req(mnad, "fwd_square")
+    long_r = dict(long_r)
+    long_r["w"] = np.concatenate([f["w"]] * 40)          # 6400 steps
+    _, short_r = _model_req(mnad, "fwd_rect")
+    t_long = _submit(server, long_r)
+    deadline = time.time() + 30
+    while server.serve_status()["steps"] < 64 and time.time() < deadline:
+        time.sleep(0.001)
+    t_short = _submit(server, short_r)
+    out_s, info_s = server.wait(t_short)
+    out_l, info_l = server.wait(t_long)
+    assert info_s["admit_s"] < info_l["done_s"]
+    assert info_s["done_s"] < info_l["done_s"]
+
+
+def test_budget_admission_cancel_and_rejection(mnad):
+    _say("serving: budget, cancel, reject")
+    f, r = _model_req(mnad, "fwd_square")
+    ctx = mnad.Context(0)
+    try:
+        # find the reserved size of one request, then allow exactly one at a time
+        ctx.serve_start(hbm_budget=0, chunk_steps=16)
+        _, info = ctx.wait(_submit(ctx, r))
+        one = info["bytes"]
+        ctx.serve_stop()
+        ctx.serve_start(hbm_budget=int(1.5 * one), chunk_steps=16)
+        ts = [_submit(ctx, r) for _ in range(3)]
+        t_cancel = _submit(ctx, r)
+        ctx.cancel(t_cancel)
+        infos = [ctx.wait(t)[1] for t in ts]
+        for a, b in zip(infos, infos[1:]):          # one at a time, in order
+            assert b["admit_s"] >= a["done_s"] - 1e-3
+        with pytest.raises(mnad.MnadError) as e:
+            ctx.wait(t_cancel)
+        assert e.value.code == mnad.ERR_CANCELLED
+        big = dict(r)
+        big["w"] = np.concatenate([f["w"]] * 4000)   # gather alone exceeds the budget
+        with pytest.raises(mnad.MnadError) as e:
+            _submit(ctx, big)
+        assert e.value.code == mnad.ERR_BUDGET
+        bad = dict(r)
+        bad["v"] = f["v"].copy()
+        bad["v"][4, 4] = np.nan
+        with pytest.raises(mnad.MnadError) as e:
+            _submit(ctx, bad)
+        assert e.value.code == mnad.ERR_ARG
+        cfl = dict(r)
+        cfl["grid"] = mnad.Grid(r["grid"].nz, r["grid"].nx, r["grid"].npml, 10.0, 10.0, 0.05)
+        with pytest.raises(mnad.MnadError) as e:
+            _submit(ctx, cfl)
+        assert e.value.code == mnad.ERR_ARG
+        ctx.serve_stop()
+    finally:
+        ctx.close()
