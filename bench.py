@@ -275,6 +275,11 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=0.0, help="C4: HBM budget (0 = 80%% free)")
     ap.add_argument("--max-active", type=int, default=64, help="C4: active requests at most")
     ap.add_argument("--seed", type=int, default=4, help="C4: workload seed")
+    ap.add_argument("--save-images-only", action="store_true",
+                    help="with --save-outputs: image and illumination only (no gathers)")
+    ap.add_argument("--save-outputs", default="",
+                    help="C2: after timing, run one more shot and save its gathers, image and "
+                         "illumination to this .npz (for tests/validate_full_size.py)")
     ap.add_argument("--shots", type=int, default=128, help="C3: number of shots")
     ap.add_argument("--batch", type=int, default=0, help="C3: shots per launch sequence (0 = by memory)")
     ap.add_argument("--plain-shots", type=int, default=2, help="many-shot runs: shots timed on the plain path")
@@ -355,6 +360,17 @@ def main():
             print("# plain shot (one propagation per launch sequence, no graph) done",
                   file=sys.stderr, flush=True)
         ctx.stack_destroy(sid)
+        if args.save_outputs:
+            sid = ctx.stack_create(g)
+            dobs, dsyn = ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], [wl["zs"]], [wl["xs"]], recs,
+                                       snap_fmt=fmt, batch=1, want_gathers=True)
+            img, ill, _ = ctx.stack_image(sid, g)
+            ctx.stack_destroy(sid)
+            extra = {} if args.save_images_only else dict(dobs=dobs[0].astype(np.float32),
+                                                          dsyn=dsyn[0].astype(np.float32))
+            np.savez_compressed(args.save_outputs, img=img, illum=ill, nt=nt, snap=args.snap,
+                                **extra)
+            print(f"# saved {args.save_outputs}", file=sys.stderr, flush=True)
 
     ms_per_step = el / args.steps * 1e3
     updates = 3.0 * N * nt * args.steps
