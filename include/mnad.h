@@ -101,6 +101,15 @@ MNAD_API int mnad_born_adj(mnad_ctx *ctx, const mnad_grid *g, const double *v, i
 /* Drop the snapshots kept for shot ss (no-op if none). */
 MNAD_API int mnad_release(mnad_ctx *ctx, int ss);
 
+/* Spill the snapshots kept for shot ss to a file / load them back as the
+ * kept snapshots of ss.  This is the cross-process hand-off the reference
+ * does with ./Wavefields/UU_<ss>.mat (fwd_PDE_exact_MNAD.m:161,
+ * Born_adj_MNAD.m:33) when the forward and the adjoint of one shot run in
+ * different parfor workers.  The file holds a small header (grid, nt, T, K,
+ * ncol, format, scale) and the snapshots in their storage format. */
+MNAD_API int mnad_snap_save(mnad_ctx *ctx, int ss, const char *path);
+MNAD_API int mnad_snap_load(mnad_ctx *ctx, int ss, const char *path);
+
 /* Snapshot schedule of fwd_PDE_exact_MNAD.m:46-47 for a velocity range. */
 MNAD_API int mnad_snap_schedule(int nt, double vmin, double vmax, int *nf, int *T,
                        int *K, int *ncol);

@@ -46,7 +46,7 @@ EXPORTS = (
     "mnad_stack_image", "mnad_rtm_shots", "mnad_fwd_shots", "mnad_mem_info",
     "mnad_timing", "mnad_get_tuning", "mnad_set_tuning", "mnad_autotune",
     "mnad_serve_start", "mnad_serve_stop", "mnad_submit", "mnad_wait", "mnad_cancel",
-    "mnad_serve_status",
+    "mnad_serve_status", "mnad_snap_save", "mnad_snap_load",
 )
 
 
@@ -112,6 +112,8 @@ def lib():
             L.mnad_born_adj.argtypes = [vp, gp, dp, C.c_int, C.c_int, ip, ip, dp, C.c_int,
                                         C.c_int, dp]
             L.mnad_release.argtypes = [vp, C.c_int]
+            L.mnad_snap_save.argtypes = [vp, C.c_int, C.c_char_p]
+            L.mnad_snap_load.argtypes = [vp, C.c_int, C.c_char_p]
             L.mnad_snap_schedule.argtypes = [C.c_int, C.c_double, C.c_double, ip, ip, ip, ip]
             L.mnad_stack_create.argtypes = [vp, gp, ip]
             L.mnad_stack_destroy.argtypes = [vp, C.c_int]
@@ -269,6 +271,12 @@ class Context:
 
     def release(self, ss):
         _check(lib().mnad_release(self._h, int(ss)))
+
+    def snap_save(self, ss, path):
+        _check(lib().mnad_snap_save(self._h, int(ss), os.fsencode(path)))
+
+    def snap_load(self, ss, path):
+        _check(lib().mnad_snap_load(self._h, int(ss), os.fsencode(path)))
 
     def stack_create(self, grid: Grid) -> int:
         sid = C.c_int()

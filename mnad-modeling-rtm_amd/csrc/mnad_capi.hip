@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -266,6 +267,91 @@ int mnad_release(mnad_ctx* ctx, int ss)
 {
     ENTER(ctx);
     ctx->kept.erase(ss);
+    return MNAD_OK;
+}
+
+namespace {
+struct SnapHeader {
+    char magic[8];   // "MNADSNP1"
+    int32_t nz, nx, npml, nt, T, K, ncol, fmt;
+    float sscale;
+    int32_t pad;
+    uint64_t bytes;
+};
+}  // namespace
+
+int mnad_snap_save(mnad_ctx* ctx, int ss, const char* path)
+{
+    ENTER(ctx);
+    if (!path) return fail(MNAD_ERR_ARG, "path is NULL");
+    auto it = ctx->kept.find(ss);
+    if (it == ctx->kept.end()) return fail(MNAD_ERR_STATE, "no snapshots kept for shot %d", ss);
+    const Kept& k = it->second;
+    const size_t ni = (size_t)(k.nz - 2 * k.npml) * (k.nx - 2 * k.npml);
+    const size_t bytes = (size_t)k.K * ni * (k.fmt == MNAD_SNAP_F32 ? 4 : 2);
+    SnapHeader h{};
+    memcpy(h.magic, "MNADSNP1", 8);
+    h.nz = k.nz; h.nx = k.nx; h.npml = k.npml; h.nt = k.nt; h.T = k.T; h.K = k.K;
+    h.ncol = k.ncol; h.fmt = k.fmt; h.sscale = k.sscale; h.bytes = bytes;
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(MNAD_ERR_ARG, "cannot open %s for writing", path);
+    bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
+    std::vector<char> buf(std::min<size_t>(bytes, (size_t)256 << 20));
+    for (size_t off = 0; ok && off < bytes; off += buf.size()) {
+        const size_t n = std::min(buf.size(), bytes - off);
+        if (hipMemcpy(buf.data(), static_cast<const char*>(k.snaps.p) + off, n,
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+            fclose(f);
+            return fail(MNAD_ERR_HIP, "copy of snapshots to the host failed");
+        }
+        ok = fwrite(buf.data(), 1, n, f) == n;
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(MNAD_ERR_ARG, "write to %s failed", path);
+    return MNAD_OK;
+}
+
+int mnad_snap_load(mnad_ctx* ctx, int ss, const char* path)
+{
+    ENTER(ctx);
+    if (!path) return fail(MNAD_ERR_ARG, "path is NULL");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(MNAD_ERR_STATE, "cannot open %s", path);
+    SnapHeader h{};
+    if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, "MNADSNP1", 8) != 0 || h.K < 0 ||
+        h.fmt < MNAD_SNAP_F32 || h.fmt > MNAD_SNAP_BF16 || h.nz <= 2 * h.npml || h.nx <= 2 * h.npml) {
+        fclose(f);
+        return fail(MNAD_ERR_ARG, "%s is not a snapshot file", path);
+    }
+    const size_t ni = (size_t)(h.nz - 2 * h.npml) * (h.nx - 2 * h.npml);
+    const size_t bytes = (size_t)h.K * ni * (h.fmt == MNAD_SNAP_F32 ? 4 : 2);
+    if (bytes != h.bytes) {
+        fclose(f);
+        return fail(MNAD_ERR_ARG, "%s: inconsistent header", path);
+    }
+    Kept k;
+    int rc = dalloc(ctx, k.snaps, bytes, false);
+    if (rc != MNAD_OK) {
+        fclose(f);
+        return rc;
+    }
+    std::vector<char> buf(std::min<size_t>(bytes, (size_t)256 << 20));
+    for (size_t off = 0; off < bytes; off += buf.size()) {
+        const size_t n = std::min(buf.size(), bytes - off);
+        if (fread(buf.data(), 1, n, f) != n) {
+            fclose(f);
+            return fail(MNAD_ERR_ARG, "%s is truncated", path);
+        }
+        if (hipMemcpy(static_cast<char*>(k.snaps.p) + off, buf.data(), n, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+            fclose(f);
+            return fail(MNAD_ERR_HIP, "copy of snapshots to the device failed");
+        }
+    }
+    fclose(f);
+    k.fmt = h.fmt; k.sscale = h.sscale; k.nz = h.nz; k.nx = h.nx; k.npml = h.npml;
+    k.nt = h.nt; k.T = h.T; k.K = h.K; k.ncol = h.ncol;
+    ctx->kept[ss] = std::move(k);
     return MNAD_OK;
 }
 
