@@ -32,12 +32,12 @@ METRIC = "Gpt-updates/s & RTM shots/hr vs plain FP32 at ≤1e-4 rel err; p50/p99
 HBM_PEAK_GBS = 8000.0
 
 
-def c2_workload(nt):
+def c2_workload(nt, npml=30):
     from __graft_entry__ import load_package
     load_package()
     import importlib
     synth = importlib.import_module("mnad_amd.synth")
-    npml, dx, dt, fc = 30, 10.0, 8e-4, 20.0
+    dx, dt, fc = 10.0, 8e-4, 20.0
     V, Vs = synth.model_c2(seed=2024)
     v, vs = synth.pad_edges(V, npml), synth.pad_edges(Vs, npml)
     w = synth.ricker(fc, dt, nt)
@@ -70,6 +70,40 @@ def c3_workload(nt, nshots):
         recs.append((np.full(xr.size, npml + 1), npml + xr))
     return dict(v=v, vs=vs, w=w, npml=npml, dx=dx, dt=dt, zs=[npml] * nshots,
                 xs=list(npml + xs_phys), recs=recs)
+
+
+def c5_workload(nt, nshots, crop=False):
+    """C5: 8001 x 2001 (+30 CPML), dx = dz = 12.5 m, faulted layered model with
+    a 4500 m/s body (seeded), Ricker 30 Hz, dt = 1 ms.  `nshots` sources evenly
+    spaced on the first physical row, split-spread receivers +-160 cells (2 km)
+    on the second row.  crop=True: the 1001 x 501 window at the model centre
+    with one shot in its middle and all 1001 receivers (the FP64 oracle case)."""
+    from __graft_entry__ import load_package
+    load_package()
+    import importlib
+    synth = importlib.import_module("mnad_amd.synth")
+    npml, dx, dt, fc = 30, 12.5, 1e-3, 30.0
+    V, Vs = synth.model_c5(seed=5)
+    if crop:
+        z0, x0 = 0, (V.shape[1] - 1001) // 2
+        V, Vs = V[z0:z0 + 501, x0:x0 + 1001], Vs[z0:z0 + 501, x0:x0 + 1001]
+    v, vs = synth.pad_edges(V, npml), synth.pad_edges(Vs, npml)
+    w = synth.ricker(fc, dt, nt)
+    nxp = V.shape[1]
+    if crop:
+        xs_phys = np.array([nxp // 2])
+        recs = [(np.full(nxp, npml + 1), npml + np.arange(nxp))]
+    else:
+        xs_phys = np.rint(np.linspace(0, nxp - 1, nshots)).astype(int)
+        recs = []
+        for x in xs_phys:
+            xr = np.arange(max(0, x - 160), min(nxp, x + 161))
+            recs.append((np.full(xr.size, npml + 1), npml + xr))
+    wl = dict(v=v, vs=vs, w=w, npml=npml, dx=dx, dt=dt, zs=[npml] * len(xs_phys),
+              xs=list(npml + xs_phys), recs=recs)
+    if crop:   # single-shot keys of the C2 path
+        wl.update(zs=npml, xs=int(npml + xs_phys[0]), zr=recs[0][0], xr=recs[0][1])
+    return wl
 
 
 def c2_multi_workload(nt, nshots):
@@ -186,7 +220,7 @@ def run_c4(args, mnad):
     done_evt = threading.Event()
     with mnad.Context(0) as ctx:
         ctx.serve_start(hbm_budget=int(args.budget_gb * 2**30), max_active=args.max_active,
-                        chunk_steps=args.chunk)
+                        chunk_steps=args.chunk, max_points=int(args.max_points))
 
         def waiter():
             k = 0
@@ -239,7 +273,8 @@ def run_c4(args, mnad):
         "config": {"workload": f"C4 serving: {n} requests, Poisson {args.qps}/s, 70% modelling / 30% RTM, "
                                "grids 256x256..2048x1024 (+20 CPML), nt 1000..8000",
                    "qps": args.qps, "requests": n, "chunk_steps": args.chunk,
-                   "budget_gb": args.budget_gb, "max_active": args.max_active},
+                   "budget_gb": args.budget_gb, "max_active": args.max_active,
+                   "max_points": args.max_points},
         "latency_ms": {"p50": round(float(np.percentile(L, 50)), 2),
                        "p90": round(float(np.percentile(L, 90)), 2),
                        "p99": round(float(np.percentile(L, 99)), 2),
@@ -268,12 +303,14 @@ def cpu_baseline(wl, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5crop"])
     ap.add_argument("--requests", type=int, default=1000, help="C4: number of requests")
     ap.add_argument("--qps", type=float, default=10.0, help="C4: Poisson arrival rate")
     ap.add_argument("--chunk", type=int, default=32, help="C4: steps between admissions")
     ap.add_argument("--budget-gb", type=float, default=0.0, help="C4: HBM budget (0 = 80%% free)")
     ap.add_argument("--max-active", type=int, default=64, help="C4: active requests at most")
+    ap.add_argument("--max-points", type=float, default=4e6,
+                    help="C4: admit while fewer grid points than this are stepping (0: no limit)")
     ap.add_argument("--seed", type=int, default=4, help="C4: workload seed")
     ap.add_argument("--save-images-only", action="store_true",
                     help="with --save-outputs: image and illumination only (no gathers)")
@@ -289,6 +326,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--snap", default="f16", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--npml", type=int, default=30, help="C2: CPML cells (30 in the workload)")
     ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -317,6 +355,11 @@ def main():
     if args.workload == "c4":
         run_c4(args, mnad)
         return
+    if args.workload == "c5":
+        nt = 12000 if args.nt == 4000 else args.nt
+        run_multi(args, mnad, c5_workload(nt, args.shots),
+                  f"C5: 8001x2001 (+30 CPML) dx=12.5 m, {args.shots} shots batched RTM, nt={nt}")
+        return
     if args.workload == "c3":
         nt = 6000 if args.nt == 4000 else args.nt
         run_multi(args, mnad, c3_workload(nt, args.shots),
@@ -326,7 +369,11 @@ def main():
         run_multi(args, mnad, c2_multi_workload(args.nt, args.shots_c2),
                   f"C2 grid: 1001x501 (+30 CPML), {args.shots_c2} shots batched RTM, nt={args.nt}")
         return
-    wl = c2_workload(args.nt)
+    if args.workload == "c5crop":
+        args.nt = 12000 if args.nt == 4000 else args.nt
+        wl = c5_workload(args.nt, 1, crop=True)
+    else:
+        wl = c2_workload(args.nt, args.npml)
     v = wl["v"]
     g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
     fmt = mnad.SNAP_FORMATS[args.snap]
@@ -390,8 +437,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded layered + correlated-random model)",
-        "config": {"workload": "C2: 1001x501 (+30 CPML) dx=10 m, nt=4000, dt=0.8 ms, Ricker 20 Hz, "
-                               "1001 receivers; modelling + 1-shot RTM per step",
+        "config": {"workload": ("C5 window: 1001x501 (+30 CPML) of the C5 model, dx=12.5 m, "
+                                f"nt={nt}, dt=1 ms, Ricker 30 Hz, 1001 receivers; modelling + "
+                                "1-shot RTM per step") if args.workload == "c5crop" else
+                               ("C2: 1001x501 (+30 CPML) dx=10 m, nt=4000, dt=0.8 ms, Ricker "
+                                "20 Hz, 1001 receivers; modelling + 1-shot RTM per step"),
                    "grid_padded": [int(v.shape[0]), int(v.shape[1])], "nt": nt,
                    "snapshots": args.snap, "updates_per_step": 3 * N * nt},
         "shots_per_hour": round(3600.0 / (ms_per_step / 1e3), 1),

@@ -166,6 +166,8 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  *   with rows_per_thread 1, 2 or 4, or block_rows 8 or 16 with 1 row;
  * kernel 1 = block_rows 4, 1 row, every per-point input issued first;
  * kernel 2 = block_rows 4, tiles + halos staged in LDS, rows_per_thread 1 or 2;
+ * kernel 3 = two time steps per launch (block_rows 16, 1 row; 60 x 12 output
+ *   tiles), shots keep four time levels -- set it before creating shots;
  * graph_chunk = step launches per captured hipGraph (<= 1: no graph);
  * strip_first != 0 dispatches the CPML-strip tiles of each XCD first. */
 typedef struct {
@@ -181,8 +183,8 @@ MNAD_API int mnad_set_tuning(mnad_ctx *ctx, const mnad_tuning *t);
 /* ---- Serving: continuous batching of requests of any grid size ---------
  * A server thread owns a private stream.  Requests are copied at submit and
  * queued; at every chunk boundary (chunk_steps time steps) the loop admits
- * queued requests while the HBM budget and max_active allow (in submission
- * order, skipping any that do not fit yet), runs every active shot of every
+ * queued requests in submission order while the HBM budget, max_active and
+ * max_points allow, runs every active shot of every
  * admitted request in one launch per step, and completes requests whose
  * last step has run.  An RTM request runs its two forward propagations
  * (v, v_smooth with snapshots), then its adjoint with imaging, in place.
@@ -213,8 +215,14 @@ typedef struct {
     uint64_t bytes;            /* device bytes reserved for the request */
 } mnad_req_info;
 
+/* hbm_budget: device bytes the admitted requests may reserve (0: 80 % of free
+ * memory); max_active: requests in flight at most; max_points: grid points
+ * (summed over the shots stepping) above which no further request is
+ * admitted -- enough to fill the GPU without stretching every request by
+ * sharing it more widely (0: no limit); chunk_steps: steps between
+ * admissions.  Admission is first come, first served. */
 MNAD_API int mnad_serve_start(mnad_ctx *ctx, uint64_t hbm_budget, int max_active,
-                              int chunk_steps);
+                              uint64_t max_points, int chunk_steps);
 MNAD_API int mnad_serve_stop(mnad_ctx *ctx);
 MNAD_API int mnad_submit(mnad_ctx *ctx, const mnad_request *req, int64_t *ticket);
 /* timeout_ms < 0 waits forever; MNAD_ERR_TIMEOUT leaves the ticket pending. */

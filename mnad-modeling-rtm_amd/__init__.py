@@ -125,7 +125,7 @@ def lib():
             L.mnad_mem_info.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
             L.mnad_autotune.argtypes = [vp, gp, dp, C.c_int, C.POINTER(Tuning),
                                         C.POINTER(C.c_int), C.POINTER(C.c_double)]
-            L.mnad_serve_start.argtypes = [vp, C.c_uint64, C.c_int, C.c_int]
+            L.mnad_serve_start.argtypes = [vp, C.c_uint64, C.c_int, C.c_uint64, C.c_int]
             L.mnad_serve_stop.argtypes = [vp]
             L.mnad_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_int64)]
             L.mnad_wait.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(ReqInfo)]
@@ -353,11 +353,12 @@ class Context:
         _check(lib().mnad_set_tuning(self._h, C.byref(t)))
 
     # -------------------------------------------------------------- serving
-    def serve_start(self, hbm_budget=0, max_active=64, chunk_steps=32):
-        """Start the continuous-batching server (budget 0: 80 % of free HBM)."""
+    def serve_start(self, hbm_budget=0, max_active=64, chunk_steps=32, max_points=0):
+        """Start the continuous-batching server (budget 0: 80 % of free HBM;
+        max_points 0: no work limit)."""
         self._reqs = {}
         _check(lib().mnad_serve_start(self._h, int(hbm_budget), int(max_active),
-                                      int(chunk_steps)))
+                                      int(max_points), int(chunk_steps)))
 
     def serve_stop(self):
         _check(lib().mnad_serve_stop(self._h))

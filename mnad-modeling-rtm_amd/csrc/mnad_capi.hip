@@ -46,9 +46,10 @@ extern "C" hipError_t mnad_launch_stack(const float* img, const float* imgc, con
 static bool tuning_ok(int kernel, int rows, int brows)
 {
     if (kernel == 0 && brows == 4) return rows == 1 || rows == 2 || rows == 4;
-    if (kernel == 0 && (brows == 8 || brows == 16)) return rows == 1;
+    if (kernel == 0 && (brows == 1 || brows == 2 || brows == 8 || brows == 16)) return rows == 1;
     if (kernel == 1) return brows == 4 && rows == 1;
     if (kernel == 2) return brows == 4 && (rows == 1 || rows == 2);
+    if (kernel == 3) return brows == 16 && rows == 1;
     return false;
 }
 

@@ -31,7 +31,10 @@ struct ShotDev {
     float kap;                      // rz / rx = dx^2 / dz^2
     float inv_dt2;                  // 1 / dt^2 (snapshot d2u)
     const float* rx;                // a / dx^2 per point, grid storage
-    float* f[2][3];                 // state [parity][U, Vs, Ws], grid storage
+    float* f[4][3];                 // state [level][U, Vs, Ws], grid storage: time level
+                                    // t lives in f[t % nlev]; nlev = 2, or 4 for the
+                                    // two-steps-per-launch kernel
+    int nlev;
     // CPML, strip-compact: x strips = 2*npml columns x nz rows,
     // z strips = 2*npml rows x nx columns.
     float* psx;                     // [9][nz * 2 npml]   (field*3 + level)

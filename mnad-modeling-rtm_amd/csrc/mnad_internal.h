@@ -92,7 +92,7 @@ struct mnad_ctx {
     std::map<int, mnad::Stack> stacks;
     int next_stack = 1;
     int graph_chunk = 64;
-    int kernel = 0;             // step kernel family: 0 registers, 1 prefetch, 2 LDS
+    int kernel = 0;             // step kernel: 0 registers, 1 prefetch, 2 LDS, 3 two steps
     int rz = 1;                 // rows per thread
     int brows = 4;              // thread rows per block (tile = 64 x brows*rz)
     int strip_first = 1;        // dispatch CPML-strip tiles first
@@ -132,7 +132,7 @@ struct ShotSpec {
 struct Shot {
     ShotDev d{};
     long long t0 = 0;                           // server: global step of local step 0
-    DevBuf state[6], psx, psz, amp, rrow, rcol, gather, radj, arec, snaps, illum, img, imgc;
+    DevBuf state[12], psx, psz, amp, rrow, rcol, gather, radj, arec, snaps, illum, img, imgc;
     std::vector<int> uid;                       // receiver r -> unique point u
     std::vector<std::pair<int, int>> upts;      // unique points (j, i)
     std::vector<double> arec_h;                 // (v dt)^2 at unique points
@@ -153,6 +153,8 @@ void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
 int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);
 int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags);
 void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table);
+// Output tile of the current step kernel (columns x rows) and steps per launch.
+void tile_shape(const mnad_ctx* c, int* tw, int* th, int* steps_per_launch);
 double algorithmic_bytes(const ShotDev& d);
 int download_gather(mnad_ctx* c, const Shot& s, int nt, int nr, double* out);
 int download_field(mnad_ctx* c, const Model& m, const Shot& s, int field, double scale,

@@ -513,6 +513,205 @@ __global__ __launch_bounds__(BX * BZ) void k_step_pf(const ShotDev* __restrict__
     if (d.T > 0 && !inx && !inz) snap_or_image(d, s, j, i, du, nu);
 }
 
+// ---------------------------------------------------- two steps per launch
+// Temporal blocking: a block of 64 x 16 threads owns a 60 x 12 output tile.
+// Phase 0 stages time level s of U, Vs, Ws on the tile plus a 4-point halo in
+// LDS; phase 1 computes level s+1 on the 64 x 16 extended region (tile plus a
+// 2-point halo, one point per thread, halo points computed redundantly and
+// never written) into LDS; phase 2 computes level s+2 on the tile.  Time
+// level t lives in f[t % 4]: the launch reads levels s-1, s and writes s+1,
+// s+2, so no block overwrites a level a neighbouring block still reads.  CPML
+// memories, receivers, snapshots and imaging are handled per phase exactly as
+// in the one-step kernels (owner thread only for every global write).
+constexpr int X2 = 64, Z2 = 16;            // extended region = block shape
+constexpr int OX2 = X2 - 4, OZ2 = Z2 - 4;  // output tile
+constexpr int LX2 = X2 + 4, LZ2 = Z2 + 4;  // staged level s (extended + 2)
+
+__device__ __forceinline__ void nbh_lds(const float* a, int W, Nbh& o)
+{
+    o.nw = a[-W - 1]; o.n = a[-W]; o.ne = a[-W + 1];
+    o.w = a[-1];      o.c = a[0];  o.e = a[1];
+    o.sw = a[W - 1];  o.s = a[W];  o.se = a[W + 1];
+}
+
+// CPML of one point for one step with the memories held in registers.
+__device__ __forceinline__ void psi_regs(const ShotDev& d, int s, int j, int i, bool inx, bool inz,
+                                         float rx, float rz, const Nbh& u, const Nbh& v,
+                                         const Nbh& w, const float* cx, float* px,
+                                         const float* cz, float* pz, float& du, float& dv,
+                                         float& dw)
+{
+    float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
+    if (d.mode == MODE_FWD && d.src_j >= 0) {
+        const float a0 = d.src_amp[s];
+        if (j == d.src_j) {
+            if (i == d.src_i) uc -= a0;
+            if (i - 1 == d.src_i) uw -= a0;
+            if (i + 1 == d.src_i) ue -= a0;
+        }
+        if (i == d.src_i) {
+            if (j - 1 == d.src_j) un -= a0;
+            if (j + 1 == d.src_j) us -= a0;
+        }
+    }
+    if (inx) {
+        du -= rx * mnad_psi(px[0], px[1], px[2], uw, uc, ue, cx[0], cx[1], cx[2]);
+        dv -= rx * mnad_psi(px[3], px[4], px[5], v.w, v.c, v.e, cx[0], cx[1], cx[2]);
+        dw -= rx * mnad_psi(px[6], px[7], px[8], w.w, w.c, w.e, cx[0], cx[1], cx[2]);
+    }
+    if (inz) {
+        du -= rz * mnad_psi(pz[0], pz[1], pz[2], un, uc, us, cz[0], cz[1], cz[2]);
+        dv -= rz * mnad_psi(pz[3], pz[4], pz[5], v.n, v.c, v.s, cz[0], cz[1], cz[2]);
+        dw -= rz * mnad_psi(pz[6], pz[7], pz[8], w.n, w.c, w.s, cz[0], cz[1], cz[2]);
+    }
+}
+
+__global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ shots,
+                                                   const TileRef* __restrict__ tiles,
+                                                   const int* __restrict__ base, int rel)
+{
+    __shared__ float cu[LZ2 * LX2], cv[LZ2 * LX2], cw[LZ2 * LX2];
+    __shared__ float nu_[Z2 * X2], nv_[Z2 * X2], nw_[Z2 * X2];
+
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;   // uniform over the block
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int x0 = ti * OX2, z0 = tj * OZ2;   // first output point
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = x0 - 2 + tx, j = z0 - 2 + ty;   // this thread's extended point
+    const int nx = d.nx, nz = d.nz, P = d.pitch, np = d.npml;
+    const bool in_grid = i >= 0 && i < nx && j >= 0 && j < nz;
+    const bool owned = in_grid && tx >= 2 && tx < X2 - 2 && ty >= 2 && ty < Z2 - 2;
+
+    const float* __restrict__ Lp[3] = {d.f[(s + 3) & 3][0], d.f[(s + 3) & 3][1], d.f[(s + 3) & 3][2]};
+    const float* __restrict__ Lc[3] = {d.f[s & 3][0], d.f[s & 3][1], d.f[s & 3][2]};
+    float* __restrict__ L1[3] = {d.f[(s + 1) & 3][0], d.f[(s + 1) & 3][1], d.f[(s + 1) & 3][2]};
+    float* __restrict__ L2[3] = {d.f[(s + 2) & 3][0], d.f[(s + 2) & 3][1], d.f[(s + 2) & 3][2]};
+
+    // ---- phase 0: stage level s (rows z0-4 .., columns x0-4 ..) in LDS
+    const int tid = ty * X2 + tx;
+    for (int e = tid; e < LZ2 * LX2; e += X2 * Z2) {
+        const int rr = e / LX2, c = e - rr * LX2;
+        const int gj = z0 - 4 + rr, gi = x0 - 4 + c;
+        const bool ok = gj >= 0 && gj < nz && gi >= 0 && gi < nx;
+        const long q = (long)(gj + MNAD_ZO) * P + (gi + MNAD_XO);
+        cu[e] = ok ? Lc[0][q] : 0.0f;
+        cv[e] = ok ? Lc[1][q] : 0.0f;
+        cw[e] = ok ? Lc[2][q] : 0.0f;
+    }
+    const long p = (long)(j + MNAD_ZO) * P + (i + MNAD_XO);
+    float pu = 0.f, pv = 0.f, pw = 0.f, rx = 0.f;
+    bool inx = false, inz = false;
+    float cx[3] = {0.f, 0.f, 0.f}, px[9], cz[3] = {0.f, 0.f, 0.f}, pz[9];
+    long qx = 0, nsx = 0, qz = 0, nsz = 0;
+    int r = -1;
+    if (in_grid) {
+        pu = Lp[0][p];
+        pv = Lp[1][p];
+        pw = Lp[2][p];
+        rx = d.rx[p];
+        inx = (i < np) || (i >= nx - np);
+        inz = (j < np) || (j >= nz - np);
+        if (inx) {
+            nsx = (long)nz * 2 * np;
+            qx = (long)j * 2 * np + ((i < np) ? i : i - (nx - 2 * np));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) cx[k] = d.cfx[k * nsx + qx];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) px[k] = d.psx[k * nsx + qx];
+        }
+        if (inz) {
+            nsz = (long)2 * np * nx;
+            qz = (long)((j < np) ? j : j - (nz - 2 * np)) * nx + i;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) cz[k] = d.cfz[k * nsz + qz];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) pz[k] = d.psz[k * nsz + qz];
+        }
+        if (d.nrec > 0) {
+            const int off = d.rrow[j];
+            if (off >= 0) r = d.rcol[off + i];
+        }
+    }
+    const float rz = rx * d.kap;
+    __syncthreads();
+
+    // ---- phase 1: level s+1 on the extended region
+    float n1u = 0.f, n1v = 0.f, n1w = 0.f;
+    if (in_grid) {
+        const int l = (ty + 2) * LX2 + (tx + 2);
+        Nbh u, v, w;
+        nbh_lds(cu + l, LX2, u);
+        nbh_lds(cv + l, LX2, v);
+        nbh_lds(cw + l, LX2, w);
+        float du, dv, dw;
+        mnad_incr(rx, rz, u, v, cv[l - 2], cv[l + 2], w, cw[l - 2 * LX2], cw[l + 2 * LX2], du, dv,
+                  dw);
+        if (inx || inz) psi_regs(d, s, j, i, inx, inz, rx, rz, u, v, w, cx, px, cz, pz, du, dv, dw);
+        n1u = (2.0f * u.c + du) - pu;
+        n1v = (2.0f * v.c + dv) - pv;
+        n1w = (2.0f * w.c + dw) - pw;
+        if (d.mode == MODE_FWD) {
+            if (j == d.src_j && i == d.src_i) n1u += d.src_amp[s + 1];
+            if (owned && r >= 0) d.gather[(long)s * d.nrec + r] = u.c;
+        } else if (r >= 0) {
+            n1u -= d.radj[(long)s * d.nrec + r];
+        }
+        if (owned) {
+            L1[0][p] = n1u;
+            L1[1][p] = n1v;
+            L1[2][p] = n1w;
+            if (d.T > 0 && !inx && !inz) snap_or_image(d, s, j, i, du, n1u);
+        }
+    }
+    nu_[tid] = n1u;
+    nv_[tid] = n1v;
+    nw_[tid] = n1w;
+    __syncthreads();
+
+    const bool second = s + 1 < d.nt;
+    if (owned && second) {
+        // ---- phase 2: level s+2 on the tile
+        Nbh u, v, w;
+        nbh_lds(nu_ + tid, X2, u);
+        nbh_lds(nv_ + tid, X2, v);
+        nbh_lds(nw_ + tid, X2, w);
+        float du, dv, dw;
+        mnad_incr(rx, rz, u, v, nv_[tid - 2], nv_[tid + 2], w, nw_[tid - 2 * X2],
+                  nw_[tid + 2 * X2], du, dv, dw);
+        if (inx || inz)
+            psi_regs(d, s + 1, j, i, inx, inz, rx, rz, u, v, w, cx, px, cz, pz, du, dv, dw);
+        const int l = (ty + 2) * LX2 + (tx + 2);   // level s at this point
+        float n2u = (2.0f * u.c + du) - cu[l];
+        const float n2v = (2.0f * v.c + dv) - cv[l];
+        const float n2w = (2.0f * w.c + dw) - cw[l];
+        if (d.mode == MODE_FWD) {
+            if (j == d.src_j && i == d.src_i) n2u += d.src_amp[s + 2];
+            if (r >= 0) d.gather[(long)(s + 1) * d.nrec + r] = u.c;
+        } else if (r >= 0) {
+            n2u -= d.radj[(long)(s + 1) * d.nrec + r];
+        }
+        L2[0][p] = n2u;
+        L2[1][p] = n2v;
+        L2[2][p] = n2w;
+        if (d.T > 0 && !inx && !inz) snap_or_image(d, s + 1, j, i, du, n2u);
+    }
+    if (owned) {   // CPML memories after the last step this launch ran
+        if (inx) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) d.psx[k * nsx + qx] = px[k];
+        }
+        if (inz) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) d.psz[k * nsz + qz] = pz[k];
+        }
+    }
+}
+
 __global__ void k_advance(int* base, int n)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *base += n;
@@ -602,6 +801,10 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL((k_step<2, 4>), g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 0 && brows == 4 && rows == 4)
         hipLaunchKernelGGL((k_step<4, 4>), g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 0 && brows == 1 && rows == 1)
+        hipLaunchKernelGGL((k_step<1, 1>), g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
+    else if (kernel == 0 && brows == 2 && rows == 1)
+        hipLaunchKernelGGL((k_step<1, 2>), g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
     else if (kernel == 0 && brows == 8 && rows == 1)
         hipLaunchKernelGGL((k_step<1, 8>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
     else if (kernel == 0 && brows == 16 && rows == 1)
@@ -612,6 +815,8 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL(k_step_lds<4>, g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 2 && brows == 4 && rows == 2)
         hipLaunchKernelGGL(k_step_lds<8>, g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 3 && brows == 16 && rows == 1)
+        hipLaunchKernelGGL(k_step2, g, dim3(X2, Z2), 0, st, shots, tiles, base, rel);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

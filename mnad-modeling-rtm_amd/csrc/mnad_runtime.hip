@@ -288,7 +288,8 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
     s.d.rx = m.rx.as<float>();
     s.d.ntx = (g.nx + 63) / 64;
     s.d.src_j = s.d.src_i = -1;
-    for (int k = 0; k < 6; ++k) {
+    s.d.nlev = c->kernel == 3 ? 4 : 2;
+    for (int k = 0; k < 3 * s.d.nlev; ++k) {
         CHK(dalloc(c, s.state[k], m.elems * sizeof(float), true));
         s.d.f[k / 3][k % 3] = s.state[k].as<float>();
     }
@@ -396,20 +397,33 @@ int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data)
 }
 
 // ------------------------------------------------------------ batch runner
+void tile_shape(const mnad_ctx* c, int* tw, int* th, int* spl)
+{
+    if (c->kernel == 3) {   // two steps per launch: 60 x 12 outputs of a 64 x 16 block
+        *tw = 60;
+        *th = 12;
+        *spl = 2;
+    } else {                // 64 columns x (block_rows * rows_per_thread) rows
+        *tw = 64;
+        *th = c->brows * c->rz;
+        *spl = 1;
+    }
+}
+
 void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table)
 {
     std::vector<TileRef> list;
     std::vector<char> strip;
-    // tile = 64 columns x (block_rows * rows_per_thread) rows
-    const int th = c->brows * c->rz;
+    int tw, th, spl;
+    tile_shape(c, &tw, &th, &spl);
     for (int k = 0; k < (int)h.size(); ++k) {
-        const int ntx = (h[k].nx + 63) / 64, ntz = (h[k].nz + th - 1) / th;
+        const int ntx = (h[k].nx + tw - 1) / tw, ntz = (h[k].nz + th - 1) / th;
         const int np = h[k].npml;
         h[k].ntx = ntx;
         for (int t = 0; t < ntx * ntz; ++t) {
             list.push_back(TileRef{k, t});
             const int tj = t / ntx, ti = t - tj * ntx;
-            strip.push_back(ti * 64 < np || ti * 64 + 63 >= h[k].nx - np || tj * th < np ||
+            strip.push_back(ti * tw < np || ti * tw + tw - 1 >= h[k].nx - np || tj * th < np ||
                             (tj + 1) * th - 1 >= h[k].nz - np);
         }
     }
@@ -451,6 +465,9 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         h[k] = shots[k]->d;
         h[k].t0 = 0;
         nsteps = std::max(nsteps, h[k].nt);
+        if (h[k].nlev != (kern == 3 ? 4 : 2))
+            return fail(MNAD_ERR_STATE, "shot built for %d time levels cannot run on step kernel %d",
+                        h[k].nlev, kern);
     }
     std::vector<TileRef> table;
     build_tiles(c, h, table);
@@ -474,17 +491,21 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     const TileRef* dt = dtab.as<TileRef>();
     int* base = dbase.as<int>();
     int chunk = c->graph_chunk;
-    if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nsteps < 2 * chunk) {
+    int tw, th, spl;
+    tile_shape(c, &tw, &th, &spl);
+    const int nlaunch = (nsteps + spl - 1) / spl;   // a launch covers spl steps
+    if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nlaunch < 2 * chunk) {
         HIPCHK(hipEventRecord(ev0, c->stream));
-        for (int s = 0; s < nsteps; ++s) HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s, kern, rz, br, c->stream));
+        for (int s = 0; s < nlaunch; ++s)
+            HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s * spl, kern, rz, br, c->stream));
     } else {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t ce = hipSuccess;
         for (int r = 0; r < chunk && ce == hipSuccess; ++r)
-            ce = mnad_launch_step(ds, dt, ntiles, base, r, kern, rz, br, c->stream);
-        if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk, c->stream);
+            ce = mnad_launch_step(ds, dt, ntiles, base, r * spl, kern, rz, br, c->stream);
+        if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk * spl, c->stream);
         hipError_t ee = hipStreamEndCapture(c->stream, &graph);
         HIPCHK(ce);
         HIPCHK(ee);
@@ -493,11 +514,11 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
             (void)hipGraphDestroy(graph);
             HIPCHK(ie);
         }
-        const int full = nsteps / chunk, rem = nsteps - full * chunk;
+        const int full = nlaunch / chunk, rem = nlaunch - full * chunk;
         hipError_t le = hipEventRecord(ev0, c->stream);   // after instantiation
         for (int k = 0; k < full && le == hipSuccess; ++k) le = hipGraphLaunch(exec, c->stream);
         for (int r = 0; r < rem && le == hipSuccess; ++r)
-            le = mnad_launch_step(ds, dt, ntiles, base, r, kern, rz, br, c->stream);
+            le = mnad_launch_step(ds, dt, ntiles, base, r * spl, kern, rz, br, c->stream);
         hipError_t se = hipStreamSynchronize(c->stream);
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
@@ -509,7 +530,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
     c->t_ms += ms;
-    c->t_launches += nsteps;
+    c->t_launches += nlaunch;
     c->t_bytes += bytes;
     return MNAD_OK;
 }
@@ -537,7 +558,7 @@ int download_field(mnad_ctx* c, const Model& m, const Shot& s, int field, double
     if (!out) return MNAD_OK;
     const mnad_grid& g = m.g;
     std::vector<float> buf(m.elems);
-    const int par = s.d.nt & 1;
+    const int par = s.d.nt % s.d.nlev;
     HIPCHK(hipMemcpyAsync(buf.data(), s.d.f[par][field], m.elems * sizeof(float),
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

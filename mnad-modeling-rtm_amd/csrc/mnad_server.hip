@@ -58,6 +58,7 @@ struct Request {
     bool cancel = false;
     double t_submit = 0, t_admit = 0, t_done = 0;
     size_t bytes = 0;
+    size_t points = 0;   // grid points stepping while the request is active
     // device side
     std::unique_ptr<Model> mt, ms;
     std::vector<std::unique_ptr<Shot>> shots;   // the shots currently stepping
@@ -83,6 +84,7 @@ struct Server {
     std::vector<Request*> active;
     std::map<int64_t, std::unique_ptr<Request>> all;
     size_t budget = 0, reserved = 0;
+    size_t max_points = 0, active_points = 0;
     int max_active = 64, chunk = 32;
     bool stop = false;
     int64_t next_id = 1;
@@ -134,6 +136,7 @@ void Server::complete(Request* r, int rc, const char* msg)
     r->state = DONE;
     r->t_done = now_s();
     reserved -= std::min(reserved, r->bytes);
+    if (r->t_admit > 0) active_points -= std::min(active_points, r->points);
     cv_done.notify_all();
 }
 
@@ -326,19 +329,20 @@ void Server::run()
             // cancellations of active requests take effect at this boundary
             for (Request* r : active)
                 if (r->cancel) r->rc = MNAD_ERR_CANCELLED;
-            // admission: submission order, skipping requests that do not fit yet
-            for (auto it = queued.begin(); it != queued.end();) {
-                Request* r = *it;
+            // admission: first come, first served, while memory, slots and
+            // the work limit allow (the first request of an idle server always fits)
+            while (!queued.empty()) {
+                Request* r = queued.front();
                 if ((int)(active.size() + admitted.size()) >= max_active) break;
-                if (reserved + r->bytes > budget) {
-                    ++it;
-                    continue;
-                }
+                if (reserved + r->bytes > budget) break;
+                if (max_points && active_points > 0 && active_points + r->points > max_points)
+                    break;
                 reserved += r->bytes;
+                active_points += r->points;
                 r->state = ACTIVE;
                 r->t_admit = now_s();
                 admitted.push_back(r);
-                it = queued.erase(it);
+                queued.pop_front();
             }
         }
         // drop cancelled active requests
@@ -412,7 +416,7 @@ void server_destroy(mnad_ctx* ctx)
 using namespace mnad;
 
 extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_active,
-                                int chunk_steps)
+                                uint64_t max_points, int chunk_steps)
 {
     if (!ctx) return fail(MNAD_ERR_ARG, "context is NULL");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -423,14 +427,17 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
         delete s;
         return rc;
     }
-    s->c->kernel = ctx->kernel;
-    s->c->rz = ctx->rz;
-    s->c->brows = ctx->brows;
+    // the server steps one time level per launch (admission at any step)
+    const bool two = ctx->kernel == 3;
+    s->c->kernel = two ? 0 : ctx->kernel;
+    s->c->rz = two ? 1 : ctx->rz;
+    s->c->brows = two ? 4 : ctx->brows;
     s->c->strip_first = ctx->strip_first;
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) (void)hipGetLastError();
     s->budget = hbm_budget ? (size_t)hbm_budget : (size_t)(0.8 * (double)freeb);
     s->max_active = max_active > 0 ? max_active : 64;
+    s->max_points = (size_t)max_points;
     s->chunk = chunk_steps > 0 ? chunk_steps : 32;
     rc = dalloc(s->c, s->dbase, sizeof(int), true);
     if (rc == MNAD_OK && hipStreamSynchronize(s->c->stream) != hipSuccess)
@@ -514,6 +521,7 @@ extern "C" int mnad_submit(mnad_ctx* ctx, const mnad_request* q, int64_t* ticket
         if (rc != MNAD_OK) return rc;
     }
     r->bytes = request_bytes(r->spec);
+    r->points = (size_t)g.nz * g.nx * (q->kind == MNAD_REQ_RTM ? 2 : 1);
     std::lock_guard<std::mutex> lk(s->mu);
     if (r->bytes > s->budget)
         return fail(MNAD_ERR_BUDGET, "request needs %zu bytes, the HBM budget is %zu", r->bytes,

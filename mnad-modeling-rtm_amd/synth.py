@@ -113,6 +113,12 @@ def model_c3(seed: int = 3, nz: int = 751, nx: int = 3401, dx: float = 10.0):
     return V, gaussian_smooth(V, 100.0, dx)
 
 
+def model_c5(seed: int = 5, nz: int = 2001, nx: int = 8001, dx: float = 12.5):
+    """Large long-recording model: the faulted layered + high-velocity-body
+    synthetic of model_c3 on an 8001 x 2001 grid at 12.5 m."""
+    return model_c3(seed=seed, nz=nz, nx=nx, dx=dx)
+
+
 def model_layered_random(rng, nz, nx, dx, vmin=1500.0, vmax=4500.0):
     """Layered-plus-random model of the serving workload (1500-4500 m/s)."""
     z = np.arange(nz) * dx

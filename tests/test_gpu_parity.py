@@ -182,7 +182,7 @@ def test_invalid_inputs_raise(mnad, ctx):
 
 
 @pytest.mark.parametrize("kernel,rz,br", [(0, 1, 4), (0, 2, 4), (0, 4, 4), (0, 1, 8), (0, 1, 16),
-                                          (1, 1, 4), (2, 1, 4), (2, 2, 4)])
+                                          (1, 1, 4), (2, 1, 4), (2, 2, 4), (3, 1, 16)])
 @pytest.mark.parametrize("strip_first", [0, 1])
 def test_launch_shapes_agree(mnad, ctx, oracle, kernel, rz, br, strip_first):
     """Every kernel family / rows-per-thread / dispatch-order variant of the
@@ -213,7 +213,7 @@ def test_autotune_and_serial_paths(mnad, ctx):
     g = _grid(mnad, f, "vs")
     tuning, per_launch, us = ctx.autotune(g, f["vs"], max_shots=4)
     assert per_launch in (1, 2, 4) and us > 0
-    assert tuning["kernel"] in (0, 2)
+    assert tuning["kernel"] in (0, 2, 3)
     n = int(f["nshots"])
     zs = [int(f[f"zs{k}"]) for k in range(n)]
     xs = [int(f[f"xs{k}"]) for k in range(n)]

@@ -41,7 +41,7 @@ def _rtm_req(mnad, name, k=0, fmt=None):
 @pytest.fixture()
 def server(mnad):
     ctx = mnad.Context(0)
-    ctx.serve_start(hbm_budget=0, max_active=64, chunk_steps=16)
+    ctx.serve_start(hbm_budget=0, max_active=64, chunk_steps=16, max_points=0)
     yield ctx
     ctx.serve_stop()
     ctx.close()
@@ -137,6 +137,23 @@ def test_budget_admission_cancel_and_rejection(mnad):
         with pytest.raises(mnad.MnadError) as e:
             _submit(ctx, cfl)
         assert e.value.code == mnad.ERR_ARG
+        ctx.serve_stop()
+    finally:
+        ctx.close()
+
+
+def test_work_limit_admits_first_come_first_served(mnad):
+    """max_points bounds the grid points stepping at once: with room for one
+    request, requests run one after another in submission order."""
+    _say("serving: work limit")
+    f, r = _model_req(mnad, "fwd_square")
+    pts = r["grid"].nz * r["grid"].nx
+    ctx = mnad.Context(0)
+    try:
+        ctx.serve_start(max_points=pts, chunk_steps=16)
+        infos = [ctx.wait(t)[1] for t in [_submit(ctx, r) for _ in range(3)]]
+        for a, b in zip(infos, infos[1:]):
+            assert b["admit_s"] >= a["done_s"] - 1e-3
         ctx.serve_stop()
     finally:
         ctx.close()

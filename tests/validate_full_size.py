@@ -1,4 +1,5 @@
-"""Full-size parity check of the C2 workload (561 x 1061 padded, nt = 4000)
+"""Full-size parity check of the C2 workload (561 x 1061 padded, nt = 4000),
+or of the 1001 x 501 window of the C5 model (--workload c5crop, nt = 12000),
 against the FP64 oracle.  Not collected by pytest (minutes of CPU): run it
 on the outputs `bench.py --save-outputs FILE` wrote on the GPU:
 
@@ -23,11 +24,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 
 
-def _forward(args):
-    which, nt, snap = args
+def _workload(name, nt):
     import bench
+    return bench.c5_workload(nt, 1, crop=True) if name == "c5crop" else bench.c2_workload(nt)
+
+
+def _forward(args):
+    which, nt, snap, name = args
     from oracle import oracle as O
-    wl = bench.c2_workload(nt)
+    wl = _workload(name, nt)
     v = wl["v"] if which == "true" else wl["vs"]
     g = O.grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
     t0 = time.time()
@@ -46,12 +51,13 @@ def main():
     ap.add_argument("outputs")
     ap.add_argument("more", nargs="*", help="further output files (other snapshot formats)")
     ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c5crop"])
     args = ap.parse_args()
-    import bench
     from oracle import oracle as O
     with Pool(2) as pool:
-        obs, syn = pool.map(_forward, [("true", args.nt, False), ("smooth", args.nt, True)])
-    wl = bench.c2_workload(args.nt)
+        obs, syn = pool.map(_forward, [("true", args.nt, False, args.workload),
+                                       ("smooth", args.nt, True, args.workload)])
+    wl = _workload(args.workload, args.nt)
     vs = wl["vs"]
     g = O.grid(vs.shape[0], vs.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
     t0 = time.time()
@@ -62,7 +68,8 @@ def main():
     img = Im.reshape(nzi, nxi) / ill
     for path in [args.outputs] + args.more:
         gpu = np.load(path)
-        res = {"workload": "C2 561x1061 nt=%d snapshots=%s" % (args.nt, str(gpu["snap"]))}
+        res = {"workload": "%s %dx%d nt=%d snapshots=%s" % (args.workload, g.nz, g.nx, args.nt,
+                                                            str(gpu["snap"]))}
         if "dobs" in gpu:
             res.update(
                 gather_obs_rel_l2=rel(gpu["dobs"], obs["gather"]),
