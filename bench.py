@@ -32,6 +32,14 @@ METRIC = "Gpt-updates/s & RTM shots/hr vs plain FP32 at ≤1e-4 rel err; p50/p99
 HBM_PEAK_GBS = 8000.0
 
 
+def kernel_name(t):
+    """Device symbol (as rocprofv3 prints it) of the step kernel a tuning runs."""
+    k, r, b = t["kernel"], t["rows_per_thread"], t["block_rows"]
+    return {0: f"mnad::k_step<{r}, {b}>", 1: "mnad::k_step_pf", 2: f"mnad::k_step_lds<{4 * r}>",
+            3: "mnad::k_step2", 4: f"mnad::k_stepw<{b}, false>", 5: f"mnad::k_stepw<{b}, true>",
+            6: f"mnad::k_stepx<{b}>"}.get(k, f"kernel {k}")
+
+
 def c2_workload(nt, npml=30):
     from __graft_entry__ import load_package
     load_package()
@@ -166,7 +174,8 @@ def run_multi(args, mnad, wl, label):
         "autotune_s": round(tune_s, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_step", "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 3),
+                     "kernel": kernel_name(tuning),
+                     "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 3),
                      "launches": int(k_launches)},
         "image_finite": bool(np.isfinite(img).all()),
     }
@@ -384,6 +393,7 @@ def main():
 
     with mnad.Context(0) as ctx:
         sid = ctx.stack_create(g)
+        kname = kernel_name(ctx.tuning())
 
         def shot(flags=0):
             ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], [wl["zs"]], [wl["xs"]], recs,
@@ -447,7 +457,7 @@ def main():
         "shots_per_hour": round(3600.0 / (ms_per_step / 1e3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_step", "avg_launch_us": round(avg_launch_us, 3),
+                     "kernel": kname, "avg_launch_us": round(avg_launch_us, 3),
                      "launches": int(k_launches)},
     }
     if plain_ms is not None:

@@ -168,6 +168,14 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  * kernel 2 = block_rows 4, tiles + halos staged in LDS, rows_per_thread 1 or 2;
  * kernel 3 = two time steps per launch (block_rows 16, 1 row; 60 x 12 output
  *   tiles), shots keep four time levels -- set it before creating shots;
+ * kernel 4 (default) = one grid row per wavefront, row origins in scalar
+ *   registers and one shared lane offset for every load and store: block_rows
+ *   1, 2, 4 or 8, 1 row;
+ * kernel 5 = kernel 4 with the CPML loads of strip points issued first:
+ *   block_rows 1 or 4, 1 row;
+ * kernel 6 = kernel 4 on the CPML strips, four columns per lane (16-byte
+ *   loads, neighbours by cross-lane shifts) in the interior: block_rows 1, 2
+ *   or 4, 1 row;
  * graph_chunk = step launches per captured hipGraph (<= 1: no graph);
  * strip_first != 0 dispatches the CPML-strip tiles of each XCD first. */
 typedef struct {

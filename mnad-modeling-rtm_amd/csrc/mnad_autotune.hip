@@ -24,7 +24,7 @@ struct Variant {
     int kernel, rz, brows;
 };
 
-const Variant kVariants[] = {{0, 1, 4}, {0, 2, 4}, {0, 4, 4}, {2, 1, 4}, {3, 1, 16}};
+const Variant kVariants[] = {{4, 1, 4}, {4, 1, 1}, {0, 1, 4}, {0, 2, 4}, {6, 1, 2}, {3, 1, 16}};
 
 // Time `steps` steps of `b` forward shots (source at the centre, no
 // receivers, no snapshots) with the context's current tuning; returns

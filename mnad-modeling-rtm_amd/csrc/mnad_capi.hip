@@ -50,6 +50,9 @@ static bool tuning_ok(int kernel, int rows, int brows)
     if (kernel == 1) return brows == 4 && rows == 1;
     if (kernel == 2) return brows == 4 && (rows == 1 || rows == 2);
     if (kernel == 3) return brows == 16 && rows == 1;
+    if (kernel == 4) return rows == 1 && (brows == 1 || brows == 2 || brows == 4 || brows == 8);
+    if (kernel == 5) return rows == 1 && (brows == 1 || brows == 4);
+    if (kernel == 6) return rows == 1 && (brows == 1 || brows == 2 || brows == 4);
     return false;
 }
 
@@ -90,7 +93,7 @@ int mnad_ctx_create(int device, mnad_ctx** out)
     if (const char* s = getenv("MNAD_RZ")) c->rz = atoi(s);
     if (const char* s = getenv("MNAD_BROWS")) c->brows = atoi(s);
     if (!tuning_ok(c->kernel, c->rz, c->brows)) {
-        c->kernel = 0;
+        c->kernel = 4;
         c->rz = 1;
         c->brows = 4;
     }

@@ -28,6 +28,10 @@ struct ShotDev {
     int mode;                       // MODE_FWD / MODE_ADJ
     int t0;                         // global step index of this shot's step 0
     int ntx;                        // tiles along x (for tile -> (tj, ti))
+    // four-column interior tiles (step kernel 6): one-point tiles are numbered
+    // below nt1, interior tiles from nt1 on, nx4 per band of rows; they update
+    // columns [npml, x4hi) of rows [npml, nz - npml), starting at x4lo
+    int nt1, nx4, x4lo, x4hi;
     float kap;                      // rz / rx = dx^2 / dz^2
     float inv_dt2;                  // 1 / dt^2 (snapshot d2u)
     const float* rx;                // a / dx^2 per point, grid storage

@@ -92,7 +92,8 @@ struct mnad_ctx {
     std::map<int, mnad::Stack> stacks;
     int next_stack = 1;
     int graph_chunk = 64;
-    int kernel = 0;             // step kernel: 0 registers, 1 prefetch, 2 LDS, 3 two steps
+    int kernel = 4;             // step kernel: 0 registers, 1 prefetch, 2 LDS, 3 two steps,
+                                // 4 wave rows, 5 wave rows + CPML prefetch, 6 four columns
     int rz = 1;                 // rows per thread
     int brows = 4;              // thread rows per block (tile = 64 x brows*rz)
     int strip_first = 1;        // dispatch CPML-strip tiles first

@@ -95,22 +95,25 @@ __device__ __forceinline__ void load9(const float* __restrict__ F, long p, int P
     o.sw = F[p + P - 1]; o.s = F[p + P]; o.se = F[p + P + 1];
 }
 
+// Snapshot element store / load: a two-way branch on the storage width,
+// the half / bfloat16 choice as a select (see snap_image_w).
 __device__ __forceinline__ void store_snap(const ShotDev& d, long idx, float x)
 {
     if (d.sfmt == SNAP_F32) {
         static_cast<float*>(d.snaps)[idx] = x;
-    } else if (d.sfmt == SNAP_F16) {
-        static_cast<__half*>(d.snaps)[idx] = __float2half(x * d.sscale);
     } else {
-        static_cast<__hip_bfloat16*>(d.snaps)[idx] = __float2bfloat16(x);
+        const unsigned short h16 = __half_as_ushort(__float2half(x * d.sscale));
+        const unsigned short b16 = __bfloat16_as_ushort(__float2bfloat16(x));
+        static_cast<unsigned short*>(d.snaps)[idx] = d.sfmt == SNAP_F16 ? h16 : b16;
     }
 }
 
 __device__ __forceinline__ float load_snap(const ShotDev& d, long idx)
 {
     if (d.sfmt == SNAP_F32) return static_cast<const float*>(d.snaps)[idx];
-    if (d.sfmt == SNAP_F16) return __half2float(static_cast<const __half*>(d.snaps)[idx]) * d.sinv;
-    return __bfloat162float(static_cast<const __hip_bfloat16*>(d.snaps)[idx]);
+    const unsigned short h = static_cast<const unsigned short*>(d.snaps)[idx];
+    return d.sfmt == SNAP_F16 ? __half2float(__ushort_as_half(h)) * d.sinv
+                              : __bfloat162float(__ushort_as_bfloat16(h));
 }
 
 // Three CPML memory triples (U, Vs, Ws) of one strip point along one axis.
@@ -215,8 +218,8 @@ __device__ __forceinline__ void snap_or_image(const ShotDev& d, int s, int j, in
                 // d2u = (nU - 2U + pU)/dt^2 = du/dt^2; snapshot k = (s+1)/T - 1.
                 const int k = (s + 1) / d.T - 1;
                 const float d2u = du * d.inv_dt2;
-                store_snap(d, (long)k * ni + q, d2u);
                 d.illum[q] += d2u * d2u;
+                store_snap(d, (long)k * ni + q, d2u);
             }
         } else {
             const int tt = d.nt - s;
@@ -511,6 +514,457 @@ __global__ __launch_bounds__(BX * BZ) void k_step_pf(const ShotDev* __restrict__
     PV[p] = nv;
     PW[p] = nw;
     if (d.T > 0 && !inx && !inz) snap_or_image(d, s, j, i, du, nu);
+}
+
+// ------------------------------------------------------ wave-row step kernel
+// Same update as k_step<1, BZR>, laid out for the gfx950 addressing modes: a
+// wavefront owns 64 consecutive points of ONE grid row, so every origin it
+// needs (field row, CPML plane row, snapshot row) is wave-uniform and lives
+// in SGPRs, and every load or store is
+//     global_load_dword v, v_lane, s[origin] offset:imm
+// with a single lane-offset VGPR shared by all of them (the compiler also
+// merges the -1/0/+1 neighbours of a row into one dwordx3 load).  The
+// descriptors hold generic pointers; they are cast to the global address
+// space, which would otherwise compile to flat instructions with 64-bit
+// per-lane address arithmetic.  All CPML loads of a point are issued before
+// its stores.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) int gint;
+typedef __attribute__((address_space(1))) unsigned short gu16;
+
+__device__ __forceinline__ gfloat* gp(float* p) { return (gfloat*)p; }
+__device__ __forceinline__ const gfloat* gp(const float* p) { return (const gfloat*)p; }
+__device__ __forceinline__ const gint* gp(const int* p) { return (const gint*)p; }
+
+// A wave-uniform pointer made opaque (readfirstlane of both halves), so that
+// the compiler keeps it in SGPRs and does not fold a per-lane index into
+// 64-bit VALU address arithmetic.
+template <class T>
+__device__ __forceinline__ T* uni(T* p)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
+// Element at a 32-bit byte offset from a global base (SGPR base + VGPR offset).
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ gfloat& at(gfloat* base, unsigned off)
+{
+    return *(gfloat*)((gchar*)base + off);
+}
+__device__ __forceinline__ const gfloat& at(const gfloat* base, unsigned off)
+{
+    return *(const gfloat*)((const gchar*)base + off);
+}
+
+// CPML state of one strip point along one axis: coefficients (c1, h c2, c3)
+// and the nine memories (field * 3 + level).  ps/cf are the plane origins of
+// the point's strip row (wave-uniform), b = 4 q with q the lane's column
+// within that row (q < 2 npml or q < 64, so b fits the 32-bit offset of the
+// SGPR-base form).
+struct PsiW {
+    float c[3], p[9];
+};
+
+__device__ __forceinline__ void psi_load(PsiW& m, const gfloat* ps, const gfloat* cf, long ns,
+                                         unsigned b)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m.c[k] = at(uni(cf + k * ns), b);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m.p[k] = at(uni(ps + k * ns), b);
+}
+
+__device__ __forceinline__ void psi_step(PsiW& m, float r, float ud, float uc, float ua, float vd,
+                                         float vc, float va, float wd, float wc, float wa,
+                                         float& du, float& dv, float& dw)
+{
+    du -= r * mnad_psi(m.p[0], m.p[1], m.p[2], ud, uc, ua, m.c[0], m.c[1], m.c[2]);
+    dv -= r * mnad_psi(m.p[3], m.p[4], m.p[5], vd, vc, va, m.c[0], m.c[1], m.c[2]);
+    dw -= r * mnad_psi(m.p[6], m.p[7], m.p[8], wd, wc, wa, m.c[0], m.c[1], m.c[2]);
+}
+
+__device__ __forceinline__ void psi_store(const PsiW& m, gfloat* ps, long ns, unsigned b)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) at(uni(ps + k * ns), b) = m.p[k];
+}
+
+// d2u snapshot + illumination (forward) or imaging (adjoint) of one interior
+// point, on the steps where the schedule takes them.  qrow: wave-uniform
+// offset of the point's interior row, q: its interior column.
+__device__ __forceinline__ void snap_image_w(const ShotDev& d, int s, long qrow, unsigned q,
+                                             float du, float nu)
+{
+    // Two-way branches on the storage width only (32 or 16 bits) with the
+    // half/bfloat16 choice as a select: written as a three-way if/else chain,
+    // hipcc compiled the bfloat16 store path so that the illumination update
+    // after it read its column offset from a register no path had written.
+    const long ni = (long)(d.nz - 2 * d.npml) * (d.nx - 2 * d.npml);
+    const bool f32 = d.sfmt == SNAP_F32, f16 = d.sfmt == SNAP_F16;
+    if (d.mode == MODE_FWD) {
+        if ((s + 1) % d.T == 0) {
+            // d2u = (nU - 2U + pU)/dt^2 = du/dt^2; snapshot k = (s+1)/T - 1.
+            const long k0 = (long)((s + 1) / d.T - 1) * ni + qrow;
+            const float d2u = du * d.inv_dt2;
+            gfloat* il = gp(d.illum) + qrow;
+            il[q] += d2u * d2u;
+            if (f32) {
+                (gp(static_cast<float*>(d.snaps)) + k0)[q] = d2u;
+            } else {
+                const unsigned short h16 = __half_as_ushort(__float2half(d2u * d.sscale));
+                const unsigned short b16 = __bfloat16_as_ushort(__float2bfloat16(d2u));
+                ((gu16*)d.snaps + k0)[q] = f16 ? h16 : b16;
+            }
+        }
+    } else {
+        const int tt = d.nt - s;
+        if (tt % d.T == 0) {
+            // UU(:, ncol - ttt + 1) = snapshot column shift + tt/T (1-based).
+            const int col = d.shift + tt / d.T;
+            if (col <= d.K) {
+                const long k0 = (long)(col - 1) * ni + qrow;
+                float sn;
+                if (f32) {
+                    sn = (gp(static_cast<const float*>(d.snaps)) + k0)[q];
+                } else {
+                    const unsigned short h = ((const gu16*)d.snaps + k0)[q];
+                    sn = f16 ? __half2float(__ushort_as_half(h)) * d.sinv
+                             : __bfloat162float(__ushort_as_bfloat16(h));
+                }
+                const float x = sn * nu;
+                gfloat* im = gp(d.img) + qrow;
+                gfloat* ic = gp(d.imgc) + qrow;
+                const float sum = im[q], c = ic[q];
+                const float y = x - c;
+                const float t = sum + y;
+                ic[q] = (t - sum) - y;
+                im[q] = t;
+            }
+        }
+    }
+}
+
+// One point per lane: the wave owns columns i0 .. i0+63 of row j.
+// PF: issue the CPML loads of strip points together with the neighbourhood
+// loads (one memory round trip for a strip point instead of two, at the cost
+// of the registers that hold them across the increment arithmetic).
+// SKIPX4: points of the four-column region (rows npml .. nz-npml-1, columns
+// npml .. x4hi-1) belong to step_x4 and are left alone.
+template <bool PF, bool SKIPX4>
+__device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0, unsigned lane)
+{
+    const int i = i0 + (int)lane;
+    if (i >= d.nx) return;
+    if (SKIPX4 && j >= d.npml && j < d.nz - d.npml && i >= d.npml && i < d.x4hi) return;
+    const int np = d.npml;
+    const bool inx = (i < np) || (i >= d.nx - np);
+    const bool inz = (j < np) || (j >= d.nz - np);   // wave-uniform
+    // strip-row origins: x strips [9][nz * 2 npml], z strips [9][2 npml * nx]
+    const long nsx = (long)d.nz * 2 * np, rowx = (long)j * 2 * np;
+    const unsigned bx = 4u * ((i < np) ? (unsigned)i : (unsigned)(i - (d.nx - 2 * np)));
+    const long nsz = (long)2 * np * d.nx;
+    const long rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.nx + i0;
+    PsiW mx, mz;
+    if (PF) {
+        if (inz) psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
+        if (inx) psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
+    }
+
+    const int cb = s & 1;
+    const long P = d.pitch;
+    const long org = (long)(j + MNAD_ZO) * P + (i0 + MNAD_XO);   // storage of (j, i0)
+    const gfloat* U = gp(d.f[cb][0]) + org;
+    const gfloat* V = gp(d.f[cb][1]) + org;
+    const gfloat* W = gp(d.f[cb][2]) + org;
+    Nbh u, v, w;
+    float vww, vee, wnn, wss;
+    {
+        const gfloat* a = (U - P) + lane;
+        u.nw = a[-1]; u.n = a[0]; u.ne = a[1];
+        a = U + lane;
+        u.w = a[-1];  u.c = a[0]; u.e = a[1];
+        a = (U + P) + lane;
+        u.sw = a[-1]; u.s = a[0]; u.se = a[1];
+        a = (V - P) + lane;
+        v.nw = a[-1]; v.n = a[0]; v.ne = a[1];
+        a = V + lane;
+        vww = a[-2];  v.w = a[-1]; v.c = a[0]; v.e = a[1]; vee = a[2];
+        a = (V + P) + lane;
+        v.sw = a[-1]; v.s = a[0]; v.se = a[1];
+        wnn = ((W - 2 * P) + lane)[0];
+        a = (W - P) + lane;
+        w.nw = a[-1]; w.n = a[0]; w.ne = a[1];
+        a = W + lane;
+        w.w = a[-1];  w.c = a[0]; w.e = a[1];
+        a = (W + P) + lane;
+        w.sw = a[-1]; w.s = a[0]; w.se = a[1];
+        wss = ((W + 2 * P) + lane)[0];
+    }
+    const float rx = (gp(d.rx) + org)[lane];
+    gfloat* PU = gp(d.f[cb ^ 1][0]) + org;
+    gfloat* PV = gp(d.f[cb ^ 1][1]) + org;
+    gfloat* PW = gp(d.f[cb ^ 1][2]) + org;
+    const float pu = PU[lane], pv = PV[lane], pw = PW[lane];
+
+    int r = -1;
+    if (d.nrec > 0) {
+        const int off = __builtin_amdgcn_readfirstlane(d.rrow[j]);   // wave-uniform
+        if (off >= 0) r = uni(gp(d.rcol) + off + i0)[lane];
+    }
+
+    const float rz = rx * d.kap;
+    float du, dv, dw;
+    mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+
+    if (inx || inz) {
+        // The CPML memories see the pre-injection U (fwd_PDE_exact_MNAD.m:95-111).
+        float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
+        if (d.mode == MODE_FWD && d.src_j >= 0) {
+            const float a0 = d.src_amp[s];
+            if (j == d.src_j) {
+                if (i == d.src_i) uc -= a0;
+                if (i - 1 == d.src_i) uw -= a0;
+                if (i + 1 == d.src_i) ue -= a0;
+            }
+            if (i == d.src_i) {
+                if (j - 1 == d.src_j) un -= a0;
+                if (j + 1 == d.src_j) us -= a0;
+            }
+        }
+        if (inx) {
+            if (!PF) psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
+            psi_step(mx, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+            psi_store(mx, gp(d.psx) + rowx, nsx, bx);
+        }
+        if (inz) {
+            if (!PF) psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
+            psi_step(mz, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+            psi_store(mz, gp(d.psz) + rowz, nsz, 4u * lane);
+        }
+    }
+
+    float nu = (2.0f * u.c + du) - pu;
+    const float nv = (2.0f * v.c + dv) - pv;
+    const float nw = (2.0f * w.c + dw) - pw;
+    if (d.mode == MODE_FWD) {
+        if (j == d.src_j && i == d.src_i) nu += d.src_amp[s + 1];
+        if (r >= 0) (gp(d.gather) + (long)s * d.nrec)[(unsigned)r] = u.c;
+    } else if (r >= 0) {
+        nu -= (gp(d.radj) + (long)s * d.nrec)[(unsigned)r];
+    }
+    PU[lane] = nu;
+    PV[lane] = nv;
+    PW[lane] = nw;
+
+    if (d.T > 0 && !inx && !inz)
+        snap_image_w(d, s, (long)(j - np) * (d.nx - 2 * np), (unsigned)(i - np), du, nu);
+}
+
+template <int BZR, bool PF>
+__global__ __launch_bounds__(BX * BZR) void k_stepw(const ShotDev* __restrict__ shots,
+                                                    const TileRef* __restrict__ tiles,
+                                                    const int* __restrict__ base, int rel)
+{
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int j = tj * BZR + __builtin_amdgcn_readfirstlane((int)threadIdx.y);   // wave row
+    if (j >= d.nz) return;
+    step_one<PF, false>(d, s, j, ti * BX, threadIdx.x);
+}
+
+// ------------------------------------------------ four columns per lane
+// The interior of a shot (rows npml .. nz-npml-1, columns npml .. nx-npml-1:
+// no CPML there) in waves of 64 lanes x 4 columns.  Lane l holds columns
+// i0 + 4l .. i0 + 4l + 3 of row j, one 16-byte load per field row, and takes
+// the columns just left / right of its four from lanes l-1 / l+1 with DPP
+// shifts across the wavefront (wave_shr:1 / wave_shl:1).  Lanes 0 and 63
+// only supply those halo columns, so a wave updates 248 columns with 15
+// loads and 3 stores per lane, against 64 columns with 15 loads for one
+// point per lane: a quarter of the vector-memory instructions and of the
+// redundant row reads.  All 64 lanes stay active through the loads and the
+// shifts.  Lane 0's first column (i0) is a multiple of 4, so every row load
+// is 16-byte aligned.
+constexpr int X4W = 4 * (BX - 2);   // columns updated per wave
+
+__device__ __forceinline__ float lane_prev(float x)   // lane l-1's x (0 in lane 0)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x),
+                                                                 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float lane_next(float x)   // lane l+1's x (0 in lane 63)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x),
+                                                                 0x130, 0xf, 0xf, false));
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) f32x4 gf32x4;
+
+// Columns -1 .. 4 of a lane's four.
+struct Row4 {
+    float e[4], l, r;
+    __device__ __forceinline__ float at(int c) const { return c < 0 ? l : (c > 3 ? r : e[c]); }
+};
+
+__device__ __forceinline__ Row4 row4(const gfloat* base, unsigned lane)
+{
+    const f32x4 x = ((const gf32x4*)base)[lane];
+    Row4 o;
+    o.e[0] = x.x; o.e[1] = x.y; o.e[2] = x.z; o.e[3] = x.w;
+    o.l = lane_prev(x.w);
+    o.r = lane_next(x.x);
+    return o;
+}
+
+__device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, int lo, int hi,
+                                        unsigned lane)
+{
+    const int cb = s & 1;
+    const long P = d.pitch;
+    const long org = (long)(j + MNAD_ZO) * P + (i0 + MNAD_XO);   // storage of (j, i0)
+    const gfloat* U = gp(d.f[cb][0]) + org;
+    const gfloat* V = gp(d.f[cb][1]) + org;
+    const gfloat* W = gp(d.f[cb][2]) + org;
+    const Row4 un = row4(U - P, lane), uc = row4(U, lane), us = row4(U + P, lane);
+    const Row4 vn = row4(V - P, lane), vc = row4(V, lane), vs = row4(V + P, lane);
+    const Row4 wn = row4(W - P, lane), wc = row4(W, lane), ws = row4(W + P, lane);
+    const f32x4 wnn = ((const gf32x4*)(W - 2 * P))[lane];
+    const f32x4 wss = ((const gf32x4*)(W + 2 * P))[lane];
+    const float vl2 = lane_prev(vc.e[2]), vr2 = lane_next(vc.e[1]);   // columns -2, +5
+    const f32x4 rx4 = ((const gf32x4*)(gp(d.rx) + org))[lane];
+    float nu[4], nv[4], nw[4], dus[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        Nbh u, v, w;
+        u.nw = un.at(e - 1); u.n = un.e[e]; u.ne = un.at(e + 1);
+        u.w = uc.at(e - 1);  u.c = uc.e[e]; u.e = uc.at(e + 1);
+        u.sw = us.at(e - 1); u.s = us.e[e]; u.se = us.at(e + 1);
+        v.nw = vn.at(e - 1); v.n = vn.e[e]; v.ne = vn.at(e + 1);
+        v.w = vc.at(e - 1);  v.c = vc.e[e]; v.e = vc.at(e + 1);
+        v.sw = vs.at(e - 1); v.s = vs.e[e]; v.se = vs.at(e + 1);
+        w.nw = wn.at(e - 1); w.n = wn.e[e]; w.ne = wn.at(e + 1);
+        w.w = wc.at(e - 1);  w.c = wc.e[e]; w.e = wc.at(e + 1);
+        w.sw = ws.at(e - 1); w.s = ws.e[e]; w.se = ws.at(e + 1);
+        const float vww = e >= 2 ? vc.e[e - 2] : (e == 1 ? vc.l : vl2);
+        const float vee = e <= 1 ? vc.e[e + 2] : (e == 2 ? vc.r : vr2);
+        const float rx = rx4[e];
+        float du, dv, dw;
+        mnad_incr(rx, rx * d.kap, u, v, vww, vee, w, wnn[e], wss[e], du, dv, dw);
+        nu[e] = 2.0f * u.c + du;
+        nv[e] = 2.0f * v.c + dv;
+        nw[e] = 2.0f * w.c + dw;
+        dus[e] = du;
+        // one column at a time: interleaving the four raises the register count
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // the previous level, loaded after the arithmetic (fewer live registers)
+    gf32x4* PU = (gf32x4*)(gp(d.f[cb ^ 1][0]) + org);
+    gf32x4* PV = (gf32x4*)(gp(d.f[cb ^ 1][1]) + org);
+    gf32x4* PW = (gf32x4*)(gp(d.f[cb ^ 1][2]) + org);
+    {
+        const f32x4 pu = PU[lane], pv = PV[lane], pw = PW[lane];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            nu[e] -= pu[e];
+            nv[e] -= pv[e];
+            nw[e] -= pw[e];
+        }
+    }
+
+    // owned columns: [lo, hi) of this tile, lanes 1 .. 62
+    const int c0 = i0 + 4 * (int)lane;
+    if (lane < 1 || lane > BX - 2 || c0 + 3 < lo || c0 >= hi) return;
+    if (d.mode == MODE_FWD && j == d.src_j) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (c0 + e == d.src_i) nu[e] += d.src_amp[s + 1];
+    }
+    if (d.nrec > 0) {
+        const int off = __builtin_amdgcn_readfirstlane(d.rrow[j]);   // wave-uniform
+        if (off >= 0) {
+            const gint* rc = uni(gp(d.rcol) + off);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = c0 + e;
+                if (c < lo || c >= hi) continue;
+                const int r = rc[(unsigned)c];
+                if (r < 0) continue;
+                if (d.mode == MODE_FWD)
+                    (gp(d.gather) + (long)s * d.nrec)[(unsigned)r] = uc.e[e];
+                else
+                    nu[e] -= (gp(d.radj) + (long)s * d.nrec)[(unsigned)r];
+            }
+        }
+    }
+    if (c0 >= lo && c0 + 3 < hi) {
+        PU[lane] = (f32x4){nu[0], nu[1], nu[2], nu[3]};
+        PV[lane] = (f32x4){nv[0], nv[1], nv[2], nv[3]};
+        PW[lane] = (f32x4){nw[0], nw[1], nw[2], nw[3]};
+    } else {
+        gfloat* qu = (gfloat*)(PU + lane);
+        gfloat* qv = (gfloat*)(PV + lane);
+        gfloat* qw = (gfloat*)(PW + lane);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (c0 + e < lo || c0 + e >= hi) continue;
+            qu[e] = nu[e];
+            qv[e] = nv[e];
+            qw[e] = nw[e];
+        }
+    }
+    if (d.T > 0) {
+        const int np = d.npml;
+        const long qrow = (long)(j - np) * (d.nx - 2 * np);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int c = c0 + e;
+            if (c < lo || c >= hi) continue;
+            snap_image_w(d, s, qrow, (unsigned)(c - np), dus[e], nu[e]);
+        }
+    }
+}
+
+// Kernel 6: CPML strips with one point per lane, the interior with four.
+// Tiles below nt1 are BX x BZR one-point tiles (tile = tj * ntx + ti, only
+// those that reach outside the interior are listed); tile nt1 + tb * nx4 + tx
+// is the four-column tile of rows npml + tb*BZR .., updating columns
+// [x4lo + tx*X4W, x4lo + (tx+1)*X4W) clipped to [npml, nx - npml).
+template <int BZR>
+__global__ __launch_bounds__(BX * BZR) void k_stepx(const ShotDev* __restrict__ shots,
+                                                    const TileRef* __restrict__ tiles,
+                                                    const int* __restrict__ base, int rel)
+{
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;
+    const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    if (tr.tile < d.nt1) {
+        const int tj = tr.tile / d.ntx;
+        const int ti = tr.tile - tj * d.ntx;
+        const int j = tj * BZR + wy;
+        if (j >= d.nz) return;
+        step_one<false, true>(d, s, j, ti * BX, threadIdx.x);
+    } else {
+        const int u = tr.tile - d.nt1;
+        const int tb = u / d.nx4;
+        const int tx = u - tb * d.nx4;
+        const int j = d.npml + tb * BZR + wy;
+        if (j >= d.nz - d.npml) return;
+        const int lo = d.x4lo + tx * X4W;
+        step_x4(d, s, j, lo - 4, lo < d.npml ? d.npml : lo, min(lo + X4W, d.x4hi),
+                threadIdx.x);
+    }
 }
 
 // ---------------------------------------------------- two steps per launch
@@ -815,6 +1269,24 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL(k_step_lds<4>, g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 2 && brows == 4 && rows == 2)
         hipLaunchKernelGGL(k_step_lds<8>, g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 4 && brows == 1 && rows == 1)
+        hipLaunchKernelGGL((k_stepw<1, false>), g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
+    else if (kernel == 4 && brows == 2 && rows == 1)
+        hipLaunchKernelGGL((k_stepw<2, false>), g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
+    else if (kernel == 4 && brows == 4 && rows == 1)
+        hipLaunchKernelGGL((k_stepw<4, false>), g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 4 && brows == 8 && rows == 1)
+        hipLaunchKernelGGL((k_stepw<8, false>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
+    else if (kernel == 6 && brows == 1 && rows == 1)
+        hipLaunchKernelGGL(k_stepx<1>, g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
+    else if (kernel == 6 && brows == 2 && rows == 1)
+        hipLaunchKernelGGL(k_stepx<2>, g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
+    else if (kernel == 6 && brows == 4 && rows == 1)
+        hipLaunchKernelGGL(k_stepx<4>, g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 5 && brows == 1 && rows == 1)
+        hipLaunchKernelGGL((k_stepw<1, true>), g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
+    else if (kernel == 5 && brows == 4 && rows == 1)
+        hipLaunchKernelGGL((k_stepw<4, true>), g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 3 && brows == 16 && rows == 1)
         hipLaunchKernelGGL(k_step2, g, dim3(X2, Z2), 0, st, shots, tiles, base, rel);
     else

@@ -404,10 +404,28 @@ void tile_shape(const mnad_ctx* c, int* tw, int* th, int* spl)
         *th = 12;
         *spl = 2;
     } else {                // 64 columns x (block_rows * rows_per_thread) rows
+                            // (kernel 6: the one-point tiles; see build_tiles)
         *tw = 64;
         *th = c->brows * c->rz;
         *spl = 1;
     }
+}
+
+// Four-column interior tiles of step kernel 6 (X4W columns, th rows each).
+static void x4_layout(ShotDev& d, int th, int nt1)
+{
+    const int np = d.npml;
+    d.nt1 = nt1;
+    d.x4lo = np & ~3;           // lane 0 of the first tile starts at x4lo - 4: 16-byte aligned
+    d.x4hi = d.nx - np;
+    const int X4W = 248;
+    if (d.x4hi <= np || d.nz - 2 * np <= 0) {
+        d.nx4 = 0;
+        d.x4hi = np;            // no interior: the one-point tiles cover everything
+    } else {
+        d.nx4 = (d.x4hi - d.x4lo + X4W - 1) / X4W;
+    }
+    (void)th;
 }
 
 void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table)
@@ -416,15 +434,28 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
     std::vector<char> strip;
     int tw, th, spl;
     tile_shape(c, &tw, &th, &spl);
+    const bool x4 = c->kernel == 6;
     for (int k = 0; k < (int)h.size(); ++k) {
         const int ntx = (h[k].nx + tw - 1) / tw, ntz = (h[k].nz + th - 1) / th;
         const int np = h[k].npml;
         h[k].ntx = ntx;
+        x4_layout(h[k], th, ntx * ntz);
+        const int nz = h[k].nz, nx = h[k].nx, x4hi = x4 ? h[k].x4hi : np;
         for (int t = 0; t < ntx * ntz; ++t) {
-            list.push_back(TileRef{k, t});
             const int tj = t / ntx, ti = t - tj * ntx;
-            strip.push_back(ti * tw < np || ti * tw + tw - 1 >= h[k].nx - np || tj * th < np ||
-                            (tj + 1) * th - 1 >= h[k].nz - np);
+            const int r0 = tj * th, r1 = std::min(nz, r0 + th), c0 = ti * tw,
+                      c1 = std::min(nx, c0 + tw);
+            // kernel 6: one-point tiles lying wholly in the four-column interior are not listed
+            if (x4 && r0 >= np && r1 <= nz - np && c0 >= np && c1 <= x4hi) continue;
+            list.push_back(TileRef{k, t});
+            strip.push_back(c0 < np || c1 > nx - np || r0 < np || r1 > nz - np);
+        }
+        if (x4) {
+            const int nb = (nz - 2 * np + th - 1) / th;
+            for (int u = 0; u < nb * h[k].nx4; ++u) {
+                list.push_back(TileRef{k, ntx * ntz + u});
+                strip.push_back(0);
+            }
         }
     }
     // XCD-aware order: workgroup b is dispatched to XCD b % 8, so XCD x gets

@@ -429,7 +429,7 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
     }
     // the server steps one time level per launch (admission at any step)
     const bool two = ctx->kernel == 3;
-    s->c->kernel = two ? 0 : ctx->kernel;
+    s->c->kernel = two ? 4 : ctx->kernel;
     s->c->rz = two ? 1 : ctx->rz;
     s->c->brows = two ? 4 : ctx->brows;
     s->c->strip_first = ctx->strip_first;
