@@ -176,6 +176,11 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  * kernel 6 = kernel 4 on the CPML strips, four columns per lane (16-byte
  *   loads, neighbours by cross-lane shifts) in the interior: block_rows 1, 2
  *   or 4, 1 row;
+ * kernel 7 = two time steps per launch, level s staged in LDS with a 4-point
+ *   halo, one grid row per wavefront: 60 x (4 rows_per_thread) output tiles
+ *   with rows_per_thread 3, 5 or 7 and block_rows (wavefronts) 4, 8 or 16
+ *   dividing 4 (rows_per_thread + 1); shots keep four time levels -- set it
+ *   before creating shots;
  * graph_chunk = step launches per captured hipGraph (<= 1: no graph);
  * strip_first != 0 dispatches the CPML-strip tiles of each XCD first. */
 typedef struct {

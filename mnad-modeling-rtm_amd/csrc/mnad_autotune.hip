@@ -5,8 +5,11 @@
 // working set (about 40 B per point per shot and step) fits the 256 MB
 // Infinity Cache; a single 3401 x 751 shot (130 MB per step) already runs
 // from it, while 595 k-point shots need several per launch to fill the
-// 256 CUs.  The kernel variant (rows per thread, LDS staging) trades
-// parallelism against load instructions the same way.
+// 256 CUs.  The kernel variant trades parallelism against memory
+// instructions the same way: one grid row per wavefront (most wavefronts,
+// best latency hiding), four columns per lane (a quarter of the load
+// instructions and redundant row reads, half the occupancy), two steps per
+// launch (half the launches and level traffic, redundant halo arithmetic).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,7 +27,8 @@ struct Variant {
     int kernel, rz, brows;
 };
 
-const Variant kVariants[] = {{4, 1, 4}, {4, 1, 1}, {0, 1, 4}, {0, 2, 4}, {6, 1, 2}, {3, 1, 16}};
+const Variant kVariants[] = {{4, 1, 4}, {4, 1, 1}, {0, 1, 4}, {6, 1, 2}, {6, 1, 4},
+                             {7, 3, 16}, {7, 5, 8}, {7, 7, 16}};
 
 // Time `steps` steps of `b` forward shots (source at the centre, no
 // receivers, no snapshots) with the context's current tuning; returns

@@ -93,7 +93,8 @@ struct mnad_ctx {
     int next_stack = 1;
     int graph_chunk = 64;
     int kernel = 4;             // step kernel: 0 registers, 1 prefetch, 2 LDS, 3 two steps,
-                                // 4 wave rows, 5 wave rows + CPML prefetch, 6 four columns
+                                // 4 wave rows, 5 wave rows + CPML prefetch, 6 four columns,
+                                // 7 two steps, wave rows + LDS
     int rz = 1;                 // rows per thread
     int brows = 4;              // thread rows per block (tile = 64 x brows*rz)
     int strip_first = 1;        // dispatch CPML-strip tiles first
@@ -154,6 +155,9 @@ void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
 int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);
 int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags);
 void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table);
+// Step kernels that advance two time levels per launch (shots keep four).
+inline bool two_step_kernel(int kernel) { return kernel == 3 || kernel == 7; }
+
 // Output tile of the current step kernel (columns x rows) and steps per launch.
 void tile_shape(const mnad_ctx* c, int* tw, int* th, int* steps_per_launch);
 double algorithmic_bytes(const ShotDev& d);

@@ -976,7 +976,9 @@ __global__ __launch_bounds__(BX * BZR) void k_stepx(const ShotDev* __restrict__ 
 // level t lives in f[t % 4]: the launch reads levels s-1, s and writes s+1,
 // s+2, so no block overwrites a level a neighbouring block still reads.  CPML
 // memories, receivers, snapshots and imaging are handled per phase exactly as
-// in the one-step kernels (owner thread only for every global write).
+// in the one-step kernels (owner thread only for every global write).  The
+// CPML memories have two sets (planes 0-8 and 9-17): a launch starting at step
+// s reads set (s/2)%2 and writes the other.
 constexpr int X2 = 64, Z2 = 16;            // extended region = block shape
 constexpr int OX2 = X2 - 4, OZ2 = Z2 - 4;  // output tile
 constexpr int LX2 = X2 + 4, LZ2 = Z2 + 4;  // staged level s (extended + 2)
@@ -1032,6 +1034,9 @@ __global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ s
     const ShotDev& d = shots[tr.shot];
     const int s = *base + rel - d.t0;
     if (s < 0 || s >= d.nt) return;   // uniform over the block
+    // CPML memories are double-buffered across launches: halo points read
+    // them too, so an owner must not overwrite the set its neighbours read
+    const int pin = (s >> 1) & 1;
     const int tj = tr.tile / d.ntx;
     const int ti = tr.tile - tj * d.ntx;
     const int x0 = ti * OX2, z0 = tj * OZ2;   // first output point
@@ -1076,7 +1081,7 @@ __global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ s
 #pragma unroll
             for (int k = 0; k < 3; ++k) cx[k] = d.cfx[k * nsx + qx];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) px[k] = d.psx[k * nsx + qx];
+            for (int k = 0; k < 9; ++k) px[k] = d.psx[(pin * 9 + k) * nsx + qx];
         }
         if (inz) {
             nsz = (long)2 * np * nx;
@@ -1084,7 +1089,7 @@ __global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ s
 #pragma unroll
             for (int k = 0; k < 3; ++k) cz[k] = d.cfz[k * nsz + qz];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) pz[k] = d.psz[k * nsz + qz];
+            for (int k = 0; k < 9; ++k) pz[k] = d.psz[(pin * 9 + k) * nsz + qz];
         }
         if (d.nrec > 0) {
             const int off = d.rrow[j];
@@ -1157,12 +1162,284 @@ __global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ s
     if (owned) {   // CPML memories after the last step this launch ran
         if (inx) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) d.psx[k * nsx + qx] = px[k];
+            for (int k = 0; k < 9; ++k) d.psx[((pin ^ 1) * 9 + k) * nsx + qx] = px[k];
         }
         if (inz) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) d.psz[k * nsz + qz] = pz[k];
+            for (int k = 0; k < 9; ++k) d.psz[((pin ^ 1) * 9 + k) * nsz + qz] = pz[k];
         }
+    }
+}
+
+// ------------------------------------------- two steps per launch, wave rows
+// A block of four wavefronts owns an output tile of 60 columns x OZ rows and
+// advances it two time levels in one launch:
+//   phase 0  stage time level s of U, Vs, Ws on the tile plus a 4-point halo
+//            (OZ+8 rows x 68 columns) in LDS, a wavefront per row;
+//   phase 1  level s+1 on the tile plus a 2-point halo (OZ+4 rows x 64
+//            columns, lane l <-> column x0-2+l) from LDS into LDS; tile
+//            points also go to global memory (the next launch's previous
+//            level), with their receivers, snapshots and imaging of step s;
+//   phase 2  level s+2 on the tile (lanes 2..61), step s+1's receivers,
+//            snapshots and imaging.
+// Per output point and two steps this reads level s with its halo, level s-1
+// and rx once and writes levels s+1 and s+2 (about 52 B against 80 B for two
+// one-step launches), with the neighbourhood reuse in LDS instead of L2, and
+// halves the launches.  Time level t lives in f[t % 4], so no block
+// overwrites a level another block still reads.  CPML memories have two sets
+// (planes 0-8, 9-17): the launch at step s reads set (s/2)%2 and writes the
+// other.  Phase 1 steps the memories of every point it computes (halo
+// included) in registers and drops them; phase 2 reloads its point's memories,
+// repeats the level s+1 update from the staged level s, steps them to level
+// s+2 and stores them (a launch that runs one step stores the s+1 set from
+// phase 1).
+constexpr int X2W = 60;   // output columns of a two-step wave-row tile
+
+// The CPML memories see U without the source injection that the previous step
+// folded into it (fwd_PDE_exact_MNAD.m:95-111): remove it from the values at
+// and around the source.
+__device__ __forceinline__ void pre_injection(const ShotDev& d, int s, int j, int i, float& uc,
+                                              float& un, float& us, float& ue, float& uw)
+{
+    if (d.mode != MODE_FWD || d.src_j < 0) return;
+    const float a0 = d.src_amp[s];
+    if (j == d.src_j) {
+        if (i == d.src_i) uc -= a0;
+        if (i - 1 == d.src_i) uw -= a0;
+        if (i + 1 == d.src_i) ue -= a0;
+    }
+    if (i == d.src_i) {
+        if (j - 1 == d.src_j) un -= a0;
+        if (j + 1 == d.src_j) us -= a0;
+    }
+}
+
+// CPML term of one point for step s (neighbourhoods u, v, w of level s): the
+// memories are read from set `pin` and, if `store` >= 0, written to set
+// `store` after the update; `upd` also receives the updated memories.
+struct PsiPt {
+    bool inx, inz;
+    long nsx, rowx, nsz, rowz;
+    unsigned bx, bz;
+};
+
+__device__ __forceinline__ PsiPt psi_point(const ShotDev& d, int j, int i, int i0, unsigned lane)
+{
+    PsiPt q;
+    const int np = d.npml;
+    q.inx = (i < np) || (i >= d.nx - np);
+    q.inz = (j < np) || (j >= d.nz - np);
+    q.nsx = (long)d.nz * 2 * np;
+    q.rowx = (long)j * 2 * np;
+    q.bx = 4u * ((i < np) ? (unsigned)i : (unsigned)(i - (d.nx - 2 * np)));
+    q.nsz = (long)2 * np * d.nx;
+    q.rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.nx + i0;
+    q.bz = 4u * lane;
+    return q;
+}
+
+template <int OZ, int NW>
+__global__ __launch_bounds__(BX * NW) void k_step2w(const ShotDev* __restrict__ shots,
+                                                    const TileRef* __restrict__ tiles,
+                                                    const int* __restrict__ base, int rel)
+{
+    constexpr int R0 = OZ + 8, R1 = OZ + 4, LW = BX + 4;
+    static_assert(R1 % NW == 0, "phase-1 rows must split evenly over the wavefronts");
+    __shared__ float s0[3 * R0 * LW];   // level s, rows z0-4 .., columns x0-4 ..
+    __shared__ float s1[3 * R1 * BX];   // level s+1, rows z0-2 .., columns x0-2 ..
+
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;   // uniform over the block
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int x0 = ti * X2W, z0 = tj * OZ;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const unsigned lane = threadIdx.x;
+    const int nx = d.nx, nz = d.nz, np = d.npml;
+    const long P = d.pitch;
+    const bool second = s + 1 < d.nt;
+    const int pin = (s >> 1) & 1;
+
+    // Per-point global inputs of this wave's phase-1 rows (r1 = wv + 4k), issued
+    // before the staging so that their latency overlaps it: rx, level s-1,
+    // receiver slot.  Phase 2 reuses rx (same thread, same point).
+    constexpr int K1 = R1 / NW;
+    float rxk[K1], pu[K1], pv[K1], pw[K1];
+    int rk[K1];
+#pragma unroll
+    for (int k = 0; k < K1; ++k) {
+        const int j = z0 - 2 + wv + NW * k, i = x0 - 2 + (int)lane;
+        rxk[k] = pu[k] = pv[k] = pw[k] = 0.0f;
+        rk[k] = -1;
+        if (j >= 0 && j < nz && i >= 0 && i < nx) {
+            const long org = (long)(j + MNAD_ZO) * P + (x0 - 2 + MNAD_XO);   // (j, x0-2)
+            rxk[k] = (gp(d.rx) + org)[lane];
+            pu[k] = (gp(d.f[(s + 3) & 3][0]) + org)[lane];
+            pv[k] = (gp(d.f[(s + 3) & 3][1]) + org)[lane];
+            pw[k] = (gp(d.f[(s + 3) & 3][2]) + org)[lane];
+            if (d.nrec > 0) {
+                const int off = __builtin_amdgcn_readfirstlane(d.rrow[j]);
+                if (off >= 0) rk[k] = uni(gp(d.rcol) + off + x0 - 2)[lane];
+            }
+        }
+    }
+
+    // ---- phase 0: stage level s (all rows of the wave issued together)
+    constexpr int K0 = (R0 + NW - 1) / NW;
+#pragma unroll
+    for (int k = 0; k < K0; ++k) {
+        const int r = wv + NW * k;
+        if (r >= R0) break;
+        const int j = z0 - 4 + r;
+        float a[3] = {0.0f, 0.0f, 0.0f}, e[3] = {0.0f, 0.0f, 0.0f};
+        if (j >= 0 && j < nz) {
+            const long org = (long)(j + MNAD_ZO) * P + (x0 - 4 + MNAD_XO);
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const gfloat* src = gp(d.f[s & 3][f]) + org;
+                a[f] = src[lane];
+                if (lane < 4) e[f] = (src + BX)[lane];
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            s0[(f * R0 + r) * LW + lane] = a[f];
+            if (lane < 4) s0[(f * R0 + r) * LW + BX + lane] = e[f];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 1: level s+1 on the tile and its 2-point halo
+#pragma unroll
+    for (int k = 0; k < K1; ++k) {
+        const int r1 = wv + NW * k;
+        const int j = z0 - 2 + r1, i = x0 - 2 + (int)lane;
+        float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+        if (j >= 0 && j < nz && i >= 0 && i < nx) {
+            const float* c = s0 + (r1 + 2) * LW + lane + 2;
+            Nbh u, v, w;
+            nbh_lds(c, LW, u);
+            nbh_lds(c + R0 * LW, LW, v);
+            nbh_lds(c + 2 * R0 * LW, LW, w);
+            const float vww = c[R0 * LW - 2], vee = c[R0 * LW + 2];
+            const float wnn = c[2 * R0 * LW - 2 * LW], wss = c[2 * R0 * LW + 2 * LW];
+            const long org = (long)(j + MNAD_ZO) * P + (x0 - 2 + MNAD_XO);   // (j, x0-2)
+            const float rx = rxk[k];
+            const int r = rk[k];
+            const float rz = rx * d.kap;
+            float du, dv, dw;
+            mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+            const bool own = j >= z0 && j < z0 + OZ && lane >= 2 && lane < BX - 2;
+            const PsiPt q = psi_point(d, j, i, x0 - 2, lane);
+            if (q.inx || q.inz) {
+                float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
+                pre_injection(d, s, j, i, uc, un, us, ue, uw);
+                const bool keep = own && !second;   // a one-step launch stores the s+1 set here
+                if (q.inx) {
+                    PsiW m;
+                    psi_load(m, gp(d.psx) + pin * 9 * q.nsx + q.rowx, gp(d.cfx) + q.rowx, q.nsx, q.bx);
+                    psi_step(m, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+                    if (keep) psi_store(m, gp(d.psx) + (pin ^ 1) * 9 * q.nsx + q.rowx, q.nsx, q.bx);
+                }
+                if (q.inz) {
+                    PsiW m;
+                    psi_load(m, gp(d.psz) + pin * 9 * q.nsz + q.rowz, gp(d.cfz) + q.rowz, q.nsz, q.bz);
+                    psi_step(m, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+                    if (keep) psi_store(m, gp(d.psz) + (pin ^ 1) * 9 * q.nsz + q.rowz, q.nsz, q.bz);
+                }
+            }
+            n0 = (2.0f * u.c + du) - pu[k];
+            n1 = (2.0f * v.c + dv) - pv[k];
+            n2 = (2.0f * w.c + dw) - pw[k];
+            if (d.mode == MODE_FWD) {
+                if (j == d.src_j && i == d.src_i) n0 += d.src_amp[s + 1];
+                if (own && r >= 0) (gp(d.gather) + (long)s * d.nrec)[(unsigned)r] = u.c;
+            } else if (r >= 0) {
+                n0 -= (gp(d.radj) + (long)s * d.nrec)[(unsigned)r];
+            }
+            if (own) {
+                (gp(d.f[(s + 1) & 3][0]) + org)[lane] = n0;
+                (gp(d.f[(s + 1) & 3][1]) + org)[lane] = n1;
+                (gp(d.f[(s + 1) & 3][2]) + org)[lane] = n2;
+                if (d.T > 0 && !q.inx && !q.inz)
+                    snap_image_w(d, s, (long)(j - np) * (nx - 2 * np), (unsigned)(i - np), du, n0);
+            }
+        }
+        s1[(0 * R1 + r1) * BX + lane] = n0;
+        s1[(1 * R1 + r1) * BX + lane] = n1;
+        s1[(2 * R1 + r1) * BX + lane] = n2;
+    }
+    __syncthreads();
+    if (!second) return;
+
+    // ---- phase 2: level s+2 on the tile; rows with (row - z0 + 2) % 4 == wave,
+    // the rows this wave computed in phase 1
+#pragma unroll
+    for (int k = 0; k < K1; ++k) {
+        const int q2 = wv + NW * k - 2;   // this wave's phase-1 row, as a tile row
+        if (q2 < 0 || q2 >= OZ) continue;
+        const int j = z0 + q2, i = x0 - 2 + (int)lane;
+        if (j >= nz) break;
+        if (lane < 2 || lane >= BX - 2 || i >= nx) continue;
+        const float* c = s1 + (q2 + 2) * BX + lane;
+        Nbh u, v, w;
+        nbh_lds(c, BX, u);
+        nbh_lds(c + R1 * BX, BX, v);
+        nbh_lds(c + 2 * R1 * BX, BX, w);
+        const float vww = c[R1 * BX - 2], vee = c[R1 * BX + 2];
+        const float wnn = c[2 * R1 * BX - 2 * BX], wss = c[2 * R1 * BX + 2 * BX];
+        const float* c0 = s0 + (q2 + 4) * LW + lane + 2;   // level s at (j, i)
+        const long org = (long)(j + MNAD_ZO) * P + (x0 - 2 + MNAD_XO);
+        const float rx = rxk[k];
+        const int r = rk[k];
+        const float rz = rx * d.kap;
+        float du, dv, dw;
+        mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+        const PsiPt q = psi_point(d, j, i, x0 - 2, lane);
+        if (q.inx || q.inz) {
+            // level s neighbourhood along the axes (staged), pre-injection at step s
+            const float* cv = c0 + R0 * LW;
+            const float* cw = c0 + 2 * R0 * LW;
+            float a_c = c0[0], a_n = c0[-LW], a_s = c0[LW], a_e = c0[1], a_w = c0[-1];
+            pre_injection(d, s, j, i, a_c, a_n, a_s, a_e, a_w);
+            float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
+            pre_injection(d, s + 1, j, i, uc, un, us, ue, uw);
+            float t0, t1, t2;   // discarded increments of the repeated level s+1 update
+            if (q.inx) {
+                PsiW m;
+                psi_load(m, gp(d.psx) + pin * 9 * q.nsx + q.rowx, gp(d.cfx) + q.rowx, q.nsx, q.bx);
+                const float rx0 = rx;
+                t0 = t1 = t2 = 0.0f;
+                psi_step(m, rx0, a_w, a_c, a_e, cv[-1], cv[0], cv[1], cw[-1], cw[0], cw[1], t0, t1, t2);
+                psi_step(m, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+                psi_store(m, gp(d.psx) + (pin ^ 1) * 9 * q.nsx + q.rowx, q.nsx, q.bx);
+            }
+            if (q.inz) {
+                PsiW m;
+                psi_load(m, gp(d.psz) + pin * 9 * q.nsz + q.rowz, gp(d.cfz) + q.rowz, q.nsz, q.bz);
+                t0 = t1 = t2 = 0.0f;
+                psi_step(m, rz, a_n, a_c, a_s, cv[-LW], cv[0], cv[LW], cw[-LW], cw[0], cw[LW], t0, t1, t2);
+                psi_step(m, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+                psi_store(m, gp(d.psz) + (pin ^ 1) * 9 * q.nsz + q.rowz, q.nsz, q.bz);
+            }
+        }
+        float n0 = (2.0f * u.c + du) - c0[0];
+        const float n1 = (2.0f * v.c + dv) - c0[R0 * LW];
+        const float n2 = (2.0f * w.c + dw) - c0[2 * R0 * LW];
+        if (d.mode == MODE_FWD) {
+            if (j == d.src_j && i == d.src_i) n0 += d.src_amp[s + 2];
+            if (r >= 0) (gp(d.gather) + (long)(s + 1) * d.nrec)[(unsigned)r] = u.c;
+        } else if (r >= 0) {
+            n0 -= (gp(d.radj) + (long)(s + 1) * d.nrec)[(unsigned)r];
+        }
+        (gp(d.f[(s + 2) & 3][0]) + org)[lane] = n0;
+        (gp(d.f[(s + 2) & 3][1]) + org)[lane] = n1;
+        (gp(d.f[(s + 2) & 3][2]) + org)[lane] = n2;
+        if (d.T > 0 && !q.inx && !q.inz)
+            snap_image_w(d, s + 1, (long)(j - np) * (nx - 2 * np), (unsigned)(i - np), du, n0);
     }
 }
 
@@ -1287,6 +1564,20 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL((k_stepw<1, true>), g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
     else if (kernel == 5 && brows == 4 && rows == 1)
         hipLaunchKernelGGL((k_stepw<4, true>), g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 7 && brows == 4 && rows == 3)
+        hipLaunchKernelGGL((k_step2w<12, 4>), g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 7 && brows == 8 && rows == 3)
+        hipLaunchKernelGGL((k_step2w<12, 8>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
+    else if (kernel == 7 && brows == 16 && rows == 3)
+        hipLaunchKernelGGL((k_step2w<12, 16>), g, dim3(BX, 16), 0, st, shots, tiles, base, rel);
+    else if (kernel == 7 && brows == 4 && rows == 5)
+        hipLaunchKernelGGL((k_step2w<20, 4>), g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 7 && brows == 8 && rows == 5)
+        hipLaunchKernelGGL((k_step2w<20, 8>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
+    else if (kernel == 7 && brows == 8 && rows == 7)
+        hipLaunchKernelGGL((k_step2w<28, 8>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
+    else if (kernel == 7 && brows == 16 && rows == 7)
+        hipLaunchKernelGGL((k_step2w<28, 16>), g, dim3(BX, 16), 0, st, shots, tiles, base, rel);
     else if (kernel == 3 && brows == 16 && rows == 1)
         hipLaunchKernelGGL(k_step2, g, dim3(X2, Z2), 0, st, shots, tiles, base, rel);
     else

@@ -288,15 +288,17 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
     s.d.rx = m.rx.as<float>();
     s.d.ntx = (g.nx + 63) / 64;
     s.d.src_j = s.d.src_i = -1;
-    s.d.nlev = c->kernel == 3 ? 4 : 2;
+    s.d.nlev = two_step_kernel(c->kernel) ? 4 : 2;
     for (int k = 0; k < 3 * s.d.nlev; ++k) {
         CHK(dalloc(c, s.state[k], m.elems * sizeof(float), true));
         s.d.f[k / 3][k % 3] = s.state[k].as<float>();
     }
     const int np = g.npml;
     if (np > 0) {
-        CHK(dalloc(c, s.psx, 9 * (size_t)g.nz * 2 * np * sizeof(float), true));
-        CHK(dalloc(c, s.psz, 9 * (size_t)2 * np * g.nx * sizeof(float), true));
+        // two-steps-per-launch kernels keep two sets of memories (read one, write the other)
+        const size_t planes = s.d.nlev == 4 ? 18 : 9;
+        CHK(dalloc(c, s.psx, planes * g.nz * 2 * np * sizeof(float), true));
+        CHK(dalloc(c, s.psz, planes * 2 * np * g.nx * sizeof(float), true));
         s.d.psx = s.psx.as<float>();
         s.d.psz = s.psz.as<float>();
         s.d.cfx = m.cfx.as<float>();
@@ -403,6 +405,10 @@ void tile_shape(const mnad_ctx* c, int* tw, int* th, int* spl)
         *tw = 60;
         *th = 12;
         *spl = 2;
+    } else if (c->kernel == 7) {   // two steps per launch: 60 x (4 rows_per_thread) outputs
+        *tw = 60;
+        *th = 4 * c->rz;
+        *spl = 2;
     } else {                // 64 columns x (block_rows * rows_per_thread) rows
                             // (kernel 6: the one-point tiles; see build_tiles)
         *tw = 64;
@@ -496,7 +502,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         h[k] = shots[k]->d;
         h[k].t0 = 0;
         nsteps = std::max(nsteps, h[k].nt);
-        if (h[k].nlev != (kern == 3 ? 4 : 2))
+        if (h[k].nlev != (two_step_kernel(kern) ? 4 : 2))
             return fail(MNAD_ERR_STATE, "shot built for %d time levels cannot run on step kernel %d",
                         h[k].nlev, kern);
     }

@@ -428,7 +428,7 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
         return rc;
     }
     // the server steps one time level per launch (admission at any step)
-    const bool two = ctx->kernel == 3;
+    const bool two = two_step_kernel(ctx->kernel);
     s->c->kernel = two ? 4 : ctx->kernel;
     s->c->rz = two ? 1 : ctx->rz;
     s->c->brows = two ? 4 : ctx->brows;

@@ -184,7 +184,9 @@ def test_invalid_inputs_raise(mnad, ctx):
 @pytest.mark.parametrize("kernel,rz,br", [(0, 1, 4), (0, 2, 4), (0, 4, 4), (0, 1, 8), (0, 1, 16),
                                           (1, 1, 4), (2, 1, 4), (2, 2, 4), (3, 1, 16),
                                           (4, 1, 1), (4, 1, 2), (4, 1, 4), (4, 1, 8),
-                                          (5, 1, 1), (5, 1, 4), (6, 1, 1), (6, 1, 2), (6, 1, 4)])
+                                          (5, 1, 1), (5, 1, 4), (6, 1, 1), (6, 1, 2), (6, 1, 4),
+                                          (7, 3, 4), (7, 3, 8), (7, 3, 16), (7, 5, 4), (7, 5, 8),
+                                          (7, 7, 8), (7, 7, 16)])
 @pytest.mark.parametrize("strip_first", [0, 1])
 def test_launch_shapes_agree(mnad, ctx, oracle, kernel, rz, br, strip_first):
     """Every kernel family / rows-per-thread / dispatch-order variant of the
@@ -207,7 +209,7 @@ def test_launch_shapes_agree(mnad, ctx, oracle, kernel, rz, br, strip_first):
 
 
 @pytest.mark.parametrize("kernel,rz,br", [(3, 1, 16), (4, 1, 4), (4, 1, 1), (5, 1, 4), (6, 1, 4),
-                                          (6, 1, 1)])
+                                          (6, 1, 1), (7, 3, 4), (7, 3, 16), (7, 7, 8)])
 def test_kernel_families_rtm_and_multi_tile(mnad, kernel, rz, br):
     """Each step-kernel family through the whole RTM (snapshots, imaging,
     receiver injection; f32, f16 and bf16 snapshots) against the stacked fixture
