@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "mnad.h"
@@ -83,6 +85,7 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
     max_shots = std::max(1, std::min(max_shots, mem_cap));
 
     const int sk = c->kernel, sr = c->rz, sb = c->brows;
+    const bool log = getenv("MNAD_AUTOTUNE_LOG") != nullptr;   // one stderr line per probe
     TunePick best{0, 1, 4, 1, 1e30};
     for (int b = 1; b <= max_shots; b *= 2) {
         double best_b = 1e30;
@@ -99,6 +102,9 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
                 c->brows = sb;
                 return rc;
             }
+            if (log)
+                fprintf(stderr, "mnad_autotune %dx%d: kernel %d rows %d block_rows %d, %d shots: %.3f us/shot-step\n",
+                        g.nz, g.nx, v.kernel, v.rz, v.brows, b, us);
             best_b = std::min(best_b, us);
             if (us < best.us) best = TunePick{v.kernel, v.rz, v.brows, b, us};
         }

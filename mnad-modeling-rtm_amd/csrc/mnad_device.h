@@ -11,6 +11,8 @@
 
 #include <stdint.h>
 
+#include "mnad_stencil.h"
+
 #define MNAD_ZO 2    // ghost rows above the grid
 #define MNAD_XO 32   // column offset of x = 0 (row starts 128-byte aligned)
 #define MNAD_ZPAD 8  // extra zero rows below the grid (rows a partial tile may read)
@@ -34,6 +36,7 @@ struct ShotDev {
     int nt1, nx4, x4lo, x4hi;
     float kap;                      // rz / rx = dx^2 / dz^2
     float inv_dt2;                  // 1 / dt^2 (snapshot d2u)
+    IncrC ic;                       // increment weights for this kap (mnad_incr)
     const float* rx;                // a / dx^2 per point, grid storage
     float* f[4][3];                 // state [level][U, Vs, Ws], grid storage: time level
                                     // t lives in f[t % nlev]; nlev = 2, or 4 for the

@@ -308,7 +308,7 @@ __global__ __launch_bounds__(BX * BZR) void k_step(const ShotDev* __restrict__ s
         const float rx = d.rx[p];
         const float rz = rx * d.kap;
         float du, dv, dw;
-        mnad_incr(rx, rz, u, v, vf[r][0], vf[r][1], w, wc[r], wc[r + 4], du, dv, dw);
+        mnad_incr(d.ic, rx, u, v, vf[r][0], vf[r][1], w, wc[r], wc[r + 4], du, dv, dw);
         finish_point(d, s, j, i, p, u, v, w, du, dv, dw, rx, rz, PU, PV, PW);
     }
 }
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(BX * BZ) void k_step_lds(const ShotDev* __restrict_
         const float rx = d.rx[p];
         const float rz = rx * d.kap;
         float du, dv, dw;
-        mnad_incr(rx, rz, u, v, b[-2], b[2], w, c[-2 * WW], c[2 * WW], du, dv, dw);
+        mnad_incr(d.ic, rx, u, v, b[-2], b[2], w, c[-2 * WW], c[2 * WW], du, dv, dw);
         finish_point(d, s, j, i, p, u, v, w, du, dv, dw, rx, rz, PU, PV, PW);
     }
 }
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(BX * BZ) void k_step_pf(const ShotDev* __restrict__
 
     const float rz = rx * d.kap;
     float du, dv, dw;
-    mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
 
     if (inx || inz) {
         // The CPML memories see the pre-injection U (fwd_PDE_exact_MNAD.m:95-111).
@@ -717,7 +717,7 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
 
     const float rz = rx * d.kap;
     float du, dv, dw;
-    mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
 
     if (inx || inz) {
         // The CPML memories see the pre-injection U (fwd_PDE_exact_MNAD.m:95-111).
@@ -858,7 +858,7 @@ __device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, 
         const float vee = e <= 1 ? vc.e[e + 2] : (e == 2 ? vc.r : vr2);
         const float rx = rx4[e];
         float du, dv, dw;
-        mnad_incr(rx, rx * d.kap, u, v, vww, vee, w, wnn[e], wss[e], du, dv, dw);
+        mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn[e], wss[e], du, dv, dw);
         nu[e] = 2.0f * u.c + du;
         nv[e] = 2.0f * v.c + dv;
         nw[e] = 2.0f * w.c + dw;
@@ -1108,7 +1108,7 @@ __global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ s
         nbh_lds(cv + l, LX2, v);
         nbh_lds(cw + l, LX2, w);
         float du, dv, dw;
-        mnad_incr(rx, rz, u, v, cv[l - 2], cv[l + 2], w, cw[l - 2 * LX2], cw[l + 2 * LX2], du, dv,
+        mnad_incr(d.ic, rx, u, v, cv[l - 2], cv[l + 2], w, cw[l - 2 * LX2], cw[l + 2 * LX2], du, dv,
                   dw);
         if (inx || inz) psi_regs(d, s, j, i, inx, inz, rx, rz, u, v, w, cx, px, cz, pz, du, dv, dw);
         n1u = (2.0f * u.c + du) - pu;
@@ -1140,7 +1140,7 @@ __global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ s
         nbh_lds(nv_ + tid, X2, v);
         nbh_lds(nw_ + tid, X2, w);
         float du, dv, dw;
-        mnad_incr(rx, rz, u, v, nv_[tid - 2], nv_[tid + 2], w, nw_[tid - 2 * X2],
+        mnad_incr(d.ic, rx, u, v, nv_[tid - 2], nv_[tid + 2], w, nw_[tid - 2 * X2],
                   nw_[tid + 2 * X2], du, dv, dw);
         if (inx || inz)
             psi_regs(d, s + 1, j, i, inx, inz, rx, rz, u, v, w, cx, px, cz, pz, du, dv, dw);
@@ -1331,7 +1331,7 @@ __global__ __launch_bounds__(BX * NW) void k_step2w(const ShotDev* __restrict__ 
             const int r = rk[k];
             const float rz = rx * d.kap;
             float du, dv, dw;
-            mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+            mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
             const bool own = j >= z0 && j < z0 + OZ && lane >= 2 && lane < BX - 2;
             const PsiPt q = psi_point(d, j, i, x0 - 2, lane);
             if (q.inx || q.inz) {
@@ -1397,7 +1397,7 @@ __global__ __launch_bounds__(BX * NW) void k_step2w(const ShotDev* __restrict__ 
         const int r = rk[k];
         const float rz = rx * d.kap;
         float du, dv, dw;
-        mnad_incr(rx, rz, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+        mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
         const PsiPt q = psi_point(d, j, i, x0 - 2, lane);
         if (q.inx || q.inz) {
             // level s neighbourhood along the axes (staged), pre-injection at step s

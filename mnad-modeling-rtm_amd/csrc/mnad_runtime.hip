@@ -284,6 +284,7 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
     s.d.nt = sp.nt;
     s.d.mode = sp.mode;
     s.d.kap = (float)((g.dx * g.dx) / (g.dz * g.dz));
+    incr_consts((g.dx * g.dx) / (g.dz * g.dz), s.d.ic);
     s.d.inv_dt2 = (float)(1.0 / (g.dt * g.dt));
     s.d.rx = m.rx.as<float>();
     s.d.ntx = (g.nx + 63) / 64;

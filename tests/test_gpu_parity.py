@@ -163,6 +163,33 @@ def test_batched_shots_equal_single_shots(mnad, ctx):
         assert np.array_equal(batched[k], alone["gather"]), k
 
 
+@pytest.mark.parametrize("kernel,rz,br", [(3, 1, 16), (7, 3, 16), (7, 5, 8), (6, 1, 2)])
+def test_two_step_and_four_column_kernels_batched(mnad, kernel, rz, br):
+    """Eight C1 shots in one launch sequence (several rounds of workgroups
+    per launch, so blocks that finish early overlap blocks that have not
+    started) equal the same shots run alone bitwise, and the default kernel to
+    FP32 rounding.  A two-step kernel whose halo points read CPML memories an
+    owner block already overwrote would break the first comparison."""
+    _say(f"batched kernel={kernel} rz={rz} br={br}")
+    v, w, npml, xs, zr, xr, dt = _c1_inputs()
+    w = w[:300]
+    g = mnad.Grid(v.shape[0], v.shape[1], npml, 10.0, 10.0, dt)
+    srcs = [npml + 5 + 24 * k for k in range(8)]
+    recs = [(zr, xr)] * 8
+    ctx = mnad.Context(0)
+    try:
+        ref = ctx.fwd_shots(g, v, w, [npml] * 8, srcs, recs, batch=8)
+        ctx.set_tuning(kernel=kernel, rows_per_thread=rz, block_rows=br)
+        batched = ctx.fwd_shots(g, v, w, [npml] * 8, srcs, recs, batch=8)
+        for k in (0, 3, 7):
+            alone = ctx.fwd_shots(g, v, w, [npml], [srcs[k]], [recs[k]], batch=1)[0]
+            assert np.array_equal(batched[k], alone), k
+        for k in range(8):
+            assert rel_l2(batched[k], ref[k]) < 1e-5, k
+    finally:
+        ctx.close()
+
+
 def test_invalid_inputs_raise(mnad, ctx):
     f = golden("fwd_square")
     g = _grid(mnad, f)
