@@ -117,6 +117,7 @@ void mnad_ctx_destroy(mnad_ctx* ctx)
         (void)hipStreamSynchronize(ctx->stream);
         ctx->kept.clear();
         ctx->stacks.clear();
+        release_batch_cache(ctx);
         ctx->pool.trim();
         (void)hipStreamDestroy(ctx->stream);
     }

@@ -69,6 +69,18 @@ struct TunePick {
     double us;               // step-kernel microseconds per shot-step
 };
 
+// One cached launch sequence of run_batch: a graph chunk of step launches
+// that reads the batch descriptors from the context's persistent buffers, so
+// it can be replayed for any batch with the same kernel and tile count.
+struct GraphEntry {
+    int kern, rz, brows, ntiles, chunk, spl;
+    const void* ds;
+    const void* dt;
+    const int* base;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
 // Image / illumination stack (compensated sums over shots).
 struct Stack {
     mnad_grid g{};
@@ -101,6 +113,14 @@ struct mnad_ctx {
     size_t budget = 0;          // bytes usable for batches (0 = from free memory)
     int autotune_on_use = 0;    // batched calls tune an unseen grid shape first
     std::map<std::tuple<int, int, int>, mnad::TunePick> tuned;   // (nz, nx, npml)
+    // run_batch: persistent batch descriptors (grow-only) and cached graph chunks
+    mnad::DevBuf bdesc, btab, bbase;
+    std::vector<mnad::GraphEntry> graphs;
+    // host -> device staging through pinned memory (upload / copy_in)
+    void* stage = nullptr;
+    size_t stage_n = 0;
+    hipEvent_t stage_ev = nullptr;
+    bool stage_pending = false;
     // step-kernel accounting (mnad_timing)
     double t_ms = 0.0;
     long long t_launches = 0;
@@ -149,6 +169,12 @@ struct Shot {
 
 int dalloc(mnad_ctx* c, DevBuf& b, size_t bytes, bool zero);
 int upload(mnad_ctx* c, DevBuf& b, const void* src, size_t bytes);
+// Host -> device copy through the context's pinned staging buffer, queued on
+// the context stream; `src` may be reused as soon as this returns.
+int copy_in(mnad_ctx* c, void* dst, const void* src, size_t bytes);
+// Releases the run_batch descriptors, cached graphs and the staging buffer
+// (stream idle).
+void release_batch_cache(mnad_ctx* c);
 int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m);
 int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s);
 void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);

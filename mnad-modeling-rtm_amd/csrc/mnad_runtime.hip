@@ -148,15 +148,78 @@ int dalloc(mnad_ctx* c, DevBuf& b, size_t bytes, bool zero)
     return MNAD_OK;
 }
 
+int copy_in(mnad_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    if (bytes == 0) return MNAD_OK;
+    // one staging buffer: wait for its previous copy before overwriting it
+    if (c->stage_pending) {
+        HIPCHK(hipEventSynchronize(c->stage_ev));
+        c->stage_pending = false;
+    }
+    if (bytes > c->stage_n) {
+        if (c->stage) (void)hipHostFree(c->stage);
+        c->stage = nullptr;
+        c->stage_n = 0;
+        const size_t n = std::max(bytes, (size_t)4 << 20);
+        if (hipHostMalloc(&c->stage, n, hipHostMallocDefault) != hipSuccess) {
+            // no pinned memory: copy from the caller's buffer and wait for it
+            (void)hipGetLastError();
+            c->stage = nullptr;
+            HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            return MNAD_OK;
+        }
+        c->stage_n = n;
+    }
+    if (!c->stage_ev) HIPCHK(hipEventCreateWithFlags(&c->stage_ev, hipEventDisableTiming));
+    std::memcpy(c->stage, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, c->stage, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->stage_ev, c->stream));
+    c->stage_pending = true;
+    return MNAD_OK;
+}
+
 int upload(mnad_ctx* c, DevBuf& b, const void* src, size_t bytes)
 {
     CHK(dalloc(c, b, bytes, false));
-    if (bytes) {
-        // The caller's host buffer may be freed on return: finish the copy.
-        HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+    return copy_in(c, b.p, src, bytes);   // the caller's buffer is free on return
+}
+
+void release_batch_cache(mnad_ctx* c)
+{
+    for (GraphEntry& e : c->graphs) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipGraphDestroy(e.graph);
     }
-    return MNAD_OK;
+    c->graphs.clear();
+    c->bdesc.reset();
+    c->btab.reset();
+    c->bbase.reset();
+    if (c->stage_pending) (void)hipEventSynchronize(c->stage_ev);
+    c->stage_pending = false;
+    if (c->stage) (void)hipHostFree(c->stage);
+    c->stage = nullptr;
+    c->stage_n = 0;
+    if (c->stage_ev) (void)hipEventDestroy(c->stage_ev);
+    c->stage_ev = nullptr;
+}
+
+// Grows a persistent batch buffer; graphs that read the old one are dropped.
+static int ensure_batch_buf(mnad_ctx* c, DevBuf& b, size_t bytes)
+{
+    if (b.n >= bytes && b.p) return MNAD_OK;
+    const void* old = b.p;
+    for (size_t k = 0; k < c->graphs.size();) {
+        GraphEntry& e = c->graphs[k];
+        if (old && (e.ds == old || e.dt == old || e.base == old)) {
+            (void)hipGraphExecDestroy(e.exec);
+            (void)hipGraphDestroy(e.graph);
+            c->graphs.erase(c->graphs.begin() + k);
+        } else {
+            ++k;
+        }
+    }
+    return dalloc(c, b, std::max(bytes, 2 * b.n), false);
 }
 
 // ----------------------------------------------------------------- model
@@ -509,11 +572,15 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     }
     std::vector<TileRef> table;
     build_tiles(c, h, table);
-    DevBuf dsh, dtab, dbase;
-    CHK(upload(c, dsh, h.data(), h.size() * sizeof(ShotDev)));
-    CHK(upload(c, dtab, table.data(), table.size() * sizeof(TileRef)));
-    CHK(dalloc(c, dbase, sizeof(int), true));
-    HIPCHK(mnad_launch_init_source(dsh.as<ShotDev>(), n, c->stream));
+    // descriptors in the context's persistent buffers, so that a cached graph
+    // chunk of the same shape replays on them
+    CHK(ensure_batch_buf(c, c->bdesc, h.size() * sizeof(ShotDev)));
+    CHK(ensure_batch_buf(c, c->btab, table.size() * sizeof(TileRef)));
+    CHK(ensure_batch_buf(c, c->bbase, sizeof(int)));
+    CHK(copy_in(c, c->bdesc.p, h.data(), h.size() * sizeof(ShotDev)));
+    CHK(copy_in(c, c->btab.p, table.data(), table.size() * sizeof(TileRef)));
+    HIPCHK(hipMemsetAsync(c->bbase.p, 0, sizeof(int), c->stream));   // step counter (not captured)
+    HIPCHK(mnad_launch_init_source(c->bdesc.as<ShotDev>(), n, c->stream));
     // Algorithmic HBM bytes of the whole sequence (see mnad_timing).
     double bytes = 0.0;
     for (int k = 0; k < n; ++k) bytes += algorithmic_bytes(h[k]);
@@ -525,9 +592,9 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
     } evg{ev0, ev1};
     const int ntiles = (int)table.size();
-    const ShotDev* ds = dsh.as<ShotDev>();
-    const TileRef* dt = dtab.as<TileRef>();
-    int* base = dbase.as<int>();
+    const ShotDev* ds = c->bdesc.as<ShotDev>();
+    const TileRef* dt = c->btab.as<TileRef>();
+    int* base = c->bbase.as<int>();
     int chunk = c->graph_chunk;
     int tw, th, spl;
     tile_shape(c, &tw, &th, &spl);
@@ -537,31 +604,42 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         for (int s = 0; s < nlaunch; ++s)
             HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s * spl, kern, rz, br, c->stream));
     } else {
-        hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t ce = hipSuccess;
-        for (int r = 0; r < chunk && ce == hipSuccess; ++r)
-            ce = mnad_launch_step(ds, dt, ntiles, base, r * spl, kern, rz, br, c->stream);
-        if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk * spl, c->stream);
-        hipError_t ee = hipStreamEndCapture(c->stream, &graph);
-        HIPCHK(ce);
-        HIPCHK(ee);
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            HIPCHK(ie);
+        for (size_t k = 0; k < c->graphs.size(); ++k) {
+            const GraphEntry& e = c->graphs[k];
+            if (e.kern == kern && e.rz == rz && e.brows == br && e.ntiles == ntiles &&
+                e.chunk == chunk && e.spl == spl && e.ds == ds && e.dt == dt && e.base == base) {
+                exec = e.exec;
+                break;
+            }
+        }
+        if (!exec) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = hipSuccess;
+            for (int r = 0; r < chunk && ce == hipSuccess; ++r)
+                ce = mnad_launch_step(ds, dt, ntiles, base, r * spl, kern, rz, br, c->stream);
+            if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk * spl, c->stream);
+            hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+            HIPCHK(ce);
+            HIPCHK(ee);
+            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                HIPCHK(ie);
+            }
+            if (c->graphs.size() >= 8) {   // keep the most recent shapes
+                (void)hipGraphExecDestroy(c->graphs.front().exec);
+                (void)hipGraphDestroy(c->graphs.front().graph);
+                c->graphs.erase(c->graphs.begin());
+            }
+            c->graphs.push_back(GraphEntry{kern, rz, br, ntiles, chunk, spl, ds, dt, base, graph, exec});
         }
         const int full = nlaunch / chunk, rem = nlaunch - full * chunk;
-        hipError_t le = hipEventRecord(ev0, c->stream);   // after instantiation
-        for (int k = 0; k < full && le == hipSuccess; ++k) le = hipGraphLaunch(exec, c->stream);
-        for (int r = 0; r < rem && le == hipSuccess; ++r)
-            le = mnad_launch_step(ds, dt, ntiles, base, r * spl, kern, rz, br, c->stream);
-        hipError_t se = hipStreamSynchronize(c->stream);
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-        HIPCHK(le);
-        HIPCHK(se);
+        HIPCHK(hipEventRecord(ev0, c->stream));   // after instantiation
+        for (int k = 0; k < full; ++k) HIPCHK(hipGraphLaunch(exec, c->stream));
+        for (int r = 0; r < rem; ++r)
+            HIPCHK(mnad_launch_step(ds, dt, ntiles, base, r * spl, kern, rz, br, c->stream));
     }
     HIPCHK(hipEventRecord(ev1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
