@@ -43,11 +43,11 @@ struct ShotDev {
                                     // two-steps-per-launch kernel
     int nlev;
     // CPML, strip-compact: x strips = 2*npml columns x nz rows,
-    // z strips = 2*npml rows x nx columns.
+    // z strips = 2*npml rows of `pitch` floats (columns 0 .. nx-1 used).
     float* psx;                     // [9][nz * 2 npml]   (field*3 + level)
-    float* psz;                     // [9][2 npml * nx]
+    float* psz;                     // [9][2 npml * pitch]
     const float* cfx;               // [3][nz * 2 npml]   c1, h*c2 (signed), c3
-    const float* cfz;               // [3][2 npml * nx]
+    const float* cfz;               // [3][2 npml * pitch]
     // source (forward)
     int src_j, src_i;               // -1 if none
     const float* src_amp;           // [nt + 1]: a_src * w(s + 1); [nt] = 0

@@ -80,7 +80,7 @@ __global__ void k_setup_model(const double* __restrict__ v, int nz, int nx,
         const double dsig = 2.0 * sig0 * x;
         const double den = 1.0 + sig * (dt / 2.0);
         const int jz = top ? j : j - (nz - np2);
-        const size_t q = (size_t)jz * nx + i, ns = (size_t)np2 * nx;
+        const size_t q = (size_t)jz * pitch + i, ns = (size_t)np2 * pitch;
         cfz[q] = (float)((1.0 - sig * (dt / 2.0)) / den);
         cfz[ns + q] = (float)((top ? dz : -dz) * ((dsig * dt) / den));
         cfz[2 * ns + q] = (float)((sig * dt) / den);
@@ -175,9 +175,9 @@ __device__ __forceinline__ void finish_point(const ShotDev& d, int s, int j, int
             psi3(d.psx, d.cfx, ns, q, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
         }
         if (inz) {
-            const long ns = (long)2 * np * d.nx;
+            const long ns = (long)2 * np * d.pitch;
             const int jz = (j < np) ? j : j - (d.nz - 2 * np);
-            const long q = (long)jz * d.nx + i;
+            const long q = (long)jz * d.pitch + i;
             psi3(d.psz, d.cfz, ns, q, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
         }
     }
@@ -456,8 +456,8 @@ __global__ __launch_bounds__(BX * BZ) void k_step_pf(const ShotDev* __restrict__
         for (int k = 0; k < 9; ++k) px[k] = d.psx[k * nsx + qx];
     }
     if (inz) {
-        nsz = (long)2 * np * d.nx;
-        qz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.nx + i;
+        nsz = (long)2 * np * d.pitch;
+        qz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.pitch + i;
 #pragma unroll
         for (int k = 0; k < 3; ++k) cz[k] = d.cfz[k * nsz + qz];
 #pragma unroll
@@ -665,8 +665,8 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
     // strip-row origins: x strips [9][nz * 2 npml], z strips [9][2 npml * nx]
     const long nsx = (long)d.nz * 2 * np, rowx = (long)j * 2 * np;
     const unsigned bx = 4u * ((i < np) ? (unsigned)i : (unsigned)(i - (d.nx - 2 * np)));
-    const long nsz = (long)2 * np * d.nx;
-    const long rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.nx + i0;
+    const long nsz = (long)2 * np * d.pitch;
+    const long rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.pitch + i0;
     PsiW mx, mz;
     if (PF) {
         if (inz) psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
@@ -1084,8 +1084,8 @@ __global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ s
             for (int k = 0; k < 9; ++k) px[k] = d.psx[(pin * 9 + k) * nsx + qx];
         }
         if (inz) {
-            nsz = (long)2 * np * nx;
-            qz = (long)((j < np) ? j : j - (nz - 2 * np)) * nx + i;
+            nsz = (long)2 * np * P;
+            qz = (long)((j < np) ? j : j - (nz - 2 * np)) * P + i;
 #pragma unroll
             for (int k = 0; k < 3; ++k) cz[k] = d.cfz[k * nsz + qz];
 #pragma unroll
@@ -1232,8 +1232,8 @@ __device__ __forceinline__ PsiPt psi_point(const ShotDev& d, int j, int i, int i
     q.nsx = (long)d.nz * 2 * np;
     q.rowx = (long)j * 2 * np;
     q.bx = 4u * ((i < np) ? (unsigned)i : (unsigned)(i - (d.nx - 2 * np)));
-    q.nsz = (long)2 * np * d.nx;
-    q.rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.nx + i0;
+    q.nsz = (long)2 * np * d.pitch;
+    q.rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.pitch + i0;
     q.bz = 4u * lane;
     return q;
 }

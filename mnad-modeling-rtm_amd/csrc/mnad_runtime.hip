@@ -274,7 +274,8 @@ int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m)
     const int np = g->npml;
     if (np > 0) {
         CHK(dalloc(c, m.cfx, 3 * (size_t)g->nz * 2 * np * sizeof(float), true));
-        CHK(dalloc(c, m.cfz, 3 * (size_t)2 * np * g->nx * sizeof(float), true));
+        // z strips in rows of `pitch` floats: lane accesses stay 128-byte aligned
+        CHK(dalloc(c, m.cfz, 3 * (size_t)2 * np * m.pitch * sizeof(float), true));
     }
     HIPCHK(mnad_launch_setup_model(dv.as<double>(), g->nz, g->nx, m.pitch, np, g->dx, g->dz,
                                    g->dt, m.rx.as<float>(), m.cfx.as<float>(),
@@ -362,7 +363,7 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
         // two-steps-per-launch kernels keep two sets of memories (read one, write the other)
         const size_t planes = s.d.nlev == 4 ? 18 : 9;
         CHK(dalloc(c, s.psx, planes * g.nz * 2 * np * sizeof(float), true));
-        CHK(dalloc(c, s.psz, planes * 2 * np * g.nx * sizeof(float), true));
+        CHK(dalloc(c, s.psz, planes * 2 * np * m.pitch * sizeof(float), true));
         s.d.psx = s.psx.as<float>();
         s.d.psz = s.psz.as<float>();
         s.d.cfx = m.cfx.as<float>();
