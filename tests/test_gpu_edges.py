@@ -1,0 +1,137 @@
+"""Edge cases of the HIP path against the FP64 oracle (relative L2 <= 1e-4 on
+gathers and final fields) for the default kernel and the four-column and
+two-step kernels: no CPML, the shortest runs, no receivers, sources on the
+grid corners, receivers on every boundary row/column, odd step counts for the
+two-step kernels (the last launch runs one step), and a grid just large
+enough for its CPML strips."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+KERNELS = [(4, 1, 4), (6, 1, 2), (7, 3, 16)]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _model(nz, nx, seed):
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz)[:, None]
+    v = 1800.0 + 900.0 * (z > nz // 2) + 60.0 * rng.standard_normal((nz, nx))
+    return np.clip(v, 1500.0, 3000.0)
+
+
+def _wavelet(nt, dt, fc=25.0):
+    t = np.arange(nt) * dt
+    tau = np.pi * (t - 1.1 / fc) * fc
+    return (1 - 2 * tau**2) * np.exp(-tau**2)
+
+
+def _compare(mnad, oracle, kernel, v, npml, w, zs, xs, zr, xr, dx=10.0, dz=10.0, dt=1e-3):
+    g = mnad.Grid(v.shape[0], v.shape[1], npml, dx, dz, dt)
+    go = oracle.grid(v.shape[0], v.shape[1], npml, dx, dz, dt)
+    o = oracle.forward(go, v, w, zs, xs, zr, xr)
+    ctx = mnad.Context(0)
+    try:
+        k, rz, br = kernel
+        ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
+        r = ctx.fwd(g, v, w, zs, xs, zr, xr)
+    finally:
+        ctx.close()
+    keys = ("U", "V", "W") + (("gather",) if len(zr) else ())
+    for key in keys:
+        ref = np.asarray(o[key], float)
+        if np.linalg.norm(ref) > 0:
+            e = rel_l2(r[key], ref)
+        else:   # nothing has reached these points yet: both must be zero
+            e = float(np.linalg.norm(np.asarray(r[key], float)))
+        assert e < TOL, (key, e)
+    return r
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_no_cpml(mnad, oracle, kernel):
+    _say(f"edge: npml 0, kernel {kernel}")
+    v = _model(70, 300, 1)
+    w = _wavelet(150, 1e-3)
+    xr = np.arange(0, 300, 3)
+    _compare(mnad, oracle, kernel, v, 0, w, 35, 150, np.full(xr.size, 2), xr)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("nt", [1, 2, 3])
+def test_shortest_runs(mnad, oracle, kernel, nt):
+    _say(f"edge: nt {nt}, kernel {kernel}")
+    v = _model(40, 270, 2)
+    w = _wavelet(40, 1e-3)[10:10 + nt]   # non-zero first samples
+    # receivers far from the source and around it (the only ones a short run reaches)
+    xr = np.concatenate([np.arange(5, 265, 7), np.arange(126, 135)])
+    zr = np.concatenate([np.full(38, 6), np.full(9, 12)])
+    _compare(mnad, oracle, kernel, v, 5, w, 12, 130, zr, xr)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_no_receivers(mnad, oracle, kernel):
+    _say(f"edge: no receivers, kernel {kernel}")
+    v = _model(40, 270, 3)
+    w = _wavelet(90, 1e-3)
+    r = _compare(mnad, oracle, kernel, v, 5, w, 12, 130, np.zeros(0, int), np.zeros(0, int))
+    assert r["gather"].shape == (90, 0)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("corner", ["nw", "se"])
+def test_source_on_grid_corner_and_boundary_receivers(mnad, oracle, kernel, corner):
+    """Source at (0, 0) or (nz-1, nx-1), inside both CPML strips; receivers
+    on the first and last rows and columns."""
+    _say(f"edge: source corner {corner}, kernel {kernel}")
+    nz, nx = 36, 280
+    v = _model(nz, nx, 4)
+    w = _wavelet(120, 1e-3)
+    zs, xs = (0, 0) if corner == "nw" else (nz - 1, nx - 1)
+    zr = np.concatenate([np.zeros(40, int), np.full(40, nz - 1), np.arange(nz), np.arange(nz)])
+    xr = np.concatenate([np.arange(0, 280, 7), np.arange(3, 280, 7), np.zeros(nz, int),
+                         np.full(nz, nx - 1)])
+    _compare(mnad, oracle, kernel, v, 6, w, zs, xs, zr, xr)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_minimal_grid_for_cpml(mnad, oracle, kernel):
+    """nz = nx = 2 npml + 1: a single interior point."""
+    _say(f"edge: minimal grid, kernel {kernel}")
+    v = _model(13, 13, 5)
+    w = _wavelet(60, 1e-3)
+    _compare(mnad, oracle, kernel, v, 6, w, 6, 6, [6, 0, 12], [6, 12, 0])
+
+
+@pytest.mark.parametrize("kernel,rz,br", [(7, 3, 16), (7, 5, 8), (3, 1, 16)])
+def test_two_step_kernels_odd_step_count_rtm(mnad, kernel, rz, br):
+    """rtm_rect has nt = 187: the last launch of a two-step kernel runs one step
+    (and stores the level s+1 CPML memories itself)."""
+    _say(f"edge: odd nt RTM, kernel {kernel}")
+    f = golden("rtm_rect")
+    assert f["w"].size % 2 == 1
+    v = f["vs"]
+    g = mnad.Grid(v.shape[0], v.shape[1], int(f["npml"]), float(f["dx"]), float(f["dz"]),
+                  float(f["dt"]))
+    ctx = mnad.Context(0)
+    try:
+        ctx.set_tuning(kernel=kernel, rows_per_thread=rz, block_rows=br)
+        sid = ctx.stack_create(g)
+        gat = ctx.rtm_shots(sid, g, f["v"], f["vs"], f["w"], [int(f["zs0"])], [int(f["xs0"])],
+                            [(f["zr0"], f["xr0"])], snap_fmt=mnad.SNAP_F32, want_gathers=True)
+        img, ill, ns = ctx.stack_image(sid, g)
+        ctx.stack_destroy(sid)
+    finally:
+        ctx.close()
+    assert rel_l2(gat[0][0], f["data0"]) < TOL
+    assert rel_l2(gat[1][0], f["datas0"]) < TOL
+    assert rel_l2(img, f["img"]) < 1e-3
