@@ -295,6 +295,31 @@ def run_c4(args, mnad):
     print(json.dumps(out), flush=True)
 
 
+def run_probe(args, mnad):
+    """Two forward shots x 32 steps on the workload's grid, run twice in one
+    launch sequence with the given step kernel: a short, fixed sequence for
+    hardware-counter runs of one kernel variant."""
+    k, rz, br = (int(x) for x in args.probe.split(","))
+    if args.workload == "c5":
+        wl = c5_workload(32, 2)
+    elif args.workload == "c3":
+        wl = c3_workload(32, 2)
+    else:
+        wl = c2_multi_workload(32, 2)
+    v = wl["v"]
+    g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    with mnad.Context(0) as ctx:
+        ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
+        for _ in range(2):
+            ctx.fwd_shots(g, v, wl["w"], wl["zs"], wl["xs"], wl["recs"], batch=2)
+        ms, n, b = ctx.timing()
+    print(json.dumps({"probe": args.probe, "workload": args.workload,
+                      "grid_padded": [int(v.shape[0]), int(v.shape[1])],
+                      "us_per_launch": round(ms * 1e3 / max(n, 1), 2), "launches": int(n),
+                      "kernel": kernel_name(dict(kernel=k, rows_per_thread=rz, block_rows=br))}),
+          flush=True)
+
+
 def cpu_baseline(wl, steps):
     """FP64 oracle (one thread) on the first `steps` forward steps of the
     true-model shot of the same workload."""
@@ -339,6 +364,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--probe", default="",
+                    help="KERNEL,ROWS,BLOCK_ROWS: time 2 forward shots x 32 steps (twice) of the "
+                         "workload's grid with that step kernel (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (x3) to calibrate the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -361,6 +389,9 @@ def main():
 
     from __graft_entry__ import load_package
     mnad = load_package()
+    if args.probe:
+        run_probe(args, mnad)
+        return
     if args.workload == "c4":
         run_c4(args, mnad)
         return
