@@ -37,7 +37,7 @@ def kernel_name(t):
     k, r, b = t["kernel"], t["rows_per_thread"], t["block_rows"]
     return {0: f"mnad::k_step<{r}, {b}>", 1: "mnad::k_step_pf", 2: f"mnad::k_step_lds<{4 * r}>",
             3: "mnad::k_step2", 4: f"mnad::k_stepw<{b}, false>", 5: f"mnad::k_stepw<{b}, true>",
-            6: f"mnad::k_stepx<{b}>"}.get(k, f"kernel {k}")
+            6: f"mnad::k_stepx<{b}>", 7: f"mnad::k_step2w<{4 * r}, {b}>"}.get(k, f"kernel {k}")
 
 
 def c2_workload(nt, npml=30):
@@ -320,18 +320,41 @@ def run_probe(args, mnad):
           flush=True)
 
 
-def cpu_baseline(wl, steps):
-    """FP64 oracle (one thread) on the first `steps` forward steps of the
-    true-model shot of the same workload."""
+def _cpu_shot(args):
+    """One oracle shot sample in a worker process: (points * steps, seconds)."""
+    wl, steps = args
     from oracle import oracle as O
     v = wl["v"]
     g = O.grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
     t0 = time.perf_counter()
     O.forward(g, v, wl["w"][:steps], wl["zs"], wl["xs"], wl["zr"], wl["xr"], want_state=False)
-    el = time.perf_counter() - t0
-    return dict(value=v.size * steps / el / 1e9, unit="Gpt-updates/s", cores=1, kind="port",
-                sample=f"FP64 oracle forward, true model, first {steps} of {wl['w'].size} steps, "
-                       f"1 thread ({el:.1f} s)")
+    return v.size * steps, time.perf_counter() - t0
+
+
+def cpu_baseline(wl, steps, procs=0):
+    """FP64 oracle on the first `steps` forward steps of the true-model shot of
+    the same workload: once on one core, then one such shot per core on `procs`
+    cores at once (the reference's parfor over shots); the aggregate rate is
+    the value.  procs = 0: the host's cores, at most 16 (one GPU's share)."""
+    from multiprocessing import get_context
+    sample = dict(v=wl["v"], w=wl["w"], npml=wl["npml"], dx=wl["dx"], dt=wl["dt"],
+                  zs=wl["zs"], xs=wl["xs"], zr=wl["zr"], xr=wl["xr"])
+    n1, t1 = _cpu_shot((sample, steps))
+    single = n1 / t1 / 1e9
+    procs = procs or min(16, os.cpu_count() or 1)
+    if procs > 1:
+        with get_context("spawn").Pool(procs) as pool:
+            t0 = time.perf_counter()
+            res = pool.map(_cpu_shot, [(sample, steps)] * procs)
+            el = time.perf_counter() - t0
+        value = sum(r[0] for r in res) / el / 1e9
+    else:
+        value, el = single, t1
+    return dict(value=value, unit="Gpt-updates/s", cores=procs, kind="port",
+                single_core_value=single,
+                sample=f"FP64 oracle forward, true model, first {steps} of {wl['w'].size} steps; "
+                       f"1 shot on 1 core ({t1:.1f} s), then {procs} such shots on {procs} "
+                       f"cores at once ({el:.1f} s)")
 
 
 def main():
@@ -362,6 +385,8 @@ def main():
     ap.add_argument("--snap", default="f16", choices=["f32", "f16", "bf16"])
     ap.add_argument("--npml", type=int, default=30, help="C2: CPML cells (30 in the workload)")
     ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="CPU baseline processes (0 = host cores, at most 16)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--probe", default="",
@@ -496,7 +521,7 @@ def main():
         out["speedup_vs_plain"] = round(plain_ms / ms_per_step, 3)
     if not args.no_cpu and args.cpu_steps > 0:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(wl, min(args.cpu_steps, nt))
+        out["cpu_baseline"] = cpu_baseline(wl, min(args.cpu_steps, nt), args.cpu_procs)
     print(json.dumps(out), flush=True)
 
 
