@@ -152,8 +152,13 @@ def run_multi(args, mnad, wl, label):
         sid = ctx.stack_create(g)
         ctx.timing(reset=True)
         t0 = time.perf_counter()
-        ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], wl["zs"], wl["xs"], wl["recs"], snap_fmt=fmt,
-                      batch=args.batch)
+        # slices of 16 shots (one or more launch groups each), with a progress line per slice
+        for k0 in range(0, nshots, 16):
+            k1 = min(nshots, k0 + 16)
+            ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], wl["zs"][k0:k1], wl["xs"][k0:k1],
+                          wl["recs"][k0:k1], snap_fmt=fmt, batch=args.batch)
+            print(f"# shots {k1}/{nshots} done ({time.perf_counter() - t0:.1f} s)",
+                  file=sys.stderr, flush=True)
         el = time.perf_counter() - t0
         k_ms, k_launches, k_bytes = ctx.timing(reset=True)
         img, ill, ns = ctx.stack_image(sid, g)
