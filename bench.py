@@ -32,6 +32,19 @@ METRIC = "Gpt-updates/s & RTM shots/hr vs plain FP32 at ≤1e-4 rel err; p50/p99
 HBM_PEAK_GBS = 8000.0
 
 
+# HBM bytes per step-kernel launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes (each scaled by what the counter reads for a 256 MiB device copy:
+# x2.0 fetch, x1.0 write), keyed by (workload, step-kernel symbol).  The
+# counters cannot be read from inside a timed run, so the line reports the
+# figure of the committed counter runs when the same kernel runs the same
+# workload, and null otherwise.
+MEASURED_TRAFFIC = {
+    # C2 RTM shot, nt = 300: forward (2 shots) 67.75 MB, adjoint 34.70 MB,
+    # equal launch counts -> 51.22 MB per launch (profiles/k_stepw_c2_rtm_shot.md)
+    ("c2", "mnad::k_stepw<4, false>"): 51.22e6,
+}
+
+
 def kernel_name(t):
     """Device symbol (as rocprofv3 prints it) of the step kernel a tuning runs."""
     k, r, b = t["kernel"], t["rows_per_thread"], t["block_rows"]
@@ -517,7 +530,10 @@ def main():
                    "snapshots": args.snap, "updates_per_step": 3 * N * nt},
         "shots_per_hour": round(3600.0 / (ms_per_step / 1e3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, kname)),
+                     "traffic_unit": "bytes per launch (rocprofv3 counter runs)",
+                     "algorithmic_bytes_per_launch": round(k_bytes / max(k_launches, 1)),
                      "kernel": kname, "avg_launch_us": round(avg_launch_us, 3),
                      "launches": int(k_launches)},
     }
