@@ -314,22 +314,25 @@ def run_c4(args, mnad):
 
 
 def run_probe(args, mnad):
-    """Two forward shots x 32 steps on the workload's grid, run twice in one
-    launch sequence with the given step kernel: a short, fixed sequence for
-    hardware-counter runs of one kernel variant."""
-    k, rz, br = (int(x) for x in args.probe.split(","))
+    """Forward shots (two, or the fourth number of --probe) x 32 steps on the
+    workload's grid, run twice in one launch sequence with the given step
+    kernel: a short, fixed sequence for hardware-counter runs of one kernel
+    variant."""
+    p = [int(x) for x in args.probe.split(",")]
+    k, rz, br = p[:3]
+    ns = p[3] if len(p) > 3 else 2
     if args.workload == "c5":
-        wl = c5_workload(32, 2)
+        wl = c5_workload(32, ns)
     elif args.workload == "c3":
-        wl = c3_workload(32, 2)
+        wl = c3_workload(32, ns)
     else:
-        wl = c2_multi_workload(32, 2)
+        wl = c2_multi_workload(32, ns)
     v = wl["v"]
     g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
     with mnad.Context(0) as ctx:
         ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
         for _ in range(2):
-            ctx.fwd_shots(g, v, wl["w"], wl["zs"], wl["xs"], wl["recs"], batch=2)
+            ctx.fwd_shots(g, v, wl["w"], wl["zs"], wl["xs"], wl["recs"], batch=ns)
         ms, n, b = ctx.timing()
     print(json.dumps({"probe": args.probe, "workload": args.workload,
                       "grid_padded": [int(v.shape[0]), int(v.shape[1])],
@@ -408,8 +411,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--probe", default="",
-                    help="KERNEL,ROWS,BLOCK_ROWS: time 2 forward shots x 32 steps (twice) of the "
-                         "workload's grid with that step kernel (for counter runs)")
+                    help="KERNEL,ROWS,BLOCK_ROWS[,SHOTS]: time SHOTS (default 2) forward shots "
+                         "x 32 steps (twice) of the workload's grid with that step kernel "
+                         "(for counter runs)")
+    ap.add_argument("--tuning", default="",
+                    help="C2 / C5 window: step kernel K,RZ,BR for the timed shots "
+                         "(default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (x3) to calibrate the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -466,6 +473,9 @@ def main():
     print(f"# C2 {v.shape[0]}x{v.shape[1]} nt={nt} snap={args.snap}", file=sys.stderr, flush=True)
 
     with mnad.Context(0) as ctx:
+        if args.tuning:
+            k, rz, br = (int(x) for x in args.tuning.split(","))
+            ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
         sid = ctx.stack_create(g)
         kname = kernel_name(ctx.tuning())
 

@@ -102,6 +102,8 @@ int mnad_ctx_create(int device, mnad_ctx** out)
         c->brows = 4;
     }
     if (const char* s = getenv("MNAD_STRIP_FIRST")) c->strip_first = atoi(s) != 0;
+    if (const char* s = getenv("MNAD_STRIP_WEIGHT")) c->strip_weight = std::max(1.0, atof(s));
+    if (const char* s = getenv("MNAD_X4_WEIGHT")) c->x4_weight = std::max(1.0, atof(s));
     if (const char* s = getenv("MNAD_AUTOTUNE")) c->autotune_on_use = atoi(s) != 0;
     *out = c;
     return MNAD_OK;

@@ -110,6 +110,10 @@ struct mnad_ctx {
     int rz = 1;                 // rows per thread
     int brows = 4;              // thread rows per block (tile = 64 x brows*rz)
     int strip_first = 1;        // dispatch CPML-strip tiles first
+    // XCD balance of one-step kernels: estimated work of a CPML-strip tile and
+    // of a four-column tile, in interior one-point tiles
+    double strip_weight = 2.0;
+    double x4_weight = 2.5;
     size_t budget = 0;          // bytes usable for batches (0 = from free memory)
     int autotune_on_use = 0;    // batched calls tune an unseen grid shape first
     std::map<std::tuple<int, int, int>, mnad::TunePick> tuned;   // (nz, nx, npml)

@@ -649,8 +649,11 @@ __device__ __forceinline__ void snap_image_w(const ShotDev& d, int s, long qrow,
 
 // One point per lane: the wave owns columns i0 .. i0+63 of row j.
 // PF: issue the CPML loads of strip points together with the neighbourhood
-// loads (one memory round trip for a strip point instead of two, at the cost
-// of the registers that hold them across the increment arithmetic).
+// loads (one memory round trip for a strip point instead of two).  One set of
+// memories is prefetched: the z set in z-strip rows (a wave-uniform choice),
+// the x set in the other rows; corner points of z-strip rows load their x set
+// after the increment.  Holding one set across the increment arithmetic keeps
+// the kernel at 8 waves per SIMD (two sets would take it to 6).
 // SKIPX4: points of the four-column region (rows npml .. nz-npml-1, columns
 // npml .. x4hi-1) belong to step_x4 and are left alone.
 template <bool PF, bool SKIPX4>
@@ -667,10 +670,10 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
     const unsigned bx = 4u * ((i < np) ? (unsigned)i : (unsigned)(i - (d.nx - 2 * np)));
     const long nsz = (long)2 * np * d.pitch;
     const long rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.pitch + i0;
-    PsiW mx, mz;
+    PsiW mp;   // PF: the prefetched set (z in z-strip rows, else x)
     if (PF) {
-        if (inz) psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
-        if (inx) psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
+        if (inz) psi_load(mp, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
+        else if (inx) psi_load(mp, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
     }
 
     const int cb = s & 1;
@@ -735,12 +738,16 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
             }
         }
         if (inx) {
-            if (!PF) psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
+            PsiW mx;
+            if (PF && !inz) mx = mp;
+            else psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
             psi_step(mx, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
             psi_store(mx, gp(d.psx) + rowx, nsx, bx);
         }
         if (inz) {
-            if (!PF) psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
+            PsiW mz;
+            if (PF) mz = mp;
+            else psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
             psi_step(mz, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
             psi_store(mz, gp(d.psz) + rowz, nsz, 4u * lane);
         }

@@ -503,6 +503,10 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
 {
     std::vector<TileRef> list;
     std::vector<char> strip;
+    std::vector<double> work;   // estimated work of each listed tile, in one-point tiles
+    // Two-step tiles are cut by count: weighting their strip tiles measured
+    // slower at two shots per launch and made no difference at four or more.
+    const double sw = two_step_kernel(c->kernel) ? 1.0 : c->strip_weight;
     int tw, th, spl;
     tile_shape(c, &tw, &th, &spl);
     const bool x4 = c->kernel == 6;
@@ -518,27 +522,45 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
                       c1 = std::min(nx, c0 + tw);
             // kernel 6: one-point tiles lying wholly in the four-column interior are not listed
             if (x4 && r0 >= np && r1 <= nz - np && c0 >= np && c1 <= x4hi) continue;
+            const bool st = c0 < np || c1 > nx - np || r0 < np || r1 > nz - np;
             list.push_back(TileRef{k, t});
-            strip.push_back(c0 < np || c1 > nx - np || r0 < np || r1 > nz - np);
+            strip.push_back(st);
+            work.push_back(st ? sw : 1.0);
         }
         if (x4) {
+            // a four-column tile: 248 columns at about 2/3 of the one-point cost per point
             const int nb = (nz - 2 * np + th - 1) / th;
             for (int u = 0; u < nb * h[k].nx4; ++u) {
                 list.push_back(TileRef{k, ntx * ntz + u});
                 strip.push_back(0);
+                work.push_back(c->x4_weight);
             }
         }
     }
-    // XCD-aware order: workgroup b is dispatched to XCD b % 8, so XCD x gets
-    // the contiguous run [x*L/8, (x+1)*L/8) of the spatial tile list and
-    // vertically adjacent tiles share halo rows in the same L2.  Within each
-    // run the CPML-strip tiles (about twice the memory traffic) go first.
+    // XCD-aware order: workgroup b is dispatched to XCD b % 8.  The spatial
+    // tile list is cut into 8 contiguous runs of equal estimated work and XCD
+    // x takes run x, so vertically adjacent tiles share halo rows in the same
+    // L2.  Cutting by work rather than by tile count matters because a CPML
+    // strip tile costs about twice an interior one (three times the bytes):
+    // equal counts hand the top and bottom strips of a shot to two XCDs,
+    // whose longer runs then set the launch time (a one-shot launch of the
+    // 561 x 1061 grid: 8.7 us by count, 7.8 us by work).  Within each run
+    // the strip tiles go first.
     const int nx8 = 8;
-    const size_t L = (list.size() + nx8 - 1) / nx8 * nx8, per = L / nx8;
-    table.assign(L, TileRef{-1, 0});
+    double total = 0.0;
+    for (double w : work) total += w;
+    std::vector<std::vector<size_t>> runs(nx8);
+    double acc = 0.0;
+    for (size_t q = 0; q < list.size(); ++q) {
+        const double mid = acc + 0.5 * work[q];
+        acc += work[q];
+        runs[std::min(nx8 - 1, (int)(mid * nx8 / total))].push_back(q);
+    }
+    size_t per = 0;
+    for (const auto& run : runs) per = std::max(per, run.size());
+    table.assign(per * nx8, TileRef{-1, 0});
     for (int x = 0; x < nx8; ++x) {
-        std::vector<size_t> run;
-        for (size_t q = x * per; q < std::min(list.size(), (x + 1) * per); ++q) run.push_back(q);
+        auto& run = runs[x];
         if (c->strip_first)
             std::stable_partition(run.begin(), run.end(), [&](size_t q) { return strip[q] != 0; });
         for (size_t m = 0; m < run.size(); ++m) table[m * nx8 + x] = list[run[m]];

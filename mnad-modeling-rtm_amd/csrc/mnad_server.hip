@@ -433,6 +433,8 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
     s->c->rz = two ? 1 : ctx->rz;
     s->c->brows = two ? 4 : ctx->brows;
     s->c->strip_first = ctx->strip_first;
+    s->c->strip_weight = ctx->strip_weight;
+    s->c->x4_weight = ctx->x4_weight;
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) (void)hipGetLastError();
     s->budget = hbm_budget ? (size_t)hbm_budget : (size_t)(0.8 * (double)freeb);
