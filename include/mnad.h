@@ -171,8 +171,9 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  * kernel 4 (default) = one grid row per wavefront, row origins in scalar
  *   registers and one shared lane offset for every load and store: block_rows
  *   1, 2, 4 or 8, 1 row;
- * kernel 5 = kernel 4 with the CPML loads of strip points issued first:
- *   block_rows 1 or 4, 1 row;
+ * kernel 5 = kernel 4 with one set of CPML memories of strip points (z in
+ *   z-strip rows, else x) loaded with the neighbourhood: block_rows 1 or 4,
+ *   1 row;
  * kernel 6 = kernel 4 on the CPML strips, four columns per lane (16-byte
  *   loads, neighbours by cross-lane shifts) in the interior: block_rows 1, 2
  *   or 4, 1 row;

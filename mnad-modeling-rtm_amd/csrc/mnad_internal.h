@@ -105,7 +105,7 @@ struct mnad_ctx {
     int next_stack = 1;
     int graph_chunk = 64;
     int kernel = 4;             // step kernel: 0 registers, 1 prefetch, 2 LDS, 3 two steps,
-                                // 4 wave rows, 5 wave rows + CPML prefetch, 6 four columns,
+                                // 4 wave rows, 5 wave rows + one CPML set prefetched, 6 four columns,
                                 // 7 two steps, wave rows + LDS
     int rz = 1;                 // rows per thread
     int brows = 4;              // thread rows per block (tile = 64 x brows*rz)
