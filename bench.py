@@ -39,9 +39,9 @@ HBM_PEAK_GBS = 8000.0
 # figure of the committed counter runs when the same kernel runs the same
 # workload, and null otherwise.
 MEASURED_TRAFFIC = {
-    # C2 RTM shot, nt = 300: forward (2 shots) 67.76 MB, adjoint 34.66 MB,
-    # equal launch counts -> 51.21 MB per launch (profiles/k_stepw_c2_rtm_shot.md)
-    ("c2", "mnad::k_stepw<4, false>"): 51.21e6,
+    # C2 RTM shot, nt = 300: forward (2 shots) 69.68 MB, adjoint 35.58 MB,
+    # equal launch counts -> 52.63 MB per launch (profiles/k_stepw_c2_rtm_shot.md)
+    ("c2", "mnad::k_stepw<1, false>"): 52.63e6,
 }
 
 

@@ -170,7 +170,7 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  *   tiles), shots keep four time levels -- set it before creating shots;
  * kernel 4 (default) = one grid row per wavefront, row origins in scalar
  *   registers and one shared lane offset for every load and store: block_rows
- *   1, 2, 4 or 8, 1 row;
+ *   1 (default), 2, 4 or 8, 1 row;
  * kernel 5 = kernel 4 with one set of CPML memories of strip points (z in
  *   z-strip rows, else x) loaded with the neighbourhood: block_rows 1 or 4,
  *   1 row;

@@ -97,9 +97,9 @@ int mnad_ctx_create(int device, mnad_ctx** out)
     if (const char* s = getenv("MNAD_RZ")) c->rz = atoi(s);
     if (const char* s = getenv("MNAD_BROWS")) c->brows = atoi(s);
     if (!tuning_ok(c->kernel, c->rz, c->brows)) {
-        c->kernel = 4;
-        c->rz = 1;
-        c->brows = 4;
+        c->kernel = kDefaultKernel;
+        c->rz = kDefaultRows;
+        c->brows = kDefaultBrows;
     }
     if (const char* s = getenv("MNAD_STRIP_FIRST")) c->strip_first = atoi(s) != 0;
     if (const char* s = getenv("MNAD_STRIP_WEIGHT")) c->strip_weight = std::max(1.0, atof(s));

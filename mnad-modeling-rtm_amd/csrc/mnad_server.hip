@@ -429,9 +429,9 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
     }
     // the server steps one time level per launch (admission at any step)
     const bool two = two_step_kernel(ctx->kernel);
-    s->c->kernel = two ? 4 : ctx->kernel;
-    s->c->rz = two ? 1 : ctx->rz;
-    s->c->brows = two ? 4 : ctx->brows;
+    s->c->kernel = two ? kDefaultKernel : ctx->kernel;
+    s->c->rz = two ? kDefaultRows : ctx->rz;
+    s->c->brows = two ? kDefaultBrows : ctx->brows;
     s->c->strip_first = ctx->strip_first;
     s->c->strip_weight = ctx->strip_weight;
     s->c->x4_weight = ctx->x4_weight;

@@ -14,7 +14,7 @@ from conftest import golden, rel_l2
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
-KERNELS = [(4, 1, 4), (5, 1, 4), (6, 1, 2), (7, 3, 16)]
+KERNELS = [(4, 1, 1), (4, 1, 4), (5, 1, 4), (6, 1, 2), (7, 3, 16)]
 
 
 def _say(msg):
