@@ -9,6 +9,7 @@
 // the *_gen.m index ranges do (neighbours beyond the padded grid omitted).
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "mnad_stencil.h"
@@ -25,6 +26,9 @@ enum Mode : int { MODE_FWD = 0, MODE_ADJ = 1 };
 // Everything one shot needs inside the step kernel.  Lives in device memory;
 // one entry per shot of a batch.
 struct ShotDev {
+    // Everything a step kernel needs before its first field load sits in the
+    // first 192 bytes (three scalar-cache lines), so a wave fetches it in one
+    // round of scalar loads.
     int nz, nx, pitch, npml;        // padded grid
     int nt;                         // number of steps
     int mode;                       // MODE_FWD / MODE_ADJ
@@ -34,27 +38,25 @@ struct ShotDev {
     // below nt1, interior tiles from nt1 on, nx4 per band of rows; they update
     // columns [npml, x4hi) of rows [npml, nz - npml), starting at x4lo
     int nt1, nx4, x4lo, x4hi;
-    float kap;                      // rz / rx = dx^2 / dz^2
-    float inv_dt2;                  // 1 / dt^2 (snapshot d2u)
-    IncrC ic;                       // increment weights for this kap (mnad_incr)
+    int nlev;
+    int nrec;                       // receivers (unique points)
+    int src_j, src_i;               // source (forward), -1 if none
     const float* rx;                // a / dx^2 per point, grid storage
     float* f[4][3];                 // state [level][U, Vs, Ws], grid storage: time level
                                     // t lives in f[t % nlev]; nlev = 2, or 4 for the
                                     // two-steps-per-launch kernel
-    int nlev;
+    const int* rrow;                // [nz]: offset into rcol, or -1
+    const int* rcol;                // [nrows_with_receivers * nx]: u or -1
+    float kap;                      // rz / rx = dx^2 / dz^2
+    float inv_dt2;                  // 1 / dt^2 (snapshot d2u)
+    IncrC ic;                       // increment weights for this kap (mnad_incr)
     // CPML, strip-compact: x strips = 2*npml columns x nz rows,
     // z strips = 2*npml rows of `pitch` floats (columns 0 .. nx-1 used).
     float* psx;                     // [9][nz * 2 npml]   (field*3 + level)
     float* psz;                     // [9][2 npml * pitch]
     const float* cfx;               // [3][nz * 2 npml]   c1, h*c2 (signed), c3
     const float* cfz;               // [3][2 npml * pitch]
-    // source (forward)
-    int src_j, src_i;               // -1 if none
     const float* src_amp;           // [nt + 1]: a_src * w(s + 1); [nt] = 0
-    // receivers (unique points)
-    int nrec;
-    const int* rrow;                // [nz]: offset into rcol, or -1
-    const int* rcol;                // [nrows_with_receivers * nx]: u or -1
     float* gather;                  // forward: [nt][nrec]
     const float* radj;              // adjoint: [nt][nrec] a_u * d(tt + 1)
     // snapshots (forward) / imaging (adjoint)
@@ -66,6 +68,8 @@ struct ShotDev {
     float* img;                     // [nzi * nxi] Kahan sum
     float* imgc;                    // [nzi * nxi] Kahan compensation
 };
+
+static_assert(offsetof(ShotDev, rcol) + sizeof(void*) <= 192, "hot descriptor fields");
 
 struct TileRef {
     int shot;   // -1: padding entry, the block exits

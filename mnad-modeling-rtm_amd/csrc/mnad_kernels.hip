@@ -676,12 +676,14 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
         else if (inx) psi_load(mp, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
     }
 
-    const int cb = s & 1;
+    // levels s and s-1: f[s & 1] and f[(s & 1) ^ 1], selected from both sets
+    // (already in SGPRs) rather than loaded by index
+    const bool odd = s & 1;
     const long P = d.pitch;
     const long org = (long)(j + MNAD_ZO) * P + (i0 + MNAD_XO);   // storage of (j, i0)
-    const gfloat* U = gp(d.f[cb][0]) + org;
-    const gfloat* V = gp(d.f[cb][1]) + org;
-    const gfloat* W = gp(d.f[cb][2]) + org;
+    const gfloat* U = gp(odd ? d.f[1][0] : d.f[0][0]) + org;
+    const gfloat* V = gp(odd ? d.f[1][1] : d.f[0][1]) + org;
+    const gfloat* W = gp(odd ? d.f[1][2] : d.f[0][2]) + org;
     Nbh u, v, w;
     float vww, vee, wnn, wss;
     {
@@ -707,9 +709,9 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
         wss = ((W + 2 * P) + lane)[0];
     }
     const float rx = (gp(d.rx) + org)[lane];
-    gfloat* PU = gp(d.f[cb ^ 1][0]) + org;
-    gfloat* PV = gp(d.f[cb ^ 1][1]) + org;
-    gfloat* PW = gp(d.f[cb ^ 1][2]) + org;
+    gfloat* PU = gp(odd ? d.f[0][0] : d.f[1][0]) + org;
+    gfloat* PV = gp(odd ? d.f[0][1] : d.f[1][1]) + org;
+    gfloat* PW = gp(odd ? d.f[0][2] : d.f[1][2]) + org;
     const float pu = PU[lane], pv = PV[lane], pw = PW[lane];
 
     int r = -1;
@@ -775,15 +777,17 @@ __global__ __launch_bounds__(BX * BZR) void k_stepw(const ShotDev* __restrict__ 
                                                     const TileRef* __restrict__ tiles,
                                                     const int* __restrict__ base, int rel)
 {
+    // One exit test after one round of descriptor loads (a padding entry reads
+    // shot 0's descriptor and exits with the others): separate early exits
+    // would make each descriptor field a dependent round of scalar loads.  The
+    // test reads a field of each of the descriptor's first two cache lines.
     const TileRef tr = tiles[blockIdx.x];
-    if (tr.shot < 0) return;
-    const ShotDev& d = shots[tr.shot];
+    const ShotDev& d = shots[tr.shot < 0 ? 0 : tr.shot];
     const int s = *base + rel - d.t0;
-    if (s < 0 || s >= d.nt) return;
     const int tj = tr.tile / d.ntx;
     const int ti = tr.tile - tj * d.ntx;
     const int j = tj * BZR + __builtin_amdgcn_readfirstlane((int)threadIdx.y);   // wave row
-    if (j >= d.nz) return;
+    if ((tr.shot < 0) | (s < 0) | (s >= d.nt) | (j >= d.nz) | (d.rx == nullptr)) return;
     step_one<PF, false>(d, s, j, ti * BX, threadIdx.x);
 }
 
