@@ -60,7 +60,8 @@ def c2_workload(nt, npml=30):
     synth = importlib.import_module("mnad_amd.synth")
     dx, dt, fc = 10.0, 8e-4, 20.0
     V, Vs = synth.model_c2(seed=2024)
-    v, vs = synth.pad_edges(V, npml), synth.pad_edges(Vs, npml)
+    # column-major, as a MATLAB caller hands them over (no conversion per call)
+    v, vs = np.asfortranarray(synth.pad_edges(V, npml)), np.asfortranarray(synth.pad_edges(Vs, npml))
     w = synth.ricker(fc, dt, nt)
     nxp = V.shape[1]
     zs, xs = npml, npml + nxp // 2
@@ -81,7 +82,7 @@ def c3_workload(nt, nshots):
     synth = importlib.import_module("mnad_amd.synth")
     npml, dx, dt, fc = 30, 10.0, 8e-4, 25.0
     V, Vs = synth.model_c3(seed=3)
-    v, vs = synth.pad_edges(V, npml), synth.pad_edges(Vs, npml)
+    v, vs = np.asfortranarray(synth.pad_edges(V, npml)), np.asfortranarray(synth.pad_edges(Vs, npml))
     w = synth.ricker(fc, dt, nt)
     nxp = V.shape[1]
     xs_phys = np.rint(np.linspace(0, nxp - 1, nshots)).astype(int)
@@ -108,7 +109,7 @@ def c5_workload(nt, nshots, crop=False):
     if crop:
         z0, x0 = 0, (V.shape[1] - 1001) // 2
         V, Vs = V[z0:z0 + 501, x0:x0 + 1001], Vs[z0:z0 + 501, x0:x0 + 1001]
-    v, vs = synth.pad_edges(V, npml), synth.pad_edges(Vs, npml)
+    v, vs = np.asfortranarray(synth.pad_edges(V, npml)), np.asfortranarray(synth.pad_edges(Vs, npml))
     w = synth.ricker(fc, dt, nt)
     nxp = V.shape[1]
     if crop:
