@@ -119,6 +119,7 @@ void mnad_ctx_destroy(mnad_ctx* ctx)
         (void)hipStreamSynchronize(ctx->stream);
         ctx->kept.clear();
         ctx->stacks.clear();
+        ctx->models.clear();
         release_batch_cache(ctx);
         ctx->pool.trim();
         (void)hipStreamDestroy(ctx->stream);

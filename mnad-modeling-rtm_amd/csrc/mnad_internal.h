@@ -5,7 +5,9 @@
 
 #include <cstdarg>
 #include <cstddef>
+#include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -81,6 +83,30 @@ struct GraphEntry {
     hipGraphExec_t exec;
 };
 
+// Device arrays that depend on the velocity model only (a / dx^2 per point,
+// CPML coefficients), shared by every Model built from the same inputs.
+struct ModelBufs {
+    DevBuf rx, cfx, cfz;
+};
+
+// One entry of the context's model cache: the grid, a 128-bit digest of the
+// velocity values and the arrays built from them.
+struct ModelKey {
+    int nz, nx, npml;
+    double dx, dz, dt;
+    uint64_t ha, hb;
+    bool operator==(const ModelKey& o) const
+    {
+        return nz == o.nz && nx == o.nx && npml == o.npml && dx == o.dx && dz == o.dz &&
+               dt == o.dt && ha == o.ha && hb == o.hb;
+    }
+};
+struct ModelEntry {
+    ModelKey key;
+    std::shared_ptr<ModelBufs> bufs;
+};
+constexpr size_t kModelCache = 4;   // models kept per context
+
 // Image / illumination stack (compensated sums over shots).
 struct Stack {
     mnad_grid g{};
@@ -106,6 +132,9 @@ struct mnad_ctx {
     mnad::Pool pool;
     std::map<int, mnad::Kept> kept;
     std::map<int, mnad::Stack> stacks;
+    // most recently used last; a repeated velocity model (every shot of a survey)
+    // is validated and hashed on the host but not uploaded or set up again
+    std::vector<mnad::ModelEntry> models;
     int next_stack = 1;
     int graph_chunk = 64;
     int kernel = mnad::kDefaultKernel;   // step kernel: 0 registers, 1 prefetch, 2 LDS, 3 two steps,
@@ -144,7 +173,10 @@ struct Model {
     size_t elems = 0;              // floats per field (rows * pitch)
     double vmin = 0, vmax = 0;
     const double* vhost = nullptr;  // caller's column-major v, valid during the call
-    DevBuf rx, cfx, cfz;
+    std::shared_ptr<ModelBufs> bufs;   // shared with the context's model cache
+    float* rx() const { return bufs->rx.as<float>(); }
+    float* cfx() const { return bufs->cfx.as<float>(); }
+    float* cfz() const { return bufs->cfz.as<float>(); }
 };
 
 struct ShotSpec {

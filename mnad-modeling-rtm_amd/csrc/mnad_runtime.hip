@@ -247,13 +247,22 @@ int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m)
     CHK(check_grid(g));
     if (!v) return fail(MNAD_ERR_ARG, "velocity is NULL");
     const size_t n = (size_t)g->nz * g->nx;
+    // validation and a 128-bit digest of the values (two independent
+    // multiply-xorshift chains over the FP64 bit patterns) in one pass
     double vmin = v[0], vmax = v[0];
+    uint64_t ha = 0x243F6A8885A308D3ull, hb = 0x13198A2E03707344ull;
     for (size_t k = 0; k < n; ++k) {
         const double x = v[k];
         if (!std::isfinite(x) || !(x > 0))
             return fail(MNAD_ERR_ARG, "velocity must be finite and positive (v[%zu] = %g)", k, x);
         vmin = std::min(vmin, x);
         vmax = std::max(vmax, x);
+        uint64_t w;
+        std::memcpy(&w, &x, sizeof w);
+        ha = (ha ^ w) * 0x100000001B3ull;
+        ha ^= ha >> 29;
+        hb = (hb + w) * 0x9E3779B97F4A7C15ull;
+        hb ^= hb >> 31;
     }
     const double courant = vmax * g->dt / std::min(g->dx, g->dz);
     if (courant > 1.0)
@@ -268,20 +277,34 @@ int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m)
     m.pitch = ((MNAD_XO + g->nx + 2) + 31) / 32 * 32;
     m.rows = g->nz + 2 * MNAD_ZO + MNAD_ZPAD;
     m.elems = (size_t)m.rows * m.pitch;
+    const ModelKey key{g->nz, g->nx, g->npml, g->dx, g->dz, g->dt, ha, hb};
+    for (size_t k = 0; k < c->models.size(); ++k) {
+        if (c->models[k].key == key) {
+            ModelEntry e = c->models[k];
+            c->models.erase(c->models.begin() + k);
+            c->models.push_back(e);
+            m.bufs = e.bufs;
+            return MNAD_OK;
+        }
+    }
+    auto bufs = std::make_shared<ModelBufs>();
     DevBuf dv;
     CHK(upload(c, dv, v, n * sizeof(double)));
-    CHK(dalloc(c, m.rx, m.elems * sizeof(float), true));
+    CHK(dalloc(c, bufs->rx, m.elems * sizeof(float), true));
     const int np = g->npml;
     if (np > 0) {
-        CHK(dalloc(c, m.cfx, 3 * (size_t)g->nz * 2 * np * sizeof(float), true));
+        CHK(dalloc(c, bufs->cfx, 3 * (size_t)g->nz * 2 * np * sizeof(float), true));
         // z strips in rows of `pitch` floats: lane accesses stay 128-byte aligned
-        CHK(dalloc(c, m.cfz, 3 * (size_t)2 * np * m.pitch * sizeof(float), true));
+        CHK(dalloc(c, bufs->cfz, 3 * (size_t)2 * np * m.pitch * sizeof(float), true));
     }
     HIPCHK(mnad_launch_setup_model(dv.as<double>(), g->nz, g->nx, m.pitch, np, g->dx, g->dz,
-                                   g->dt, m.rx.as<float>(), m.cfx.as<float>(),
-                                   m.cfz.as<float>(), c->stream));
+                                   g->dt, bufs->rx.as<float>(), bufs->cfx.as<float>(),
+                                   bufs->cfz.as<float>(), c->stream));
     // dv goes back to the pool when this returns; the stream order keeps the
     // set-up kernel ahead of any later reuse of that memory.
+    m.bufs = bufs;
+    c->models.push_back(ModelEntry{key, bufs});
+    if (c->models.size() > kModelCache) c->models.erase(c->models.begin());
     return MNAD_OK;
 }
 
@@ -350,7 +373,7 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
     s.d.kap = (float)((g.dx * g.dx) / (g.dz * g.dz));
     incr_consts((g.dx * g.dx) / (g.dz * g.dz), s.d.ic);
     s.d.inv_dt2 = (float)(1.0 / (g.dt * g.dt));
-    s.d.rx = m.rx.as<float>();
+    s.d.rx = m.rx();
     s.d.ntx = (g.nx + 63) / 64;
     s.d.src_j = s.d.src_i = -1;
     s.d.nlev = two_step_kernel(c->kernel) ? 4 : 2;
@@ -366,8 +389,8 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
         CHK(dalloc(c, s.psz, planes * 2 * np * m.pitch * sizeof(float), true));
         s.d.psx = s.psx.as<float>();
         s.d.psz = s.psz.as<float>();
-        s.d.cfx = m.cfx.as<float>();
-        s.d.cfz = m.cfz.as<float>();
+        s.d.cfx = m.cfx();
+        s.d.cfz = m.cfz();
     }
     CHK(build_receivers(c, m, sp.nr, sp.zr, sp.xr, s));
     const size_t ni = (size_t)(g.nz - 2 * np) * (g.nx - 2 * np);

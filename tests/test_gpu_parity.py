@@ -302,6 +302,30 @@ def test_autotune_and_serial_paths(mnad, ctx):
     assert rel_l2(imgs[0], imgs[1]) < 1e-4
 
 
+def test_model_cache_follows_the_values(mnad, ctx):
+    """The context keeps the device arrays of recent velocity models, keyed by
+    the grid and a digest of the values.  The same values reuse them, changed
+    values (also in the same host buffer) rebuild them, more models than the
+    cache holds evict the oldest: every call equals a fresh context's result
+    bitwise."""
+    _say("model cache")
+    f = golden("fwd_rect")
+    g = _grid(mnad, f)
+    args = (f["w"], int(f["zs"]), int(f["xs"]), f["zr"], f["xr"])
+    scales = [1.0, 1.05, 0.97, 1.02, 0.99, 1.01]
+    fresh = mnad.Context(0)
+    try:
+        want = {sc: fresh.fwd(g, f["v"] * sc, *args, want_state=False)["gather"]
+                for sc in scales}
+    finally:
+        fresh.close()
+    buf = np.asfortranarray(f["v"].copy())
+    for sc in [1.0, 1.05, 1.0] + scales + [1.0, 1.05]:
+        buf[...] = f["v"] * sc            # same host buffer, new values
+        got = ctx.fwd(g, buf, *args, want_state=False)["gather"]
+        assert np.array_equal(got, want[sc]), sc
+
+
 def _c2_inputs(nt):
     import bench
     return bench.c2_workload(nt)
