@@ -86,10 +86,14 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
 
     const int sk = c->kernel, sr = c->rz, sb = c->brows;
     const bool log = getenv("MNAD_AUTOTUNE_LOG") != nullptr;   // one stderr line per probe
-    TunePick best{0, 1, 4, 1, 1e30};
-    for (int b = 1; b <= max_shots; b *= 2) {
+    TunePick best{0, 1, 4, 1, 1e30, {}};
+    constexpr int nv = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+    double curves[nv][8] = {};   // per variant: us per shot-step at 1, 2, 4, ... shots
+    int kbest = 0;
+    for (int b = 1, kb = 0; b <= max_shots && kb < 8; b *= 2, ++kb) {
         double best_b = 1e30;
-        for (const Variant& v : kVariants) {
+        for (int iv = 0; iv < nv; ++iv) {
+            const Variant& v = kVariants[iv];
             c->kernel = v.kernel;
             c->rz = v.rz;
             c->brows = v.brows;
@@ -106,7 +110,11 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
                 fprintf(stderr, "mnad_autotune %dx%d: kernel %d rows %d block_rows %d, %d shots: %.3f us/shot-step\n",
                         g.nz, g.nx, v.kernel, v.rz, v.brows, b, us);
             best_b = std::min(best_b, us);
-            if (us < best.us) best = TunePick{v.kernel, v.rz, v.brows, b, us};
+            curves[iv][kb] = us;
+            if (us < best.us) {
+                best = TunePick{v.kernel, v.rz, v.brows, b, us, {}};
+                kbest = iv;
+            }
         }
         // stop growing the batch once it no longer helps (working set past the cache)
         if (b > 1 && best_b > 1.05 * best.us) break;
@@ -114,6 +122,7 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
     c->kernel = sk;
     c->rz = sr;
     c->brows = sb;
+    for (int k = 0; k < 8; ++k) best.curve[k] = curves[kbest][k];
     *out = best;
     c->tuned[std::make_tuple(g.nz, g.nx, g.npml)] = best;
     return MNAD_OK;

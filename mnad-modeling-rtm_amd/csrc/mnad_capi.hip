@@ -426,7 +426,8 @@ size_t shot_bytes(const Model& m, int nt, int nu)
     return 6 * m.elems * 4 + 9 * (g.nz * 2 * np + 2 * np * g.nx) * 4 + (size_t)nt * nu * 4 + 4096;
 }
 
-int pick_batch(mnad_ctx* ctx, size_t per_shot, int nshots, int batch)
+// Shots whose per-shot bytes fit the context's budget (at least 1).
+int mem_cap(mnad_ctx* ctx, size_t per_shot)
 {
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
@@ -434,7 +435,12 @@ int pick_batch(mnad_ctx* ctx, size_t per_shot, int nshots, int batch)
         freeb = 0;
     }
     size_t budget = ctx->budget ? ctx->budget : (size_t)(0.85 * (double)(freeb + ctx->pool.cached));
-    int b = (int)std::max<size_t>(1, budget / std::max<size_t>(per_shot, 1));
+    return (int)std::min<size_t>(1 << 20, std::max<size_t>(1, budget / std::max<size_t>(per_shot, 1)));
+}
+
+int pick_batch(mnad_ctx* ctx, size_t per_shot, int nshots, int batch)
+{
+    int b = mem_cap(ctx, per_shot);
     if (batch > 0) b = std::min(b, batch);
     b = std::max(1, std::min(b, nshots));
     const int groups = (nshots + b - 1) / b;     // even groups: no small last batch
@@ -478,9 +484,15 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
     int maxnr = 0;
     for (int k = 0; k < sh->nshots; ++k) maxnr = std::max(maxnr, sh->rec_off[k + 1] - sh->rec_off[k]);
     const size_t esz = snap_fmt == MNAD_SNAP_F32 ? 4 : 2;
-    const size_t per = 3 * shot_bytes(mt, nt, maxnr) + (size_t)K * ni * esz + 4 * ni * 4;
+    const bool serial = (flags & MNAD_FLAG_SERIAL) != 0;
+    // Two schedules for groups of B shots.  In order: one launch sequence
+    // runs the group's 2B forwards, the next its B adjoints.  Pipelined: one
+    // sequence runs the group's forwards together with the previous group's
+    // adjoints (3B propagations), so two groups' snapshots are alive at once.
+    const size_t per_ord = 3 * shot_bytes(mt, nt, maxnr) + (size_t)K * ni * esz + 4 * ni * 4;
+    const size_t per_pip = per_ord + (size_t)K * ni * esz;
     const TunePick* tp = nullptr;
-    if (!(flags & MNAD_FLAG_SERIAL)) {
+    if (!serial) {
         tp = find_tuned(ctx, *g);
         if (!tp && batch == 0 && ((flags & MNAD_FLAG_AUTOTUNE) || ctx->autotune_on_use)) {
             TunePick pick;
@@ -489,15 +501,42 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
         }
     }
     TuningGuard tg(ctx, tp);
-    if (batch == 0 && tp) batch = std::max(1, tp->batch / 2);
-    if (flags & MNAD_FLAG_SERIAL) batch = 1;
-    const int B = pick_batch(ctx, per, sh->nshots, batch);
-    for (int k0 = 0; k0 < sh->nshots; k0 += B) {
-        const int k1 = std::min(sh->nshots, k0 + B), nb = k1 - k0;
-        std::vector<Shot> fw(2 * nb);
-        std::vector<Shot*> run;
-        for (int b = 0; b < nb; ++b) {
-            const int k = k0 + b, r0 = sh->rec_off[k], nr = sh->rec_off[k + 1] - r0;
+    bool pipelined = !serial;
+    if (!serial && batch == 0 && tp) {
+        // the tuned kernel's time per shot-step against shots per launch:
+        // per shot, in order costs 2 t(2B) + t(B), pipelined 3 t(3B)
+        double best = 1e300;
+        const int cap_ord = mem_cap(ctx, per_ord), cap_pip = mem_cap(ctx, per_pip);
+        for (int b = 1; b <= sh->nshots; ++b) {
+            const double t2 = tp->at(2 * b), t1 = tp->at(b), t3 = tp->at(3 * b);
+            if (b <= cap_ord && t2 > 0 && t1 > 0 && 2 * t2 + t1 < best) {
+                best = 2 * t2 + t1;
+                batch = b;
+                pipelined = false;
+            }
+            if (b <= cap_pip && t3 > 0 && 3 * t3 <= best) {
+                best = 3 * t3;
+                batch = b;
+                pipelined = true;
+            }
+        }
+        if (batch == 0) batch = 1;
+    }
+    if (serial) batch = 1;
+    const int B = pick_batch(ctx, pipelined ? per_pip : per_ord, sh->nshots, batch);
+    if (getenv("MNAD_AUTOTUNE_LOG"))
+        fprintf(stderr, "mnad_rtm_shots %dx%d: %d shots, %s, %d per group\n", g->nz, g->nx,
+                sh->nshots, serial ? "serial" : (pipelined ? "pipelined" : "in order"), B);
+
+    struct Group {
+        int k0 = 0, nb = 0;
+        std::vector<Shot> fw, ad;   // fw[2b]: observed (true model), fw[2b+1]: background
+    };
+    // forward shots of shots [k0, k0 + nb)
+    auto forwards = [&](Group& gr) -> int {
+        gr.fw.resize(2 * gr.nb);
+        for (int b = 0; b < gr.nb; ++b) {
+            const int k = gr.k0 + b, r0 = sh->rec_off[k], nr = sh->rec_off[k + 1] - r0;
             ShotSpec sp;
             sp.mode = MODE_FWD;
             sp.nt = nt;
@@ -507,28 +546,24 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
             sp.nr = nr;
             sp.zr = sh->zr + r0;
             sp.xr = sh->xr + r0;
-            CHK(make_shot(ctx, mt, sp, fw[2 * b]));          // observed: true model
+            CHK(make_shot(ctx, mt, sp, gr.fw[2 * b]));          // observed: true model
             sp.T = T;
             sp.K = K;
             sp.ncol = ncol;
             sp.snap_fmt = snap_fmt;
-            CHK(make_shot(ctx, ms, sp, fw[2 * b + 1]));      // background + snapshots
-            run.push_back(&fw[2 * b]);
-            run.push_back(&fw[2 * b + 1]);
+            CHK(make_shot(ctx, ms, sp, gr.fw[2 * b + 1]));      // background + snapshots
         }
-        if (flags & MNAD_FLAG_SERIAL) {
-            for (Shot* sp : run) CHK(run_batch(ctx, std::vector<Shot*>{sp}, flags));
-        } else {
-            CHK(run_batch(ctx, run, flags));
-        }
-        std::vector<Shot> ad(nb);
-        run.clear();
-        for (int b = 0; b < nb; ++b) {
-            const int k = k0 + b, r0 = sh->rec_off[k], nr = sh->rec_off[k + 1] - r0;
-            if (dobs) CHK(download_gather(ctx, fw[2 * b], nt, nr, dobs + (size_t)r0 * nt));
-            if (dsyn) CHK(download_gather(ctx, fw[2 * b + 1], nt, nr, dsyn + (size_t)r0 * nt));
-            fw[2 * b].release_state();
-            fw[2 * b + 1].release_state();
+        return MNAD_OK;
+    };
+    // after the forwards ran: gathers out, d_obs - d_syn and the adjoint shots
+    auto adjoints = [&](Group& gr) -> int {
+        gr.ad.resize(gr.nb);
+        for (int b = 0; b < gr.nb; ++b) {
+            const int k = gr.k0 + b, r0 = sh->rec_off[k], nr = sh->rec_off[k + 1] - r0;
+            if (dobs) CHK(download_gather(ctx, gr.fw[2 * b], nt, nr, dobs + (size_t)r0 * nt));
+            if (dsyn) CHK(download_gather(ctx, gr.fw[2 * b + 1], nt, nr, dsyn + (size_t)r0 * nt));
+            gr.fw[2 * b].release_state();
+            gr.fw[2 * b + 1].release_state();
             ShotSpec sp;
             sp.mode = MODE_ADJ;
             sp.nt = nt;
@@ -538,27 +573,72 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
             sp.T = T;
             sp.K = K;
             sp.ncol = ncol;
-            CHK(make_shot(ctx, ms, sp, ad[b]));
-            if (ad[b].nu > 0)
-                HIPCHK(mnad_launch_make_radj(fw[2 * b].gather.as<float>(),
-                                             fw[2 * b + 1].gather.as<float>(),
-                                             ad[b].arec.as<float>(), nt, ad[b].nu,
-                                             ad[b].radj.as<float>(), ctx->stream));
-            attach_snaps(ad[b], std::move(fw[2 * b + 1].snaps), snap_fmt, fw[2 * b + 1].d.sscale);
-            run.push_back(&ad[b]);
+            CHK(make_shot(ctx, ms, sp, gr.ad[b]));
+            if (gr.ad[b].nu > 0)
+                HIPCHK(mnad_launch_make_radj(gr.fw[2 * b].gather.as<float>(),
+                                             gr.fw[2 * b + 1].gather.as<float>(),
+                                             gr.ad[b].arec.as<float>(), nt, gr.ad[b].nu,
+                                             gr.ad[b].radj.as<float>(), ctx->stream));
+            attach_snaps(gr.ad[b], std::move(gr.fw[2 * b + 1].snaps), snap_fmt,
+                         gr.fw[2 * b + 1].d.sscale);
         }
-        if (flags & MNAD_FLAG_SERIAL) {
-            for (Shot* sp : run) CHK(run_batch(ctx, std::vector<Shot*>{sp}, flags));
-        } else {
-            CHK(run_batch(ctx, run, flags));
-        }
-        for (int b = 0; b < nb; ++b)
-            HIPCHK(mnad_launch_stack(ad[b].img.as<float>(), ad[b].imgc.as<float>(),
-                                     fw[2 * b + 1].illum.as<float>(), (float)T, (long)ni,
+        return MNAD_OK;
+    };
+    // after the adjoints ran: add their images and illuminations to the stack
+    auto stack = [&](Group& gr) -> int {
+        for (int b = 0; b < gr.nb; ++b)
+            HIPCHK(mnad_launch_stack(gr.ad[b].img.as<float>(), gr.ad[b].imgc.as<float>(),
+                                     gr.fw[2 * b + 1].illum.as<float>(), (float)T, (long)ni,
                                      st.isum.as<float>(), st.isumc.as<float>(),
                                      st.lsum.as<float>(), st.lsumc.as<float>(), ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        st.nshots += nb;
+        st.nshots += gr.nb;
+        return MNAD_OK;
+    };
+
+    if (!pipelined) {   // in order; serial: one propagation per launch sequence
+        for (int k0 = 0; k0 < sh->nshots; k0 += B) {
+            Group gr;
+            gr.k0 = k0;
+            gr.nb = std::min(sh->nshots, k0 + B) - k0;
+            CHK(forwards(gr));
+            std::vector<Shot*> run;
+            for (Shot& s1 : gr.fw) run.push_back(&s1);
+            if (serial) {
+                for (Shot* s1 : run) CHK(run_batch(ctx, std::vector<Shot*>{s1}, flags));
+            } else {
+                CHK(run_batch(ctx, run, flags));
+            }
+            CHK(adjoints(gr));
+            run.clear();
+            for (Shot& s1 : gr.ad) run.push_back(&s1);
+            if (serial) {
+                for (Shot* s1 : run) CHK(run_batch(ctx, std::vector<Shot*>{s1}, flags));
+            } else {
+                CHK(run_batch(ctx, run, flags));
+            }
+            CHK(stack(gr));
+        }
+        return MNAD_OK;
+    }
+    std::unique_ptr<Group> prev;   // forwards done, adjoints still to run
+    for (int k0 = 0; k0 < sh->nshots || prev; k0 += B) {
+        std::unique_ptr<Group> cur;
+        std::vector<Shot*> run;
+        if (k0 < sh->nshots) {
+            cur.reset(new Group);
+            cur->k0 = k0;
+            cur->nb = std::min(sh->nshots, k0 + B) - k0;
+            CHK(forwards(*cur));
+            for (Shot& s1 : cur->fw) run.push_back(&s1);
+        }
+        if (prev) {
+            CHK(adjoints(*prev));
+            for (Shot& s1 : prev->ad) run.push_back(&s1);
+        }
+        CHK(run_batch(ctx, run, flags));
+        if (prev) CHK(stack(*prev));
+        prev = std::move(cur);
     }
     return MNAD_OK;
 }

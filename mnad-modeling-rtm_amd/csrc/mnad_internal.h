@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstddef>
 #include <cstdint>
@@ -69,6 +70,21 @@ struct TunePick {
     int kernel, rz, brows;   // step-kernel variant
     int batch;               // shots per launch
     double us;               // step-kernel microseconds per shot-step
+    // the chosen variant's microseconds per shot-step with 1, 2, 4, ... shots
+    // per launch, as far as the search went (0: not measured)
+    double curve[8];
+    // estimated microseconds per shot-step for n shots per launch (log-linear
+    // between measured sizes); a negative value if n lies past the measured ones
+    double at(int n) const
+    {
+        if (n < 1) return -1.0;
+        const double x = std::log2((double)n);
+        const int k = (int)x;
+        if (k >= 8 || curve[k] <= 0) return -1.0;
+        if (x == k) return curve[k];
+        if (k + 1 >= 8 || curve[k + 1] <= 0) return -1.0;
+        return curve[k] + (curve[k + 1] - curve[k]) * (x - k);
+    }
 };
 
 // One cached launch sequence of run_batch: a graph chunk of step launches

@@ -302,6 +302,38 @@ def test_autotune_and_serial_paths(mnad, ctx):
     assert rel_l2(imgs[0], imgs[1]) < 1e-4
 
 
+def test_pipelined_rtm_groups_match_one_group(mnad, ctx):
+    """Shots in groups of one or two run pipelined (the adjoints of group g
+    share their launches with the forwards of group g + 1); with batch = 0 the
+    tuned curve picks the in-order or the pipelined schedule.  The stacked
+    image, the illumination and the gathers equal those of one group of all
+    shots and of the plain serial path, bitwise."""
+    _say("pipelined rtm groups")
+    f = golden("rtm_two_shot")
+    g = _grid(mnad, f, "vs")
+    ctx.autotune(g, f["vs"], max_shots=4)
+    n = int(f["nshots"])
+    zs = [int(f[f"zs{k}"]) for k in range(n)] * 2
+    xs = [int(f[f"xs{k}"]) for k in range(n)] * 2
+    recs = [(f[f"zr{k}"], f[f"xr{k}"]) for k in range(n)] * 2
+    out = []
+    for batch, flags in ((len(zs), 0), (1, 0), (2, 0), (0, 0),
+                         (1, mnad.FLAG_SERIAL | mnad.FLAG_NO_GRAPH)):
+        sid = ctx.stack_create(g)
+        gat = ctx.rtm_shots(sid, g, f["v"], f["vs"], f["w"], zs, xs, recs,
+                            snap_fmt=mnad.SNAP_F16, batch=batch, flags=flags, want_gathers=True)
+        img, ill, ns = ctx.stack_image(sid, g)
+        ctx.stack_destroy(sid)
+        assert ns == len(zs)
+        out.append((img, ill, gat))
+    for img, ill, gat in out[1:]:
+        assert np.array_equal(img, out[0][0])
+        assert np.array_equal(ill, out[0][1])
+        for a, b in zip(gat[0] + gat[1], out[0][2][0] + out[0][2][1]):
+            assert np.array_equal(a, b)
+    assert rel_l2(out[0][0], f["img"]) < IMAGE_TOL
+
+
 def test_model_cache_follows_the_values(mnad, ctx):
     """The context keeps the device arrays of recent velocity models, keyed by
     the grid and a digest of the values.  The same values reuse them, changed
