@@ -538,7 +538,10 @@ def main():
                                ("C2: 1001x501 (+30 CPML) dx=10 m, nt=4000, dt=0.8 ms, Ricker "
                                 "20 Hz, 1001 receivers; modelling + 1-shot RTM per step"),
                    "grid_padded": [int(v.shape[0]), int(v.shape[1])], "nt": nt,
-                   "snapshots": args.snap, "updates_per_step": 3 * N * nt},
+                   "snapshots": args.snap, "updates_per_step": 3 * N * nt,
+                   "snapshots_note": ("f16 with a per-shot power-of-two scale; bf16 (--snap "
+                                      "bf16) moves the same bytes, but its normalized C2 "
+                                      "image misses the 1e-3 bound (2.2e-3)")},
         "shots_per_hour": round(3600.0 / (ms_per_step / 1e3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
