@@ -334,6 +334,31 @@ def test_pipelined_rtm_groups_match_one_group(mnad, ctx):
     assert rel_l2(out[0][0], f["img"]) < IMAGE_TOL
 
 
+def test_pipelined_rtm_full_size_c2(mnad, ctx):
+    """At the full C2 size (561 x 1061, nt 300, 4 shots), pipelined groups of
+    one shot and one group of four give the same stacked image bitwise."""
+    _say("pipelined rtm, full-size C2")
+    wl = _c2_inputs(300)
+    v = wl["v"]
+    g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    xs = [wl["xs"] - 300, wl["xs"] - 100, wl["xs"] + 100, wl["xs"] + 300]
+    zs = [wl["zs"]] * 4
+    recs = [(wl["zr"], wl["xr"])] * 4
+    imgs = []
+    for batch in (4, 1):
+        sid = ctx.stack_create(g)
+        ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], zs, xs, recs, snap_fmt=mnad.SNAP_F16,
+                      batch=batch)
+        img, ill, ns = ctx.stack_image(sid, g)
+        ctx.stack_destroy(sid)
+        assert ns == 4
+        imgs.append((img, ill))
+    lit = imgs[0][1] > 0
+    assert lit.mean() > 0.001
+    assert np.array_equal(imgs[0][1], imgs[1][1])
+    assert np.array_equal(imgs[0][0][lit], imgs[1][0][lit])
+
+
 def test_model_cache_follows_the_values(mnad, ctx):
     """The context keeps the device arrays of recent velocity models, keyed by
     the grid and a digest of the values.  The same values reuse them, changed
