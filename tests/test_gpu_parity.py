@@ -359,6 +359,47 @@ def test_pipelined_rtm_full_size_c2(mnad, ctx):
     assert np.array_equal(imgs[0][0][lit], imgs[1][0][lit])
 
 
+@pytest.mark.parametrize("fmt", ["SNAP_F32", "SNAP_F16", "SNAP_BF16"])
+def test_snapshot_file_handoff_is_exact(mnad, ctx, tmp_path, fmt):
+    """The cross-process hand-off of the source wavefield (mnad_snap_save in
+    one context, mnad_snap_load in another, replacing ./Wavefields/UU_<ss>.mat)
+    gives the adjoint bitwise the image it computes from the snapshots kept on
+    the device, for every storage format; truncated or foreign files and a
+    missing file are rejected."""
+    _say(f"snapshot file hand-off {fmt}")
+    f = golden("rtm_rect")
+    g = _grid(mnad, f, "vs")
+    sf = getattr(mnad, fmt)
+    w, zs, xs, zr, xr = f["w"], int(f["zs0"]), int(f["xs0"]), f["zr0"], f["xr0"]
+    obs = ctx.fwd(g, f["v"], w, zs, xs, zr, xr, want_state=False)["gather"]
+    syn = ctx.fwd(g, f["vs"], w, zs, xs, zr, xr, ss=5, indx=1, snap_fmt=sf, want_state=False)
+    im_dev = ctx.born_adj(g, f["vs"], obs - syn["gather"], zr, xr, ss=5)
+    ctx.fwd(g, f["vs"], w, zs, xs, zr, xr, ss=5, indx=1, snap_fmt=sf, want_state=False)
+    path = str(tmp_path / "UU_5.mnad")
+    ctx.snap_save(5, path)
+    ctx.release(5)
+    other = mnad.Context(0)
+    try:
+        other.snap_load(5, path)
+        im_file = other.born_adj(g, f["vs"], obs - syn["gather"], zr, xr, ss=5)
+        data = open(path, "rb").read()
+        bad = str(tmp_path / "bad.mnad")
+        with open(bad, "wb") as fh:
+            fh.write(data[: len(data) // 2])
+        with pytest.raises(mnad.MnadError):
+            other.snap_load(6, bad)
+        with open(bad, "wb") as fh:
+            fh.write(b"NOTSNAPS" + data[8:])
+        with pytest.raises(mnad.MnadError):
+            other.snap_load(6, bad)
+        with pytest.raises(mnad.MnadError):
+            other.snap_load(6, str(tmp_path / "missing.mnad"))
+    finally:
+        other.close()
+    assert np.array_equal(im_dev, im_file)
+    assert np.abs(im_dev).max() > 0
+
+
 def test_model_cache_follows_the_values(mnad, ctx):
     """The context keeps the device arrays of recent velocity models, keyed by
     the grid and a digest of the values.  The same values reuse them, changed
