@@ -1,0 +1,187 @@
+// Host-only check of the launch tile table (build_tiles in csrc/mnad_runtime.hip).
+// Built with hipcc against the runtime source alone; the kernel launchers are
+// stubbed (the tile builder calls none of them) and no GPU is touched.
+//
+// For each case it prints one line "ok <name>" or "FAIL <name>: <reason>":
+//  - every grid point of every shot is covered by exactly one listed tile
+//    (one-point tiles of kernels 4/7, and for kernel 6 one-point tiles off
+//    the four-column interior plus the four-column tiles);
+//  - the table length is a multiple of 8 (XCD x takes entries b % 8 == x);
+//  - each XCD's tiles are a contiguous range of the shot-major tile order;
+//  - the estimated work per XCD differs by at most two tile weights, half a
+//    tile at either end of each XCD's range (two-step kernels: counts);
+//  - with strip_first, each XCD dispatches its CPML-strip tiles first.
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "mnad_internal.h"
+
+extern "C" {
+hipError_t mnad_launch_setup_model(const double*, int, int, int, int, double, double, double,
+                                   float*, float*, float*, hipStream_t) { return hipErrorNotSupported; }
+hipError_t mnad_launch_step(const mnad::ShotDev*, const mnad::TileRef*, int, const int*, int, int,
+                            int, int, hipStream_t) { return hipErrorNotSupported; }
+hipError_t mnad_launch_advance(int*, int, hipStream_t) { return hipErrorNotSupported; }
+hipError_t mnad_launch_init_source(mnad::ShotDev*, int, hipStream_t) { return hipErrorNotSupported; }
+hipError_t mnad_launch_make_radj(const float*, const float*, const float*, int, int, float*,
+                                 hipStream_t) { return hipErrorNotSupported; }
+hipError_t mnad_launch_stack(const float*, const float*, const float*, float, long, float*, float*,
+                             float*, float*, hipStream_t) { return hipErrorNotSupported; }
+hipError_t mnad_launch_fill(float*, long, float, hipStream_t) { return hipErrorNotSupported; }
+}
+
+namespace mnad {
+int autotune(mnad_ctx*, const Model&, int, TunePick*) { return MNAD_ERR_STATE; }
+void server_destroy(mnad_ctx*) {}
+}
+
+using namespace mnad;
+
+struct Shape { int nz, nx, np; };
+
+static bool is_strip_tile(int r0, int r1, int c0, int c1, const Shape& s)
+{
+    return c0 < s.np || c1 > s.nx - s.np || r0 < s.np || r1 > s.nz - s.np;
+}
+
+static std::string check(const char* name, int kernel, int rz, int brows, int strip_first,
+                         const std::vector<Shape>& shapes)
+{
+    mnad_ctx c;
+    c.kernel = kernel;
+    c.rz = rz;
+    c.brows = brows;
+    c.strip_first = strip_first;
+    std::vector<ShotDev> h(shapes.size());
+    for (size_t k = 0; k < shapes.size(); ++k) {
+        h[k] = ShotDev{};
+        h[k].nz = shapes[k].nz;
+        h[k].nx = shapes[k].nx;
+        h[k].npml = shapes[k].np;
+    }
+    std::vector<TileRef> table;
+    build_tiles(&c, h, table);
+    int tw, th, spl;
+    tile_shape(&c, &tw, &th, &spl);
+    const bool x4 = kernel == 6, two = two_step_kernel(kernel);
+    char buf[256];
+    if (table.size() % 8 != 0) return "table length not a multiple of 8";
+
+    // coverage: every point exactly once
+    std::vector<std::vector<int>> cover(shapes.size());
+    for (size_t k = 0; k < shapes.size(); ++k) cover[k].assign((size_t)shapes[k].nz * shapes[k].nx, 0);
+    std::vector<std::pair<int, int>> listed;
+    for (const TileRef& t : table) {
+        if (t.shot < 0) continue;
+        if (t.shot >= (int)shapes.size()) return "shot index out of range";
+        listed.emplace_back(t.shot, t.tile);
+        const Shape& s = shapes[t.shot];
+        const ShotDev& d = h[t.shot];
+        if (x4 && t.tile >= d.nt1) {
+            const int u = t.tile - d.nt1, tb = u / d.nx4, tx = u - tb * d.nx4;
+            const int lo = d.x4lo + tx * 248;
+            for (int j = s.np + tb * th; j < std::min(s.nz - s.np, s.np + (tb + 1) * th); ++j)
+                for (int i = std::max(lo, s.np); i < std::min(lo + 248, d.x4hi); ++i)
+                    cover[t.shot][(size_t)j * s.nx + i]++;
+        } else {
+            const int tj = t.tile / d.ntx, ti = t.tile - tj * d.ntx;
+            for (int j = tj * th; j < std::min(s.nz, (tj + 1) * th); ++j)
+                for (int i = ti * tw; i < std::min(s.nx, (ti + 1) * tw); ++i) {
+                    if (x4 && j >= s.np && j < s.nz - s.np && i >= s.np && i < d.x4hi) continue;
+                    cover[t.shot][(size_t)j * s.nx + i]++;
+                }
+        }
+    }
+    for (size_t k = 0; k < shapes.size(); ++k)
+        for (size_t q = 0; q < cover[k].size(); ++q)
+            if (cover[k][q] != 1) {
+                snprintf(buf, sizeof buf, "shot %zu point %zu covered %d times", k, q, cover[k][q]);
+                return buf;
+            }
+    std::vector<std::pair<int, int>> sorted = listed;
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return "tile listed twice";
+
+    // per XCD: contiguous range, balanced work, strip tiles first
+    auto weight = [&](const TileRef& t, bool* strip) {
+        const Shape& s = shapes[t.shot];
+        const ShotDev& d = h[t.shot];
+        if (x4 && t.tile >= d.nt1) {
+            *strip = false;
+            return c.x4_weight;
+        }
+        const int tj = t.tile / d.ntx, ti = t.tile - tj * d.ntx;
+        *strip = is_strip_tile(tj * th, std::min(s.nz, (tj + 1) * th), ti * tw,
+                               std::min(s.nx, (ti + 1) * tw), s);
+        return two ? 1.0 : (*strip ? c.strip_weight : 1.0);
+    };
+    double wmax = 0.0, wlo = 1e300, whi = 0.0;
+    for (int x = 0; x < 8; ++x) {
+        std::vector<std::pair<int, int>> mine;
+        double w = 0.0;
+        bool seen_interior = false;
+        for (size_t b = x; b < table.size(); b += 8) {
+            const TileRef& t = table[b];
+            if (t.shot < 0) continue;
+            bool strip = false;
+            const double tw1 = weight(t, &strip);
+            w += tw1;
+            wmax = std::max(wmax, tw1);
+            if (strip_first) {
+                if (!strip) seen_interior = true;
+                else if (seen_interior) return "a strip tile after an interior tile on one XCD";
+            }
+            mine.emplace_back(t.shot, t.tile);
+        }
+        wlo = std::min(wlo, w);
+        whi = std::max(whi, w);
+        if (mine.empty()) continue;
+        std::sort(mine.begin(), mine.end());
+        const auto first = std::lower_bound(sorted.begin(), sorted.end(), mine.front());
+        if (first + (long)mine.size() > sorted.end() ||
+            !std::equal(mine.begin(), mine.end(), first))
+            return "an XCD's tiles are not a contiguous range";
+    }
+    // a run gains or loses at most half a tile at each end of its range
+    if (whi - wlo > 2 * wmax + 1e-9) {
+        snprintf(buf, sizeof buf, "work per XCD from %.1f to %.1f (tile weight at most %.1f)", wlo,
+                 whi, wmax);
+        return buf;
+    }
+    return "";
+}
+
+int main()
+{
+    struct Case {
+        const char* name;
+        int kernel, rz, brows, strip_first;
+        std::vector<Shape> shapes;
+    };
+    const Shape c2{561, 1061, 30}, c1{241, 241, 20}, tiny{13, 13, 6}, nocpml{70, 300, 0};
+    const std::vector<Case> cases = {
+        {"k4_b1_c2", 4, 1, 1, 1, {c2}},
+        {"k4_b4_c2x2", 4, 1, 4, 1, {c2, c2}},
+        {"k4_b1_mixed", 4, 1, 1, 1, {c2, c1, tiny, nocpml}},
+        {"k4_b2_nostripfirst", 4, 1, 2, 0, {c1, c2}},
+        {"k0_r2_c1", 0, 2, 4, 1, {c1}},
+        {"k6_b2_c2", 6, 1, 2, 1, {c2}},
+        {"k6_b4_mixed", 6, 1, 4, 1, {c1, c2, tiny, nocpml}},
+        {"k7_r5_b8_c2x3", 7, 5, 8, 1, {c2, c2, c2}},
+        {"k7_r3_b16_mixed", 7, 3, 16, 1, {c1, tiny, c2}},
+        {"k3_c1", 3, 1, 16, 1, {c1}},
+    };
+    int bad = 0;
+    for (const Case& cs : cases) {
+        const std::string r = check(cs.name, cs.kernel, cs.rz, cs.brows, cs.strip_first, cs.shapes);
+        if (r.empty()) {
+            printf("ok %s\n", cs.name);
+        } else {
+            printf("FAIL %s: %s\n", cs.name, r.c_str());
+            ++bad;
+        }
+    }
+    return bad ? 1 : 0;
+}
