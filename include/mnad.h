@@ -182,6 +182,13 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  *   with rows_per_thread 3, 5 or 7 and block_rows (wavefronts) 4, 8 or 16
  *   dividing 4 (rows_per_thread + 1); shots keep four time levels -- set it
  *   before creating shots;
+ * kernel 9 = resident blocks: one cooperative launch runs every step of a
+ *   batch; a workgroup of 8 wavefronts keeps a 64-column block of one shot in
+ *   LDS for the whole launch and exchanges only its two-cell edges with its
+ *   neighbours (agent-scope atomic records, no per-step launch or cache
+ *   flush).  block_rows 8; rows_per_thread = blocks per CU wanted (1-4; fewer
+ *   are used when the batch does not fit, MNAD_ERR_ARG when no block height up
+ *   to 64 rows fits);
  * graph_chunk = step launches per captured hipGraph (<= 1: no graph);
  * strip_first != 0 dispatches the CPML-strip tiles of each XCD first. */
 typedef struct {

@@ -76,4 +76,11 @@ struct TileRef {
     int tile;   // tile index within the shot: tj * ntx + ti
 };
 
+// One block of the resident step kernel (kernel 9): rows r0 .. r0 + bh - 1 and
+// columns c0 .. c0 + 63 of one shot, owned by one workgroup for a whole launch.
+struct ResBlock {
+    int shot, r0, c0, pad;
+    int nbr[8];   // blocks holding the eight surrounding cells, -1 past the grid
+};
+
 }  // namespace mnad

@@ -168,6 +168,7 @@ struct mnad_ctx {
     std::map<std::tuple<int, int, int>, mnad::TunePick> tuned;   // (nz, nx, npml)
     // run_batch: persistent batch descriptors (grow-only) and cached graph chunks
     mnad::DevBuf bdesc, btab, bbase;
+    mnad::DevBuf bedge;   // resident step kernel: edge records of its blocks
     std::vector<mnad::GraphEntry> graphs;
     // host -> device staging through pinned memory (upload / copy_in)
     void* stage = nullptr;

@@ -791,6 +791,410 @@ __global__ __launch_bounds__(BX * BZR) void k_stepw(const ShotDev* __restrict__ 
     step_one<PF, false>(d, s, j, ti * BX, threadIdx.x);
 }
 
+// ------------------------------------------------ resident blocks (kernel 9)
+// One cooperative launch runs every step of a batch.  Workgroup b owns
+// blocks[b] for the whole launch: columns c0 .. c0+63 of one shot and rows
+// r0 .. r0+bh-1.
+//  - Level s of U, Vs, Ws sits in LDS with a ring of the two cells around the
+//    block, in two buffers: step s reads one and writes level s+1 into the
+//    other, over level s-1, which is the update's previous level at the same
+//    point.  rx sits in a third LDS plane.  The per-point update is
+//    step_one's, read from LDS.
+//  - Each step computes the edge cells first (rows 0, 1, bh-2, bh-1 and
+//    columns 0, 1, 62, 63: waves 0-3 take the four edge rows, waves 4-7 one
+//    edge column each, a lane per row) and sends them out at once, then the
+//    interior (rows 2 .. bh-3, columns 2 .. 61, a wave per row), behind which
+//    the neighbours' edges arrive.
+//  - Edge cells go out as 64-bit records {value, level} in the block's edge
+//    area of `ebuf`, one agent-scope atomic store each; level L uses slot
+//    L & 1.  The ring of level s comes back as agent-scope atomic loads of the
+//    neighbours' records, each retried until it carries level s.  Both are
+//    coherent per access across the XCDs: no flags, fences or cache-wide
+//    write-back or invalidation.
+//  - Two slots suffice: a block writes level L+1 (over level L-1) only after
+//    it has read its neighbours' level L, which they publish after they have
+//    read its level L-1.
+//  - Every wait is bounded: a block that gives up sets *err and leaves, and
+//    blocks that see *err leave too, so the launch always drains.  The launch
+//    is cooperative, so all blocks are resident at once.
+//  - After its last step a block writes its last two levels in full to the
+//    level buffers in global memory.
+// CPML memories, gathers, snapshots and the image stay in global memory; each
+// point's are touched only by the block that owns it.
+constexpr int RES_NW = 8;            // wavefronts per block
+constexpr int RES_LW = BX + 4;       // LDS row: the block's 64 columns and 2 each side
+constexpr int RES_SPIN = 1 << 20;    // polls of one record before a wait gives up
+
+// Edge area of one block: [slot][field][ne] records, ne = 256 + 4 bh:
+//   top rows 0, 1 (r * 64 + c), bottom rows bh-2, bh-1 (128 + ...),
+//   left columns 0, 1 (256 + 2 r + c), right columns 62, 63 (256 + 2 bh + ...).
+__host__ __device__ __forceinline__ int res_edge_cells(int bh) { return 256 + 4 * bh; }
+
+typedef unsigned long long u64;
+
+__device__ __forceinline__ u64 edge_rec(float x, int level)
+{
+    return (u64)__float_as_uint(x) | ((u64)(unsigned)level << 32);
+}
+
+// What every point of a block's step shares.
+struct ResStep {
+    const float* LU;   // level s, buffer with ring
+    float* NU;         // level s+1 written over level s-1
+    const float* LRX;  // rx [bh x 64]
+    u64* out;          // this block's records of level s+1
+    int plane, ne, bh, r0, c0, s;
+    float a0, a1;      // source samples s and s+1
+    bool src;
+};
+
+// One point (block row r, column c; grid row j, column i) of step s.  ROWW:
+// the wave holds one block row (j wave-uniform, row origins in SGPRs);
+// otherwise a lane per row (the edge-column waves).  rr: receiver slot or -1.
+template <bool ROWW>
+__device__ __forceinline__ void res_point(const ShotDev& d, const ResStep& t, int r, int c,
+                                          int rr)
+{
+    const int j = t.r0 + r, i = t.c0 + c;
+    const int np = d.npml, nz = d.nz, nx = d.nx;
+    const long P = d.pitch;
+    const int o = (r + 2) * RES_LW + (c + 2);
+    const float* LU = t.LU;
+    const float* LV = LU + t.plane;
+    const float* LW = LU + 2 * t.plane;
+    Nbh u, v, w;
+    u.nw = LU[o - RES_LW - 1]; u.n = LU[o - RES_LW]; u.ne = LU[o - RES_LW + 1];
+    u.w = LU[o - 1];           u.c = LU[o];          u.e = LU[o + 1];
+    u.sw = LU[o + RES_LW - 1]; u.s = LU[o + RES_LW]; u.se = LU[o + RES_LW + 1];
+    v.nw = LV[o - RES_LW - 1]; v.n = LV[o - RES_LW]; v.ne = LV[o - RES_LW + 1];
+    v.w = LV[o - 1];           v.c = LV[o];          v.e = LV[o + 1];
+    v.sw = LV[o + RES_LW - 1]; v.s = LV[o + RES_LW]; v.se = LV[o + RES_LW + 1];
+    w.nw = LW[o - RES_LW - 1]; w.n = LW[o - RES_LW]; w.ne = LW[o - RES_LW + 1];
+    w.w = LW[o - 1];           w.c = LW[o];          w.e = LW[o + 1];
+    w.sw = LW[o + RES_LW - 1]; w.s = LW[o + RES_LW]; w.se = LW[o + RES_LW + 1];
+    const float vww = LV[o - 2], vee = LV[o + 2];
+    const float wnn = LW[o - 2 * RES_LW], wss = LW[o + 2 * RES_LW];
+    const float rx = t.LRX[r * BX + c], rz = rx * d.kap;
+    float du, dv, dw;
+    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+
+    const bool inx = (i < np) || (i >= nx - np);
+    const bool inz = (j < np) || (j >= nz - np);
+    if (inx || inz) {
+        // The CPML memories see the pre-injection U (fwd_PDE_exact_MNAD.m:95-111).
+        float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
+        if (t.src) {
+            const float a0 = t.a0;
+            if (j == d.src_j) {
+                if (i == d.src_i) uc -= a0;
+                if (i - 1 == d.src_i) uw -= a0;
+                if (i + 1 == d.src_i) ue -= a0;
+            }
+            if (i == d.src_i) {
+                if (j - 1 == d.src_j) un -= a0;
+                if (j + 1 == d.src_j) us -= a0;
+            }
+        }
+        if (inx) {
+            const long nsx = (long)nz * 2 * np;
+            const unsigned xo = (i < np) ? (unsigned)i : (unsigned)(i - (nx - 2 * np));
+            PsiW mx;
+            if (ROWW) {
+                const long rowx = (long)j * 2 * np;
+                psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, 4u * xo);
+                psi_step(mx, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+                psi_store(mx, gp(d.psx) + rowx, nsx, 4u * xo);
+            } else {
+                const unsigned b = 4u * ((unsigned)j * 2u * (unsigned)np + xo);
+                psi_load(mx, gp(d.psx), gp(d.cfx), nsx, b);
+                psi_step(mx, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+                psi_store(mx, gp(d.psx), nsx, b);
+            }
+        }
+        if (inz) {
+            const long nsz = (long)2 * np * P;
+            const int zr = (j < np) ? j : j - (nz - 2 * np);
+            PsiW mz;
+            if (ROWW) {
+                const long rowz = (long)zr * P + t.c0;
+                psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * (unsigned)c);
+                psi_step(mz, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+                psi_store(mz, gp(d.psz) + rowz, nsz, 4u * (unsigned)c);
+            } else {
+                const unsigned b = 4u * ((unsigned)zr * (unsigned)P + (unsigned)c);
+                psi_load(mz, gp(d.psz) + t.c0, gp(d.cfz) + t.c0, nsz, b);
+                psi_step(mz, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+                psi_store(mz, gp(d.psz) + t.c0, nsz, b);
+            }
+        }
+    }
+
+    float* NU = t.NU;
+    // level s-1 at this point: the cell level s+1 replaces
+    float nu = (2.0f * u.c + du) - NU[o];
+    const float nv = (2.0f * v.c + dv) - NU[t.plane + o];
+    const float nw = (2.0f * w.c + dw) - NU[2 * t.plane + o];
+    const int s = t.s;
+    if (d.mode == MODE_FWD) {
+        if (j == d.src_j && i == d.src_i) nu += t.a1;
+        if (rr >= 0) (gp(d.gather) + (long)s * d.nrec)[(unsigned)rr] = u.c;
+    } else if (rr >= 0) {
+        nu -= (gp(d.radj) + (long)s * d.nrec)[(unsigned)rr];
+    }
+    NU[o] = nu;
+    NU[t.plane + o] = nv;
+    NU[2 * t.plane + o] = nw;
+    // edge records of level s+1 (a corner cell goes to two of them)
+    const int bh = t.bh;
+    int e0 = -1, e1 = -1;
+    if (r < 2) e0 = r * BX + c;
+    else if (r >= bh - 2) e0 = 128 + (r - (bh - 2)) * BX + c;
+    if (c < 2) e1 = 256 + 2 * r + c;
+    else if (c >= BX - 2) e1 = 256 + 2 * bh + 2 * r + (c - (BX - 2));
+    if (e0 >= 0 || e1 >= 0) {
+        const float nf[3] = {nu, nv, nw};
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const u64 x = edge_rec(nf[f], s + 1);
+            if (e0 >= 0)
+                __hip_atomic_store(t.out + f * t.ne + e0, x, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (e1 >= 0)
+                __hip_atomic_store(t.out + f * t.ne + e1, x, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (d.T > 0 && !inx && !inz)
+        snap_image_w(d, s, (long)(j - np) * (nx - 2 * np), (unsigned)(i - np), du, nu);
+}
+
+// Receiver slot of grid point (j, i), -1 if none.  The load is consumed
+// before returning, so no load stays in flight past the call.
+__device__ __forceinline__ int res_rec(const ShotDev& d, int j, int i)
+{
+    if (d.nrec <= 0) return -1;
+    const int off = d.rrow[j];
+    int x = off >= 0 ? d.rcol[off + i] : -1;
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+__global__ __launch_bounds__(BX * RES_NW) void k_res(const ShotDev* __restrict__ shots,
+                                                     const ResBlock* __restrict__ blocks,
+                                                     u64* ebuf, int* err, int nsteps, int bh)
+{
+    // [2 buffers][U, Vs, Ws][(bh + 4) x RES_LW], then rx [bh x BX]
+    extern __shared__ float lds[];
+    __shared__ int give_up;
+    const ResBlock& rb = blocks[blockIdx.x];
+    const ShotDev& d = shots[rb.shot];
+    const int lane = (int)threadIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int tid = wv * BX + lane;
+    const int nst = min(nsteps, d.nt);
+    if (nst <= 0) return;
+    const int plane = (bh + 4) * RES_LW;
+    float* LRX = lds + 6 * plane;
+    const int ne = res_edge_cells(bh);
+    const long area = 2L * 3 * ne;   // records per block
+    const int nz = d.nz, nx = d.nx;
+    const long P = d.pitch;
+    const int r0 = rb.r0, c0 = rb.c0;
+    const int ring = 4 * RES_LW + 4 * bh;   // cells of the ring
+    u64* mine = ebuf + (long)blockIdx.x * area;
+    if (tid == 0) give_up = 0;
+
+    // Level 0 into buffer 0, level -1 into buffer 1, rx.  Cells past the grid
+    // (a partial block's last rows or columns) stand for the zero ghost cells
+    // and are never written again, so they start cleared in both buffers.
+    for (int t = tid; t < (bh + 4) * RES_LW; t += BX * RES_NW) {
+        const int rr = t / RES_LW - 2, cc = t % RES_LW - 2;
+        const int j = r0 + rr, ii = c0 + cc;
+        const bool in = j >= 0 && j < nz && ii >= 0 && ii < nx;
+        const long o = (long)(j + MNAD_ZO) * P + (ii + MNAD_XO);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            lds[f * plane + t] = in ? d.f[0][f][o] : 0.0f;
+            lds[(3 + f) * plane + t] = in ? d.f[1][f][o] : 0.0f;
+        }
+        if (rr >= 0 && rr < bh && cc >= 0 && cc < BX) LRX[rr * BX + cc] = in ? d.rx[o] : 0.0f;
+    }
+
+    // This wave's points.  Edge pass: waves 0-3 block row erow (a lane per
+    // column), waves 4-7 block column ecol of rows 2 .. bh-3 (a lane per row).
+    // Interior pass: rows 2 + wv, 2 + wv + RES_NW, ... (lanes 2 .. 61).
+    const int erow = wv == 0 ? 0 : (wv == 1 ? 1 : (wv == 2 ? bh - 2 : bh - 1));
+    const int ecol = wv == 4 ? 0 : (wv == 5 ? 1 : (wv == 6 ? BX - 2 : BX - 1));
+    const bool ecol_on = wv >= 4 && lane < bh - 4 && r0 + 2 + lane < nz && c0 + ecol < nx;
+    const bool erow_on = wv < 4 && r0 + erow < nz && c0 + lane < nx;
+    // receiver slots: the edge point's (looked up once), and the first two
+    // interior rows with receivers (in registers; a global load inside the
+    // row loop would make every row wait for the stores before it); further
+    // receiver rows look theirs up each step
+    int rec_e = -1;
+    if (ecol_on) rec_e = res_rec(d, r0 + 2 + lane, c0 + ecol);
+    if (erow_on) rec_e = res_rec(d, r0 + erow, c0 + lane);
+    int ra = -1, rb2 = -1, reca = -1, recb = -1;
+    bool more = false;
+    if (d.nrec > 0) {
+        for (int r = 2 + wv; r < bh - 2 && r0 + r < nz; r += RES_NW) {
+            const int off = __builtin_amdgcn_readfirstlane(d.rrow[r0 + r]);
+            if (off < 0) continue;
+            const int x = c0 + lane < nx ? d.rcol[off + c0 + lane] : -1;
+            if (ra < 0) {
+                ra = r;
+                reca = x;
+            } else if (rb2 < 0) {
+                rb2 = r;
+                recb = x;
+            } else {
+                more = true;
+            }
+        }
+    }
+
+    // The ring cells this thread reloads every step (at most two: the ring
+    // has at most 4 * 68 + 4 * 64 cells for 512 threads), as LDS offsets and
+    // the neighbour records that hold them (-1: past the grid, stays zero).
+    int rl[2];
+    long rsrc[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int t = tid + h * BX * RES_NW;
+        rl[h] = -1;
+        rsrc[h] = -1;
+        if (t >= ring) continue;
+        int rr, cc;
+        if (t < 4 * RES_LW) {
+            const int k = t / RES_LW;
+            rr = k < 2 ? k - 2 : bh + k - 2;
+            cc = t % RES_LW - 2;
+        } else {
+            const int u = t - 4 * RES_LW, k = u & 3;
+            rr = u >> 2;
+            cc = k < 2 ? k - 2 : BX + k - 2;
+        }
+        rl[h] = (rr + 2) * RES_LW + (cc + 2);
+        const int j = r0 + rr, ii = c0 + cc;
+        if (j < 0 || j >= nz || ii < 0 || ii >= nx) continue;
+        // owner block (oz, ox) relative to this one, and the cell's record there
+        const int oz = rr < 0 ? -1 : (rr >= bh ? 1 : 0);
+        const int ox = cc < 0 ? -1 : (cc >= BX ? 1 : 0);
+        const int lr = rr - oz * bh, lc = cc - ox * BX;   // row / column in the owner
+        int idx;
+        if (oz < 0) idx = 128 + (lr - (bh - 2)) * BX + lc;   // owner's bottom rows
+        else if (oz > 0) idx = lr * BX + lc;                 // owner's top rows
+        else if (ox < 0) idx = 256 + 2 * bh + 2 * lr + (lc - (BX - 2));   // owner's right columns
+        else idx = 256 + 2 * lr + lc;                        // owner's left columns
+        const int k = (oz + 1) * 3 + (ox + 1);
+        const int nb = rb.nbr[k < 4 ? k : k - 1];
+        if (nb >= 0) rsrc[h] = (long)nb * area + idx;
+    }
+    __syncthreads();
+
+    ResStep st;
+    st.plane = plane;
+    st.ne = ne;
+    st.bh = bh;
+    st.r0 = r0;
+    st.c0 = c0;
+    st.LRX = LRX;
+    st.src = d.mode == MODE_FWD && d.src_j >= 0;
+    int cur = 0;   // buffer holding level s
+    for (int s = 0; s < nst; ++s) {
+        // this step's source samples, loaded before the barrier so that no
+        // load is left in flight inside the row loops
+        st.a0 = st.a1 = 0.0f;
+        if (st.src) {
+            st.a0 = d.src_amp[s];
+            st.a1 = d.src_amp[s + 1];
+        }
+        __syncthreads();   // level s and its ring are in buffer cur
+        if (give_up) return;
+
+        st.s = s;
+        st.LU = lds + cur * 3 * plane;
+        st.NU = lds + (cur ^ 1) * 3 * plane;
+        st.out = mine + (long)((s + 1) & 1) * 3 * ne;
+        // edge pass: level s+1 of the edge cells goes out first
+        if (erow_on) res_point<true>(d, st, erow, lane, rec_e);
+        if (ecol_on) res_point<false>(d, st, 2 + lane, ecol, rec_e);
+        // the ring of level s+1 (the neighbours' slot (s+1) & 1) is requested
+        // now and arrives behind the interior pass
+        const bool next = s + 1 < nst;
+        const long so = (long)((s + 1) & 1) * 3 * ne;
+        u64 x[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+                x[h][f] = (next && rsrc[h] >= 0)
+                              ? __hip_atomic_load(ebuf + rsrc[h] + so + f * ne, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : 0ull;
+        // interior pass
+        const bool lin = lane >= 2 && lane < BX - 2 && c0 + lane < nx;
+        for (int r = 2 + wv; r < bh - 2; r += RES_NW) {
+            if (r0 + r >= nz) break;
+            int rr = r == ra ? reca : (r == rb2 ? recb : -1);
+            if (more && r != ra && r != rb2 && lin) rr = res_rec(d, r0 + r, c0 + lane);
+            if (lin) res_point<true>(d, st, r, lane, rr);
+        }
+        // ring of level s+1 into the other buffer (nobody reads its ring
+        // during step s); records not yet at level s+1 are polled
+        if (next) {
+            float* L = lds + (cur ^ 1) * 3 * plane;
+            bool ok = true;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (rl[h] < 0) continue;
+                if (rsrc[h] < 0) {
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) L[f * plane + rl[h]] = 0.0f;
+                    continue;
+                }
+                const u64* srcp = ebuf + rsrc[h] + so;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    u64 y = x[h][f];
+                    for (int n = 0; ok && (int)(y >> 32) != s + 1; ++n) {
+                        if (n >= RES_SPIN || __hip_atomic_load(err, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                            ok = false;
+                            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        y = __hip_atomic_load(srcp + f * ne, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    L[f * plane + rl[h]] = __uint_as_float((unsigned)y);
+                }
+            }
+            if (!ok) give_up = 1;
+        }
+        cur ^= 1;
+    }
+
+    // last two levels in full: level nst in buffer cur, level nst-1 in the
+    // other (other waves computed some of this wave's rows in the last step)
+    __syncthreads();
+    const float* LA = lds + cur * 3 * plane;
+    const float* LB = lds + (cur ^ 1) * 3 * plane;
+    const int i = c0 + lane;
+    for (int r = wv; r < bh; r += RES_NW) {
+        const int j = r0 + r;
+        if (j >= nz || i >= nx) continue;
+        const int o = (r + 2) * RES_LW + (lane + 2);
+        const long g = (long)(j + MNAD_ZO) * P + (i + MNAD_XO);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            d.f[nst & 1][f][g] = LA[f * plane + o];
+            d.f[(nst + 1) & 1][f][g] = LB[f * plane + o];
+        }
+    }
+}
+
 // ------------------------------------------------ four columns per lane
 // The interior of a shot (rows npml .. nz-npml-1, columns npml .. nx-npml-1:
 // no CPML there) in waves of 64 lanes x 4 columns.  Lane l holds columns
@@ -1594,6 +1998,49 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+// Resident kernel: block heights 4 .. 64.
+static const void* res_kernel(int bh)
+{
+    if (bh < 4 || bh > 64) return nullptr;
+    return reinterpret_cast<const void*>(&k_res);
+}
+
+static size_t res_lds(int bh)
+{
+    return ((size_t)2 * 3 * (bh + 4) * RES_LW + (size_t)bh * BX) * sizeof(float);
+}
+
+int mnad_resident_max_rows() { return 64; }
+
+// Blocks of height bh that one CU holds at once.
+hipError_t mnad_resident_fit(int bh, int* per_cu)
+{
+    const void* k = res_kernel(bh);
+    if (!k) return hipErrorInvalidValue;
+    const size_t lds = res_lds(bh);
+    if (lds > 65536) {   // above the default dynamic-LDS limit of a launch
+        const hipError_t e =
+            hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, BX * RES_NW, lds);
+}
+
+// 64-bit edge records one block of height bh exchanges (both slots, three fields).
+long mnad_resident_edge_records(int bh) { return 2L * 3 * res_edge_cells(bh); }
+
+// Every step of a batch in one cooperative launch (all blocks resident);
+// ebuf (nblocks * mnad_resident_edge_records(bh) records) and *err must be zero.
+hipError_t mnad_launch_resident(const ShotDev* shots, const ResBlock* blocks, int nblocks,
+                                unsigned long long* ebuf, int* err, int nsteps, int bh,
+                                hipStream_t st)
+{
+    const void* k = res_kernel(bh);
+    if (!k || nblocks <= 0) return hipErrorInvalidValue;
+    void* args[] = {&shots, &blocks, &ebuf, &err, &nsteps, &bh};
+    return hipLaunchCooperativeKernel(k, dim3(nblocks), dim3(BX, RES_NW), args, res_lds(bh), st);
 }
 
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st)

@@ -42,6 +42,12 @@ hipError_t mnad_launch_stack(const float* img, const float* imgc, const float* i
                              long n, float* isum, float* isumc, float* lsum, float* lsumc,
                              hipStream_t st);
 hipError_t mnad_launch_fill(float* p, long n, float val, hipStream_t st);
+int mnad_resident_max_rows();
+hipError_t mnad_resident_fit(int bh, int* per_cu);
+long mnad_resident_edge_records(int bh);
+hipError_t mnad_launch_resident(const ShotDev* shots, const ResBlock* blocks, int nblocks,
+                                unsigned long long* ebuf, int* err, int nsteps, int bh,
+                                hipStream_t st);
 }
 
 static thread_local std::string g_err;
@@ -195,6 +201,7 @@ void release_batch_cache(mnad_ctx* c)
     c->bdesc.reset();
     c->btab.reset();
     c->bbase.reset();
+    c->bedge.reset();
     if (c->stage_pending) (void)hipEventSynchronize(c->stage_ev);
     c->stage_pending = false;
     if (c->stage) (void)hipHostFree(c->stage);
@@ -601,6 +608,122 @@ double algorithmic_bytes(const ShotDev& d)
     return b;
 }
 
+// Block height of the resident kernel for a batch: the smallest (at least 4)
+// at which the batch takes at most `per_cu` blocks per CU; 0 if none up to
+// the kernel's limit does.  *nblocks gets the count at that height.
+int resident_rows(const std::vector<ShotDev>& h, int ncu, int per_cu, long* nblocks)
+{
+    const long cap = (long)std::max(per_cu, 1) * ncu;
+    for (int bh = 4; bh <= mnad_resident_max_rows(); ++bh) {
+        long n = 0;
+        for (const ShotDev& d : h) n += (long)((d.nx + 63) / 64) * ((d.nz + bh - 1) / bh);
+        if (n <= cap) {
+            *nblocks = n;
+            return bh;
+        }
+    }
+    return 0;
+}
+
+// Kernel 9: every step of the batch in one cooperative launch of resident
+// blocks (64 columns x bh rows of one shot each; c->rz = blocks per CU).
+static int run_resident(mnad_ctx* c, std::vector<ShotDev>& h, int nsteps, double bytes)
+{
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    // the most blocks per CU up to c->rz at which the batch fits on the device
+    long total = 0;
+    int bh = 0, per_cu = 0;
+    for (int want = std::max(c->rz, 1); want >= 1 && bh == 0; --want) {
+        const int b = resident_rows(h, ncu, want, &total);
+        if (b == 0) continue;
+        int fit = 0;
+        HIPCHK(mnad_resident_fit(b, &fit));
+        if ((long)fit * ncu >= total) {
+            bh = b;
+            per_cu = fit;
+        }
+    }
+    // MNAD_RESIDENT_ROWS forces a (larger) block height, e.g. to test long
+    // blocks on small grids
+    if (const char* e = getenv("MNAD_RESIDENT_ROWS")) {
+        const int f = atoi(e);
+        if (bh > 0 && f > bh && f <= mnad_resident_max_rows()) {
+            long n = 0;
+            for (const ShotDev& d : h) n += (long)((d.nx + 63) / 64) * ((d.nz + f - 1) / f);
+            int fit = 0;
+            HIPCHK(mnad_resident_fit(f, &fit));
+            if ((long)fit * ncu >= n) {
+                bh = f;
+                total = n;
+                per_cu = fit;
+            }
+        }
+    }
+    if (bh == 0)
+        return fail(MNAD_ERR_ARG, "resident step kernel: the batch does not fit on the device "
+                    "as blocks of 64 x %d points or fewer", mnad_resident_max_rows());
+    (void)per_cu;
+    std::vector<ResBlock> blocks;
+    blocks.reserve(total);
+    for (int k = 0; k < (int)h.size(); ++k) {
+        const int nbx = (h[k].nx + 63) / 64, nbz = (h[k].nz + bh - 1) / bh;
+        const int first = (int)blocks.size();
+        for (int bz = 0; bz < nbz; ++bz)
+            for (int bx = 0; bx < nbx; ++bx) {
+                ResBlock rb{};
+                rb.shot = k;
+                rb.r0 = bz * bh;
+                rb.c0 = bx * 64;
+                int m = 0;
+                for (int oz = -1; oz <= 1; ++oz)
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        if (oz == 0 && ox == 0) continue;
+                        const int z = bz + oz, x = bx + ox;
+                        rb.nbr[m++] = (z >= 0 && z < nbz && x >= 0 && x < nbx)
+                                          ? first + z * nbx + x : -1;
+                    }
+                blocks.push_back(rb);
+            }
+    }
+    CHK(ensure_batch_buf(c, c->bdesc, h.size() * sizeof(ShotDev)));
+    CHK(ensure_batch_buf(c, c->btab, blocks.size() * sizeof(ResBlock)));
+    // edge records of every block, then the give-up word
+    const long nrec = (long)blocks.size() * mnad_resident_edge_records(bh);
+    CHK(ensure_batch_buf(c, c->bedge, (size_t)nrec * 8 + 256));
+    CHK(copy_in(c, c->bdesc.p, h.data(), h.size() * sizeof(ShotDev)));
+    CHK(copy_in(c, c->btab.p, blocks.data(), blocks.size() * sizeof(ResBlock)));
+    auto* ebuf = c->bedge.as<unsigned long long>();
+    int* err = reinterpret_cast<int*>(ebuf + nrec);
+    // records (level 0: no step waits for it) and the give-up word start at
+    // zero (0.0f has all bits clear)
+    HIPCHK(mnad_launch_fill(c->bedge.as<float>(), nrec * 2 + 1, 0.0f, c->stream));
+    HIPCHK(mnad_launch_init_source(c->bdesc.as<ShotDev>(), (int)h.size(), c->stream));
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } evg{ev0, ev1};
+    HIPCHK(hipEventRecord(ev0, c->stream));
+    HIPCHK(mnad_launch_resident(c->bdesc.as<ShotDev>(), c->btab.as<ResBlock>(),
+                                (int)blocks.size(), ebuf, err, nsteps, bh, c->stream));
+    HIPCHK(hipEventRecord(ev1, c->stream));
+    int gave_up = 0;
+    HIPCHK(hipMemcpyAsync(&gave_up, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (gave_up)
+        return fail(MNAD_ERR_HIP, "resident step kernel: a block gave up waiting for its "
+                    "neighbours; the batch's fields are not valid");
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+    c->t_ms += ms;
+    c->t_launches += 1;
+    c->t_bytes += bytes;
+    return MNAD_OK;
+}
+
 int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
 {
     const int n = (int)shots.size();
@@ -615,6 +738,11 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         if (h[k].nlev != (two_step_kernel(kern) ? 4 : 2))
             return fail(MNAD_ERR_STATE, "shot built for %d time levels cannot run on step kernel %d",
                         h[k].nlev, kern);
+    }
+    if (kern == 9) {
+        double bytes = 0.0;
+        for (int k = 0; k < n; ++k) bytes += algorithmic_bytes(h[k]);
+        return run_resident(c, h, nsteps, bytes);
     }
     std::vector<TileRef> table;
     build_tiles(c, h, table);

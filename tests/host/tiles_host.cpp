@@ -30,6 +30,14 @@ hipError_t mnad_launch_make_radj(const float*, const float*, const float*, int, 
 hipError_t mnad_launch_stack(const float*, const float*, const float*, float, long, float*, float*,
                              float*, float*, hipStream_t) { return hipErrorNotSupported; }
 hipError_t mnad_launch_fill(float*, long, float, hipStream_t) { return hipErrorNotSupported; }
+int mnad_resident_max_rows() { return 64; }
+hipError_t mnad_resident_fit(int, int*) { return hipErrorNotSupported; }
+long mnad_resident_edge_records(int) { return 0; }
+hipError_t mnad_launch_resident(const mnad::ShotDev*, const mnad::ResBlock*, int,
+                                unsigned long long*, int*, int, int, hipStream_t)
+{
+    return hipErrorNotSupported;
+}
 }
 
 namespace mnad {

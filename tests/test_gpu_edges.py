@@ -1,6 +1,6 @@
 """Edge cases of the HIP path against the FP64 oracle (relative L2 <= 1e-4 on
 gathers and final fields) for the default kernel, its CPML-prefetch variant,
-the four-column kernel and the two-step kernel: no CPML, the shortest runs, no
+the four-column kernel, the two-step kernel and the resident-block kernel: no CPML, the shortest runs, no
 receivers, sources on the grid corners, receivers on every boundary
 row/column, odd step counts for the two-step kernels (the last launch runs one
 step), and a grid just large enough for its CPML strips."""
@@ -14,7 +14,7 @@ from conftest import golden, rel_l2
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
-KERNELS = [(4, 1, 1), (4, 1, 4), (5, 1, 4), (6, 1, 2), (7, 3, 16)]
+KERNELS = [(4, 1, 1), (4, 1, 4), (5, 1, 4), (6, 1, 2), (7, 3, 16), (9, 1, 8), (9, 4, 8)]
 
 
 def _say(msg):

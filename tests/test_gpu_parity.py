@@ -163,7 +163,7 @@ def test_batched_shots_equal_single_shots(mnad, ctx):
         assert np.array_equal(batched[k], alone["gather"]), k
 
 
-@pytest.mark.parametrize("kernel,rz,br", [(3, 1, 16), (7, 3, 16), (7, 5, 8), (6, 1, 2)])
+@pytest.mark.parametrize("kernel,rz,br", [(3, 1, 16), (7, 3, 16), (7, 5, 8), (6, 1, 2), (9, 2, 8)])
 def test_two_step_and_four_column_kernels_batched(mnad, kernel, rz, br):
     """Eight C1 shots in one launch sequence (several rounds of workgroups
     per launch, so blocks that finish early overlap blocks that have not
@@ -236,7 +236,8 @@ def test_launch_shapes_agree(mnad, ctx, oracle, kernel, rz, br, strip_first):
 
 
 @pytest.mark.parametrize("kernel,rz,br", [(3, 1, 16), (4, 1, 4), (4, 1, 1), (5, 1, 4), (6, 1, 4),
-                                          (6, 1, 1), (7, 3, 4), (7, 3, 16), (7, 7, 8)])
+                                          (6, 1, 1), (7, 3, 4), (7, 3, 16), (7, 7, 8), (9, 1, 8),
+                                          (9, 4, 8)])
 def test_kernel_families_rtm_and_multi_tile(mnad, kernel, rz, br):
     """Each step-kernel family through the whole RTM (snapshots, imaging,
     receiver injection; f32, f16 and bf16 snapshots) against the stacked fixture
@@ -357,6 +358,65 @@ def test_pipelined_rtm_full_size_c2(mnad, ctx):
     assert lit.mean() > 0.001
     assert np.array_equal(imgs[0][1], imgs[1][1])
     assert np.array_equal(imgs[0][0][lit], imgs[1][0][lit])
+
+
+def test_resident_kernel_full_size_c2_rtm_bitwise(mnad):
+    """The resident-block kernel (one cooperative launch for every step, LDS
+    blocks exchanging edge records) gives the default kernel's gathers, image
+    and illumination bitwise on a full-size C2 RTM shot (561 x 1061, nt 300:
+    two forwards in one launch, then the adjoint)."""
+    _say("resident kernel, full-size C2 RTM")
+    wl = _c2_inputs(300)
+    v = wl["v"]
+    g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    out = []
+    for tun in ((4, 1, 1), (9, 2, 8)):
+        ctx = mnad.Context(0)
+        try:
+            ctx.set_tuning(kernel=tun[0], rows_per_thread=tun[1], block_rows=tun[2])
+            sid = ctx.stack_create(g)
+            gat = ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], [wl["zs"]], [wl["xs"]],
+                                [(wl["zr"], wl["xr"])], snap_fmt=mnad.SNAP_F16, batch=1,
+                                want_gathers=True)
+            img, ill, _ = ctx.stack_image(sid, g)
+            ctx.stack_destroy(sid)
+            out.append((gat[0][0], gat[1][0], img, ill))
+        finally:
+            ctx.close()
+    assert np.abs(out[0][0]).max() > 0
+    assert (out[0][3] > 0).mean() > 0.001
+    for a, b in zip(out[0], out[1]):   # the image is NaN where nothing was lit
+        assert np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("rows", ["", "24", "40", "64"])
+def test_resident_kernel_block_heights(mnad, rows, monkeypatch):
+    """Blocks of 4 (the smallest, several block rows per shot) up to 64 rows
+    (C1 in four block rows, the last one partial) and three shots per launch:
+    gathers and final fields equal the default kernel's bitwise, and a second
+    launch on the same context (edge records left by the first) too."""
+    _say(f"resident kernel, block height {rows or 'auto'}")
+    if rows:
+        monkeypatch.setenv("MNAD_RESIDENT_ROWS", rows)
+    v, w, npml, xs, zr, xr, dt = _c1_inputs()
+    w = w[:250]
+    g = mnad.Grid(v.shape[0], v.shape[1], npml, 10.0, 10.0, dt)
+    srcs = [npml + 3, xs, v.shape[1] - npml - 4]
+    recs = [(zr, xr)] * 3
+    ctx = mnad.Context(0)
+    try:
+        ref = ctx.fwd_shots(g, v, w, [npml] * 3, srcs, recs, batch=3)
+        ref1 = ctx.fwd(g, v, w, npml, xs, zr, xr)
+        ctx.set_tuning(kernel=9, rows_per_thread=1, block_rows=8)
+        for rep in range(2):
+            got = ctx.fwd_shots(g, v, w, [npml] * 3, srcs, recs, batch=3)
+            for k in range(3):
+                assert np.array_equal(got[k], ref[k]), (rep, k)
+        got1 = ctx.fwd(g, v, w, npml, xs, zr, xr)
+        for key in ("gather", "U", "V", "W"):
+            assert np.array_equal(got1[key], ref1[key]), key
+    finally:
+        ctx.close()
 
 
 @pytest.mark.parametrize("fmt", ["SNAP_F32", "SNAP_F16", "SNAP_BF16"])
