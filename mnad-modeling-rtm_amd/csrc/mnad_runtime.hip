@@ -633,16 +633,13 @@ static int run_resident(mnad_ctx* c, std::vector<ShotDev>& h, int nsteps, double
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     // the most blocks per CU up to c->rz at which the batch fits on the device
     long total = 0;
-    int bh = 0, per_cu = 0;
+    int bh = 0;
     for (int want = std::max(c->rz, 1); want >= 1 && bh == 0; --want) {
         const int b = resident_rows(h, ncu, want, &total);
         if (b == 0) continue;
         int fit = 0;
         HIPCHK(mnad_resident_fit(b, &fit));
-        if ((long)fit * ncu >= total) {
-            bh = b;
-            per_cu = fit;
-        }
+        if ((long)fit * ncu >= total) bh = b;
     }
     // MNAD_RESIDENT_ROWS forces a (larger) block height, e.g. to test long
     // blocks on small grids
@@ -656,14 +653,12 @@ static int run_resident(mnad_ctx* c, std::vector<ShotDev>& h, int nsteps, double
             if ((long)fit * ncu >= n) {
                 bh = f;
                 total = n;
-                per_cu = fit;
             }
         }
     }
     if (bh == 0)
         return fail(MNAD_ERR_ARG, "resident step kernel: the batch does not fit on the device "
                     "as blocks of 64 x %d points or fewer", mnad_resident_max_rows());
-    (void)per_cu;
     std::vector<ResBlock> blocks;
     blocks.reserve(total);
     for (int k = 0; k < (int)h.size(); ++k) {
