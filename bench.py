@@ -422,7 +422,34 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (x3) to calibrate the "
                          "FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--copy-ceiling", action="store_true",
+                    help="only time plain 1 and 4 GiB device copies and fills (HIP events): "
+                         "the HBM rate a simple streaming kernel reaches on this device")
     args = ap.parse_args()
+    if args.copy_ceiling:
+        import torch
+        out = {}
+        for gib in (1, 4):
+            n = gib * 2**30 // 4
+            x = torch.ones(n, device="cuda")
+            y = torch.empty_like(x)
+            for name, fn, nbytes in (("copy", lambda: y.copy_(x), 8 * n),
+                                     ("fill", lambda: y.fill_(2.0), 4 * n)):
+                for _ in range(3):
+                    fn()
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(10):
+                    fn()
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / 10
+                out[f"{name}_{gib}GiB_GBps"] = round(nbytes / ms / 1e6, 1)
+            del x, y
+        print(json.dumps(out), flush=True)
+        return
     if args.copy_calibration:
         import torch
         n = 256 * 2**20 // 4
