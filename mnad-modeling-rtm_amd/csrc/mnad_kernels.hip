@@ -814,16 +814,16 @@ __global__ __launch_bounds__(BX * BZR) void k_stepw(const ShotDev* __restrict__ 
 //  - Two slots suffice: a block writes level L+1 (over level L-1) only after
 //    it has read its neighbours' level L, which they publish after they have
 //    read its level L-1.
-//  - Every wait is bounded: a block that gives up sets *err and leaves, and
-//    blocks that see *err leave too, so the launch always drains.  The launch
-//    is cooperative, so all blocks are resident at once.
+//  - Every wait is bounded (`spin` polls of a record): a block that gives up
+//    sets *err and leaves, and blocks that see *err leave too, so the launch
+//    always drains.  The launch is cooperative, so all blocks are resident at
+//    once.
 //  - After its last step a block writes its last two levels in full to the
 //    level buffers in global memory.
 // CPML memories, gathers, snapshots and the image stay in global memory; each
 // point's are touched only by the block that owns it.
 constexpr int RES_NW = 8;            // wavefronts per block
 constexpr int RES_LW = BX + 4;       // LDS row: the block's 64 columns and 2 each side
-constexpr int RES_SPIN = 1 << 20;    // polls of one record before a wait gives up
 
 // Edge area of one block: [slot][field][ne] records, ne = 256 + 4 bh:
 //   top rows 0, 1 (r * 64 + c), bottom rows bh-2, bh-1 (128 + ...),
@@ -981,8 +981,10 @@ __device__ __forceinline__ int res_rec(const ShotDev& d, int j, int i)
 
 __global__ __launch_bounds__(BX * RES_NW) void k_res(const ShotDev* __restrict__ shots,
                                                      const ResBlock* __restrict__ blocks,
-                                                     u64* ebuf, int* err, int nsteps, int bh)
+                                                     u64* ebuf, int* err, int nsteps, int bh,
+                                                     int spin)
 {
+    // spin: polls of one edge record before the wait gives up
     // [2 buffers][U, Vs, Ws][(bh + 4) x RES_LW], then rx [bh x BX]
     extern __shared__ float lds[];
     __shared__ int give_up;
@@ -1158,7 +1160,7 @@ __global__ __launch_bounds__(BX * RES_NW) void k_res(const ShotDev* __restrict__
                 for (int f = 0; f < 3; ++f) {
                     u64 y = x[h][f];
                     for (int n = 0; ok && (int)(y >> 32) != s + 1; ++n) {
-                        if (n >= RES_SPIN || __hip_atomic_load(err, __ATOMIC_RELAXED,
+                        if (n >= spin || __hip_atomic_load(err, __ATOMIC_RELAXED,
                                                                __HIP_MEMORY_SCOPE_AGENT) != 0) {
                             ok = false;
                             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2034,12 +2036,12 @@ long mnad_resident_edge_records(int bh) { return 2L * 3 * res_edge_cells(bh); }
 // Every step of a batch in one cooperative launch (all blocks resident);
 // ebuf (nblocks * mnad_resident_edge_records(bh) records) and *err must be zero.
 hipError_t mnad_launch_resident(const ShotDev* shots, const ResBlock* blocks, int nblocks,
-                                unsigned long long* ebuf, int* err, int nsteps, int bh,
+                                unsigned long long* ebuf, int* err, int nsteps, int bh, int spin,
                                 hipStream_t st)
 {
     const void* k = res_kernel(bh);
-    if (!k || nblocks <= 0) return hipErrorInvalidValue;
-    void* args[] = {&shots, &blocks, &ebuf, &err, &nsteps, &bh};
+    if (!k || nblocks <= 0 || spin < 0) return hipErrorInvalidValue;
+    void* args[] = {&shots, &blocks, &ebuf, &err, &nsteps, &bh, &spin};
     return hipLaunchCooperativeKernel(k, dim3(nblocks), dim3(BX, RES_NW), args, res_lds(bh), st);
 }
 

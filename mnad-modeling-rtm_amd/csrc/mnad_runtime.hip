@@ -46,7 +46,7 @@ int mnad_resident_max_rows();
 hipError_t mnad_resident_fit(int bh, int* per_cu);
 long mnad_resident_edge_records(int bh);
 hipError_t mnad_launch_resident(const ShotDev* shots, const ResBlock* blocks, int nblocks,
-                                unsigned long long* ebuf, int* err, int nsteps, int bh,
+                                unsigned long long* ebuf, int* err, int nsteps, int bh, int spin,
                                 hipStream_t st);
 }
 
@@ -702,8 +702,12 @@ static int run_resident(mnad_ctx* c, std::vector<ShotDev>& h, int nsteps, double
         ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
     } evg{ev0, ev1};
     HIPCHK(hipEventRecord(ev0, c->stream));
+    // polls of one neighbour record before a block gives up (about a second);
+    // MNAD_RESIDENT_SPIN lowers it, e.g. to 0 to exercise the give-up path
+    int spin = 1 << 20;
+    if (const char* e = getenv("MNAD_RESIDENT_SPIN")) spin = std::max(0, atoi(e));
     HIPCHK(mnad_launch_resident(c->bdesc.as<ShotDev>(), c->btab.as<ResBlock>(),
-                                (int)blocks.size(), ebuf, err, nsteps, bh, c->stream));
+                                (int)blocks.size(), ebuf, err, nsteps, bh, spin, c->stream));
     HIPCHK(hipEventRecord(ev1, c->stream));
     int gave_up = 0;
     HIPCHK(hipMemcpyAsync(&gave_up, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));

@@ -34,7 +34,7 @@ int mnad_resident_max_rows() { return 64; }
 hipError_t mnad_resident_fit(int, int*) { return hipErrorNotSupported; }
 long mnad_resident_edge_records(int) { return 0; }
 hipError_t mnad_launch_resident(const mnad::ShotDev*, const mnad::ResBlock*, int,
-                                unsigned long long*, int*, int, int, hipStream_t)
+                                unsigned long long*, int*, int, int, int, hipStream_t)
 {
     return hipErrorNotSupported;
 }

@@ -419,6 +419,43 @@ def test_resident_kernel_block_heights(mnad, rows, monkeypatch):
         ctx.close()
 
 
+def test_resident_kernel_give_up_path(mnad, monkeypatch):
+    """With the poll bound forced to zero from the host (MNAD_RESIDENT_SPIN=0),
+    a block whose neighbour record is not yet out gives up at once; the launch
+    still drains (every block exits), the call reports the documented error or
+    completes exactly, and the same context then runs the batch exactly with
+    the normal bound."""
+    _say("resident kernel, give-up path")
+    v, w, npml, xs, zr, xr, dt = _c1_inputs()
+    w = w[:200]
+    g = mnad.Grid(v.shape[0], v.shape[1], npml, 10.0, 10.0, dt)
+    srcs = [npml + 3, xs, v.shape[1] - npml - 4]
+    recs = [(zr, xr)] * 3
+    ctx = mnad.Context(0)
+    try:
+        ref = ctx.fwd_shots(g, v, w, [npml] * 3, srcs, recs, batch=3)
+        ctx.set_tuning(kernel=9, rows_per_thread=2, block_rows=8)
+        monkeypatch.setenv("MNAD_RESIDENT_SPIN", "0")
+        gave_up = 0
+        for _ in range(3):
+            try:
+                got = ctx.fwd_shots(g, v, w, [npml] * 3, srcs, recs, batch=3)
+            except mnad.MnadError as e:
+                assert e.code == mnad.ERR_HIP
+                assert "gave up waiting for its neighbours" in str(e)
+                gave_up += 1
+            else:
+                for k in range(3):
+                    assert np.array_equal(got[k], ref[k]), k
+        _say(f"  gave up in {gave_up} of 3 launches")
+        monkeypatch.delenv("MNAD_RESIDENT_SPIN")
+        got = ctx.fwd_shots(g, v, w, [npml] * 3, srcs, recs, batch=3)
+        for k in range(3):
+            assert np.array_equal(got[k], ref[k]), k
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("fmt", ["SNAP_F32", "SNAP_F16", "SNAP_BF16"])
 def test_snapshot_file_handoff_is_exact(mnad, ctx, tmp_path, fmt):
     """The cross-process hand-off of the source wavefield (mnad_snap_save in
