@@ -240,6 +240,8 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags);
 void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table);
 // Step kernels that advance two time levels per launch (shots keep four).
 inline bool two_step_kernel(int kernel) { return kernel == 3 || kernel == 7; }
+// Step kernel that runs a whole batch in one launch (resident blocks).
+inline bool resident_kernel(int kernel) { return kernel == 9; }
 
 // Output tile of the current step kernel (columns x rows) and steps per launch.
 void tile_shape(const mnad_ctx* c, int* tw, int* th, int* steps_per_launch);

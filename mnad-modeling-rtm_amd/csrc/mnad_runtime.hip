@@ -738,7 +738,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
             return fail(MNAD_ERR_STATE, "shot built for %d time levels cannot run on step kernel %d",
                         h[k].nlev, kern);
     }
-    if (kern == 9) {
+    if (resident_kernel(kern)) {
         double bytes = 0.0;
         for (int k = 0; k < n; ++k) bytes += algorithmic_bytes(h[k]);
         return run_resident(c, h, nsteps, bytes);

@@ -427,11 +427,12 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
         delete s;
         return rc;
     }
-    // the server steps one time level per launch (admission at any step)
-    const bool two = two_step_kernel(ctx->kernel);
-    s->c->kernel = two ? kDefaultKernel : ctx->kernel;
-    s->c->rz = two ? kDefaultRows : ctx->rz;
-    s->c->brows = two ? kDefaultBrows : ctx->brows;
+    // the server steps one time level per launch (admission at any step), so
+    // the two-step and the whole-batch kernels fall back to the default
+    const bool other = two_step_kernel(ctx->kernel) || resident_kernel(ctx->kernel);
+    s->c->kernel = other ? kDefaultKernel : ctx->kernel;
+    s->c->rz = other ? kDefaultRows : ctx->rz;
+    s->c->brows = other ? kDefaultBrows : ctx->brows;
     s->c->strip_first = ctx->strip_first;
     s->c->strip_weight = ctx->strip_weight;
     s->c->x4_weight = ctx->x4_weight;

@@ -157,3 +157,28 @@ def test_work_limit_admits_first_come_first_served(mnad):
         ctx.serve_stop()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("kernel,rz,br", [(9, 2, 8), (7, 3, 16)])
+def test_server_on_whole_batch_or_two_step_tuning(mnad, kernel, rz, br):
+    """A context tuned to a kernel that cannot step one level per launch (the
+    resident-block kernel runs a whole batch per launch, the two-step kernels
+    two levels) still serves: the server falls back to the default kernel and
+    its results equal those of a server on a default context bitwise."""
+    _say(f"serving with context kernel {kernel}")
+    f, r = _model_req(mnad, "fwd_square")
+    out = []
+    for tun in (None, (kernel, rz, br)):
+        ctx = mnad.Context(0)
+        try:
+            if tun:
+                ctx.set_tuning(kernel=tun[0], rows_per_thread=tun[1], block_rows=tun[2])
+            ctx.serve_start(hbm_budget=0, chunk_steps=16)
+            res, info = ctx.wait(_submit(ctx, r))
+            assert info["status"] == mnad.OK
+            out.append(np.asarray(res["gather"]))
+            ctx.serve_stop()
+        finally:
+            ctx.close()
+    assert rel_l2(out[0], f["gather"]) < 1e-4
+    assert np.array_equal(out[0], out[1])
