@@ -2021,13 +2021,14 @@ hipError_t mnad_resident_fit(int bh, int* per_cu)
 {
     const void* k = res_kernel(bh);
     if (!k) return hipErrorInvalidValue;
-    const size_t lds = res_lds(bh);
-    if (lds > 65536) {   // above the default dynamic-LDS limit of a launch
-        const hipError_t e =
-            hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, BX * RES_NW, lds);
+    // Allow the largest block (64 rows, above the default 64 KiB dynamic-LDS
+    // limit of a launch) once and for all: the same value on every call, so
+    // contexts on other threads cannot lower it between this check and their
+    // launch.
+    const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)res_lds(64));
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k, BX * RES_NW, res_lds(bh));
 }
 
 // 64-bit edge records one block of height bh exchanges (both slots, three fields).
