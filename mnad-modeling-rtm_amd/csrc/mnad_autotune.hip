@@ -30,7 +30,7 @@ struct Variant {
 };
 
 const Variant kVariants[] = {{4, 1, 4}, {4, 1, 1}, {0, 1, 4}, {6, 1, 2}, {6, 1, 4},
-                             {7, 3, 16}, {7, 5, 8}, {7, 7, 16}};
+                             {10, 1, 2}, {7, 3, 16}, {7, 5, 8}, {7, 7, 16}};
 
 // Time `steps` steps of `b` forward shots (source at the centre, no
 // receivers, no snapshots) with the context's current tuning; returns

@@ -52,7 +52,7 @@ static bool tuning_ok(int kernel, int rows, int brows)
     if (kernel == 3) return brows == 16 && rows == 1;
     if (kernel == 4) return rows == 1 && (brows == 1 || brows == 2 || brows == 4 || brows == 8);
     if (kernel == 5) return rows == 1 && (brows == 1 || brows == 4);
-    if (kernel == 6) return rows == 1 && (brows == 1 || brows == 2 || brows == 4);
+    if (kernel == 6 || kernel == 10) return rows == 1 && (brows == 1 || brows == 2 || brows == 4);
     if (kernel == 7)   // output rows 4 * rows; (rows + 1) * 4 phase-1 rows split over block_rows waves
         return (rows == 3 && (brows == 4 || brows == 8 || brows == 16)) ||
                (rows == 5 && (brows == 4 || brows == 8)) ||

@@ -240,6 +240,8 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags);
 void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table);
 // Step kernels that advance two time levels per launch (shots keep four).
 inline bool two_step_kernel(int kernel) { return kernel == 3 || kernel == 7; }
+// Columns per lane of the interior tiles of kernels 6 (4) and 10 (2); 0 for others.
+int interior_columns_per_lane(int kernel);
 // Step kernel that runs a whole batch in one launch (resident blocks).
 inline bool resident_kernel(int kernel) { return kernel == 9; }
 

@@ -655,7 +655,7 @@ __device__ __forceinline__ void snap_image_w(const ShotDev& d, int s, long qrow,
 // after the increment.  Holding one set across the increment arithmetic keeps
 // the kernel at 8 waves per SIMD (two sets would take it to 6).
 // SKIPX4: points of the four-column region (rows npml .. nz-npml-1, columns
-// npml .. x4hi-1) belong to step_x4 and are left alone.
+// npml .. x4hi-1) belong to step_xn and are left alone.
 template <bool PF, bool SKIPX4>
 __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0, unsigned lane)
 {
@@ -1197,19 +1197,18 @@ __global__ __launch_bounds__(BX * RES_NW) void k_res(const ShotDev* __restrict__
     }
 }
 
-// ------------------------------------------------ four columns per lane
+// ------------------------------------------------ four (or two) columns per lane
 // The interior of a shot (rows npml .. nz-npml-1, columns npml .. nx-npml-1:
-// no CPML there) in waves of 64 lanes x 4 columns.  Lane l holds columns
-// i0 + 4l .. i0 + 4l + 3 of row j, one 16-byte load per field row, and takes
-// the columns just left / right of its four from lanes l-1 / l+1 with DPP
-// shifts across the wavefront (wave_shr:1 / wave_shl:1).  Lanes 0 and 63
-// only supply those halo columns, so a wave updates 248 columns with 15
-// loads and 3 stores per lane, against 64 columns with 15 loads for one
-// point per lane: a quarter of the vector-memory instructions and of the
-// redundant row reads.  All 64 lanes stay active through the loads and the
-// shifts.  Lane 0's first column (i0) is a multiple of 4, so every row load
-// is 16-byte aligned.
-constexpr int X4W = 4 * (BX - 2);   // columns updated per wave
+// no CPML there) in waves of 64 lanes x NC columns (kernel 6: NC = 4,
+// kernel 10: NC = 2).  Lane l holds columns i0 + NC l .. i0 + NC l + NC-1 of
+// row j, one 4NC-byte load per field row, and takes the columns just left /
+// right of its own from lanes l-1 / l+1 with DPP shifts across the wavefront
+// (wave_shr:1 / wave_shl:1).  Lanes 0 and 63 only supply those halo columns,
+// so a wave updates 62 NC columns with 15 loads and 3 stores per lane,
+// against 64 columns with 15 loads for one point per lane.  All 64 lanes stay
+// active through the loads and the shifts.  Lane 0's first column (i0) is a
+// multiple of NC, so every row load is aligned to its size.
+constexpr int X4W = 4 * (BX - 2);   // columns updated per wave, four columns per lane
 
 __device__ __forceinline__ float lane_prev(float x)   // lane l-1's x (0 in lane 0)
 {
@@ -1225,42 +1224,55 @@ __device__ __forceinline__ float lane_next(float x)   // lane l+1's x (0 in lane
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) f32x4 gf32x4;
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) f32x2 gf32x2;
 
-// Columns -1 .. 4 of a lane's four.
-struct Row4 {
-    float e[4], l, r;
-    __device__ __forceinline__ float at(int c) const { return c < 0 ? l : (c > 3 ? r : e[c]); }
+template <int NC> struct VecN;
+template <> struct VecN<4> { typedef f32x4 T; typedef gf32x4 G; };
+template <> struct VecN<2> { typedef f32x2 T; typedef gf32x2 G; };
+
+// Columns -1 .. NC of a lane's NC.
+template <int NC>
+struct RowN {
+    float e[NC], l, r;
+    __device__ __forceinline__ float at(int c) const { return c < 0 ? l : (c > NC - 1 ? r : e[c]); }
 };
 
-__device__ __forceinline__ Row4 row4(const gfloat* base, unsigned lane)
+template <int NC>
+__device__ __forceinline__ RowN<NC> rown(const gfloat* base, unsigned lane)
 {
-    const f32x4 x = ((const gf32x4*)base)[lane];
-    Row4 o;
-    o.e[0] = x.x; o.e[1] = x.y; o.e[2] = x.z; o.e[3] = x.w;
-    o.l = lane_prev(x.w);
-    o.r = lane_next(x.x);
+    const typename VecN<NC>::T x = ((const typename VecN<NC>::G*)base)[lane];
+    RowN<NC> o;
+#pragma unroll
+    for (int e = 0; e < NC; ++e) o.e[e] = x[e];
+    o.l = lane_prev(x[NC - 1]);
+    o.r = lane_next(x[0]);
     return o;
 }
 
-__device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, int lo, int hi,
+template <int NC>
+__device__ __forceinline__ void step_xn(const ShotDev& d, int s, int j, int i0, int lo, int hi,
                                         unsigned lane)
 {
+    typedef typename VecN<NC>::T VT;
+    typedef typename VecN<NC>::G GVT;
     const int cb = s & 1;
     const long P = d.pitch;
     const long org = (long)(j + MNAD_ZO) * P + (i0 + MNAD_XO);   // storage of (j, i0)
     const gfloat* U = gp(d.f[cb][0]) + org;
     const gfloat* V = gp(d.f[cb][1]) + org;
     const gfloat* W = gp(d.f[cb][2]) + org;
-    const Row4 un = row4(U - P, lane), uc = row4(U, lane), us = row4(U + P, lane);
-    const Row4 vn = row4(V - P, lane), vc = row4(V, lane), vs = row4(V + P, lane);
-    const Row4 wn = row4(W - P, lane), wc = row4(W, lane), ws = row4(W + P, lane);
-    const f32x4 wnn = ((const gf32x4*)(W - 2 * P))[lane];
-    const f32x4 wss = ((const gf32x4*)(W + 2 * P))[lane];
-    const float vl2 = lane_prev(vc.e[2]), vr2 = lane_next(vc.e[1]);   // columns -2, +5
-    const f32x4 rx4 = ((const gf32x4*)(gp(d.rx) + org))[lane];
-    float nu[4], nv[4], nw[4], dus[4];
+    const RowN<NC> un = rown<NC>(U - P, lane), uc = rown<NC>(U, lane), us = rown<NC>(U + P, lane);
+    const RowN<NC> vn = rown<NC>(V - P, lane), vc = rown<NC>(V, lane), vs = rown<NC>(V + P, lane);
+    const RowN<NC> wn = rown<NC>(W - P, lane), wc = rown<NC>(W, lane), ws = rown<NC>(W + P, lane);
+    const VT wnn = ((const GVT*)(W - 2 * P))[lane];
+    const VT wss = ((const GVT*)(W + 2 * P))[lane];
+    // columns -2 and NC+1 (the Vs row's second neighbours beyond the lane)
+    const float vl2 = lane_prev(vc.e[NC - 2]), vr2 = lane_next(vc.e[1]);
+    const VT rx4 = ((const GVT*)(gp(d.rx) + org))[lane];
+    float nu[NC], nv[NC], nw[NC], dus[NC];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < NC; ++e) {
         Nbh u, v, w;
         u.nw = un.at(e - 1); u.n = un.e[e]; u.ne = un.at(e + 1);
         u.w = uc.at(e - 1);  u.c = uc.e[e]; u.e = uc.at(e + 1);
@@ -1272,7 +1284,7 @@ __device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, 
         w.w = wc.at(e - 1);  w.c = wc.e[e]; w.e = wc.at(e + 1);
         w.sw = ws.at(e - 1); w.s = ws.e[e]; w.se = ws.at(e + 1);
         const float vww = e >= 2 ? vc.e[e - 2] : (e == 1 ? vc.l : vl2);
-        const float vee = e <= 1 ? vc.e[e + 2] : (e == 2 ? vc.r : vr2);
+        const float vee = e <= NC - 3 ? vc.e[e + 2] : (e == NC - 2 ? vc.r : vr2);
         const float rx = rx4[e];
         float du, dv, dw;
         mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn[e], wss[e], du, dv, dw);
@@ -1280,17 +1292,17 @@ __device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, 
         nv[e] = 2.0f * v.c + dv;
         nw[e] = 2.0f * w.c + dw;
         dus[e] = du;
-        // one column at a time: interleaving the four raises the register count
+        // one column at a time: interleaving them raises the register count
         __builtin_amdgcn_sched_barrier(0);
     }
     // the previous level, loaded after the arithmetic (fewer live registers)
-    gf32x4* PU = (gf32x4*)(gp(d.f[cb ^ 1][0]) + org);
-    gf32x4* PV = (gf32x4*)(gp(d.f[cb ^ 1][1]) + org);
-    gf32x4* PW = (gf32x4*)(gp(d.f[cb ^ 1][2]) + org);
+    GVT* PU = (GVT*)(gp(d.f[cb ^ 1][0]) + org);
+    GVT* PV = (GVT*)(gp(d.f[cb ^ 1][1]) + org);
+    GVT* PW = (GVT*)(gp(d.f[cb ^ 1][2]) + org);
     {
-        const f32x4 pu = PU[lane], pv = PV[lane], pw = PW[lane];
+        const VT pu = PU[lane], pv = PV[lane], pw = PW[lane];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < NC; ++e) {
             nu[e] -= pu[e];
             nv[e] -= pv[e];
             nw[e] -= pw[e];
@@ -1298,11 +1310,11 @@ __device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, 
     }
 
     // owned columns: [lo, hi) of this tile, lanes 1 .. 62
-    const int c0 = i0 + 4 * (int)lane;
-    if (lane < 1 || lane > BX - 2 || c0 + 3 < lo || c0 >= hi) return;
+    const int c0 = i0 + NC * (int)lane;
+    if (lane < 1 || lane > BX - 2 || c0 + NC - 1 < lo || c0 >= hi) return;
     if (d.mode == MODE_FWD && j == d.src_j) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
+        for (int e = 0; e < NC; ++e)
             if (c0 + e == d.src_i) nu[e] += d.src_amp[s + 1];
     }
     if (d.nrec > 0) {
@@ -1310,7 +1322,7 @@ __device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, 
         if (off >= 0) {
             const gint* rc = uni(gp(d.rcol) + off);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < NC; ++e) {
                 const int c = c0 + e;
                 if (c < lo || c >= hi) continue;
                 const int r = rc[(unsigned)c];
@@ -1322,16 +1334,23 @@ __device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, 
             }
         }
     }
-    if (c0 >= lo && c0 + 3 < hi) {
-        PU[lane] = (f32x4){nu[0], nu[1], nu[2], nu[3]};
-        PV[lane] = (f32x4){nv[0], nv[1], nv[2], nv[3]};
-        PW[lane] = (f32x4){nw[0], nw[1], nw[2], nw[3]};
+    if (c0 >= lo && c0 + NC - 1 < hi) {
+        VT a, b2, c2;
+#pragma unroll
+        for (int e = 0; e < NC; ++e) {
+            a[e] = nu[e];
+            b2[e] = nv[e];
+            c2[e] = nw[e];
+        }
+        PU[lane] = a;
+        PV[lane] = b2;
+        PW[lane] = c2;
     } else {
         gfloat* qu = (gfloat*)(PU + lane);
         gfloat* qv = (gfloat*)(PV + lane);
         gfloat* qw = (gfloat*)(PW + lane);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < NC; ++e) {
             if (c0 + e < lo || c0 + e >= hi) continue;
             qu[e] = nu[e];
             qv[e] = nv[e];
@@ -1342,7 +1361,7 @@ __device__ __forceinline__ void step_x4(const ShotDev& d, int s, int j, int i0, 
         const int np = d.npml;
         const long qrow = (long)(j - np) * (d.nx - 2 * np);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < NC; ++e) {
             const int c = c0 + e;
             if (c < lo || c >= hi) continue;
             snap_image_w(d, s, qrow, (unsigned)(c - np), dus[e], nu[e]);
@@ -1379,8 +1398,41 @@ __global__ __launch_bounds__(BX * BZR) void k_stepx(const ShotDev* __restrict__ 
         const int j = d.npml + tb * BZR + wy;
         if (j >= d.nz - d.npml) return;
         const int lo = d.x4lo + tx * X4W;
-        step_x4(d, s, j, lo - 4, lo < d.npml ? d.npml : lo, min(lo + X4W, d.x4hi),
-                threadIdx.x);
+        step_xn<4>(d, s, j, lo - 4, lo < d.npml ? d.npml : lo, min(lo + X4W, d.x4hi),
+                   threadIdx.x);
+    }
+}
+
+// Kernel 10: kernel 6 with two columns per lane in the interior (124 columns
+// per wave, half the registers of four).
+constexpr int XC2 = 2 * (BX - 2);   // columns updated per wave, two columns per lane
+
+template <int BZR>
+__global__ __launch_bounds__(BX * BZR) void k_stepx2(const ShotDev* __restrict__ shots,
+                                                     const TileRef* __restrict__ tiles,
+                                                     const int* __restrict__ base, int rel)
+{
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;
+    const int wy = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    if (tr.tile < d.nt1) {
+        const int tj = tr.tile / d.ntx;
+        const int ti = tr.tile - tj * d.ntx;
+        const int j = tj * BZR + wy;
+        if (j >= d.nz) return;
+        step_one<false, true>(d, s, j, ti * BX, threadIdx.x);
+    } else {
+        const int u = tr.tile - d.nt1;
+        const int tb = u / d.nx4;
+        const int tx = u - tb * d.nx4;
+        const int j = d.npml + tb * BZR + wy;
+        if (j >= d.nz - d.npml) return;
+        const int lo = d.x4lo + tx * XC2;
+        step_xn<2>(d, s, j, lo - 2, lo < d.npml ? d.npml : lo, min(lo + XC2, d.x4hi),
+                   threadIdx.x);
     }
 }
 
@@ -1977,6 +2029,12 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL(k_stepx<2>, g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
     else if (kernel == 6 && brows == 4 && rows == 1)
         hipLaunchKernelGGL(k_stepx<4>, g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 10 && brows == 1 && rows == 1)
+        hipLaunchKernelGGL(k_stepx2<1>, g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
+    else if (kernel == 10 && brows == 2 && rows == 1)
+        hipLaunchKernelGGL(k_stepx2<2>, g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
+    else if (kernel == 10 && brows == 4 && rows == 1)
+        hipLaunchKernelGGL(k_stepx2<4>, g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 5 && brows == 1 && rows == 1)
         hipLaunchKernelGGL((k_stepw<1, true>), g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
     else if (kernel == 5 && brows == 4 && rows == 1)

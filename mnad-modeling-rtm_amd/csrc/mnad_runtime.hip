@@ -512,22 +512,24 @@ void tile_shape(const mnad_ctx* c, int* tw, int* th, int* spl)
     }
 }
 
-// Four-column interior tiles of step kernel 6 (X4W columns, th rows each).
-static void x4_layout(ShotDev& d, int th, int nt1)
+// Interior tiles of step kernels 6 and 10: nc = 4 or 2 columns per lane,
+// 62 nc columns and th rows per tile.
+static void x4_layout(ShotDev& d, int nc, int nt1)
 {
     const int np = d.npml;
     d.nt1 = nt1;
-    d.x4lo = np & ~3;           // lane 0 of the first tile starts at x4lo - 4: 16-byte aligned
+    d.x4lo = np & ~(nc - 1);   // lane 0 of the first tile starts at x4lo - nc: aligned loads
     d.x4hi = d.nx - np;
-    const int X4W = 248;
+    const int width = nc * 62;
     if (d.x4hi <= np || d.nz - 2 * np <= 0) {
         d.nx4 = 0;
         d.x4hi = np;            // no interior: the one-point tiles cover everything
     } else {
-        d.nx4 = (d.x4hi - d.x4lo + X4W - 1) / X4W;
+        d.nx4 = (d.x4hi - d.x4lo + width - 1) / width;
     }
-    (void)th;
 }
+
+int interior_columns_per_lane(int kernel) { return kernel == 6 ? 4 : (kernel == 10 ? 2 : 0); }
 
 void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table)
 {
@@ -539,18 +541,19 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
     const double sw = two_step_kernel(c->kernel) ? 1.0 : c->strip_weight;
     int tw, th, spl;
     tile_shape(c, &tw, &th, &spl);
-    const bool x4 = c->kernel == 6;
+    const int nc = interior_columns_per_lane(c->kernel);
+    const bool x4 = nc > 0;
     for (int k = 0; k < (int)h.size(); ++k) {
         const int ntx = (h[k].nx + tw - 1) / tw, ntz = (h[k].nz + th - 1) / th;
         const int np = h[k].npml;
         h[k].ntx = ntx;
-        x4_layout(h[k], th, ntx * ntz);
+        x4_layout(h[k], x4 ? nc : 4, ntx * ntz);
         const int nz = h[k].nz, nx = h[k].nx, x4hi = x4 ? h[k].x4hi : np;
         for (int t = 0; t < ntx * ntz; ++t) {
             const int tj = t / ntx, ti = t - tj * ntx;
             const int r0 = tj * th, r1 = std::min(nz, r0 + th), c0 = ti * tw,
                       c1 = std::min(nx, c0 + tw);
-            // kernel 6: one-point tiles lying wholly in the four-column interior are not listed
+            // kernels 6, 10: one-point tiles lying wholly in the interior are not listed
             if (x4 && r0 >= np && r1 <= nz - np && c0 >= np && c1 <= x4hi) continue;
             const bool st = c0 < np || c1 > nx - np || r0 < np || r1 > nz - np;
             list.push_back(TileRef{k, t});
@@ -558,12 +561,13 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
             work.push_back(st ? sw : 1.0);
         }
         if (x4) {
-            // a four-column tile: 248 columns at about 2/3 of the one-point cost per point
+            // a four-column tile: 248 columns at about 2/3 of the one-point cost per
+            // point; a two-column tile is taken as half of one
             const int nb = (nz - 2 * np + th - 1) / th;
             for (int u = 0; u < nb * h[k].nx4; ++u) {
                 list.push_back(TileRef{k, ntx * ntz + u});
                 strip.push_back(0);
-                work.push_back(c->x4_weight);
+                work.push_back(c->x4_weight * nc / 4.0);
             }
         }
     }

@@ -4,8 +4,8 @@
 //
 // For each case it prints one line "ok <name>" or "FAIL <name>: <reason>":
 //  - every grid point of every shot is covered by exactly one listed tile
-//    (one-point tiles of kernels 4/7, and for kernel 6 one-point tiles off
-//    the four-column interior plus the four-column tiles);
+//    (one-point tiles of kernels 4/7, and for kernels 6 and 10 one-point
+//    tiles off the multi-column interior plus the interior tiles);
 //  - the table length is a multiple of 8 (XCD x takes entries b % 8 == x);
 //  - each XCD's tiles are a contiguous range of the shot-major tile order;
 //  - the estimated work per XCD differs by at most two tile weights, half a
@@ -73,7 +73,8 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
     build_tiles(&c, h, table);
     int tw, th, spl;
     tile_shape(&c, &tw, &th, &spl);
-    const bool x4 = kernel == 6, two = two_step_kernel(kernel);
+    const int nc = interior_columns_per_lane(kernel);
+    const bool x4 = nc > 0, two = two_step_kernel(kernel);
     char buf[256];
     if (table.size() % 8 != 0) return "table length not a multiple of 8";
 
@@ -89,9 +90,9 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
         const ShotDev& d = h[t.shot];
         if (x4 && t.tile >= d.nt1) {
             const int u = t.tile - d.nt1, tb = u / d.nx4, tx = u - tb * d.nx4;
-            const int lo = d.x4lo + tx * 248;
+            const int lo = d.x4lo + tx * 62 * nc;
             for (int j = s.np + tb * th; j < std::min(s.nz - s.np, s.np + (tb + 1) * th); ++j)
-                for (int i = std::max(lo, s.np); i < std::min(lo + 248, d.x4hi); ++i)
+                for (int i = std::max(lo, s.np); i < std::min(lo + 62 * nc, d.x4hi); ++i)
                     cover[t.shot][(size_t)j * s.nx + i]++;
         } else {
             const int tj = t.tile / d.ntx, ti = t.tile - tj * d.ntx;
@@ -118,7 +119,7 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
         const ShotDev& d = h[t.shot];
         if (x4 && t.tile >= d.nt1) {
             *strip = false;
-            return c.x4_weight;
+            return c.x4_weight * nc / 4.0;
         }
         const int tj = t.tile / d.ntx, ti = t.tile - tj * d.ntx;
         *strip = is_strip_tile(tj * th, std::min(s.nz, (tj + 1) * th), ti * tw,
@@ -177,6 +178,8 @@ int main()
         {"k0_r2_c1", 0, 2, 4, 1, {c1}},
         {"k6_b2_c2", 6, 1, 2, 1, {c2}},
         {"k6_b4_mixed", 6, 1, 4, 1, {c1, c2, tiny, nocpml}},
+        {"k10_b1_c2", 10, 1, 1, 1, {c2}},
+        {"k10_b2_mixed", 10, 1, 2, 1, {c1, c2, tiny, nocpml}},
         {"k7_r5_b8_c2x3", 7, 5, 8, 1, {c2, c2, c2}},
         {"k7_r3_b16_mixed", 7, 3, 16, 1, {c1, tiny, c2}},
         {"k3_c1", 3, 1, 16, 1, {c1}},
