@@ -182,6 +182,8 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  *   with rows_per_thread 3, 5 or 7 and block_rows (wavefronts) 4, 8 or 16
  *   dividing 4 (rows_per_thread + 1); shots keep four time levels -- set it
  *   before creating shots;
+ * kernel 10 = kernel 6 with two columns per lane in the interior (8-byte
+ *   loads, 124 columns per wave, fewer registers): block_rows 1, 2 or 4, 1 row;
  * kernel 9 = resident blocks: one cooperative launch runs every step of a
  *   batch; a workgroup of 8 wavefronts keeps a 64-column block of one shot in
  *   LDS for the whole launch and exchanges only its two-cell edges with its
