@@ -1293,7 +1293,7 @@ __device__ __forceinline__ void step_xn(const ShotDev& d, int s, int j, int i0, 
         nw[e] = 2.0f * w.c + dw;
         dus[e] = du;
         // one column at a time: interleaving them raises the register count
-        __builtin_amdgcn_sched_barrier(0);
+        if (NC == 4) __builtin_amdgcn_sched_barrier(0);
     }
     // the previous level, loaded after the arithmetic (fewer live registers)
     GVT* PU = (GVT*)(gp(d.f[cb ^ 1][0]) + org);
