@@ -10,7 +10,8 @@
 //  - each XCD's tiles are a contiguous range of the shot-major tile order;
 //  - the estimated work per XCD differs by at most two tile weights, half a
 //    tile at either end of each XCD's range (two-step kernels: counts);
-//  - with strip_first, each XCD dispatches its CPML-strip tiles first.
+//  - with strip_first, each XCD dispatches its CPML-strip tiles first;
+// and the block heights the resident kernel (9) picks for a few batches.
 #include <algorithm>
 #include <cstdio>
 #include <string>
@@ -46,6 +47,10 @@ void server_destroy(mnad_ctx*) {}
 }
 
 using namespace mnad;
+
+namespace mnad {
+int resident_rows(const std::vector<ShotDev>& h, int ncu, int per_cu, long* nblocks);
+}
 
 struct Shape { int nz, nx, np; };
 
@@ -185,6 +190,40 @@ int main()
         {"k3_c1", 3, 1, 16, 1, {c1}},
     };
     int bad = 0;
+    // Block heights of the resident kernel (9): the smallest height >= 4 whose
+    // 64-column blocks number at most per_cu x ncu; 0 when none up to 64 does.
+    struct RCase {
+        const char* name;
+        std::vector<Shape> shapes;
+        int ncu, per_cu, bh;
+        long nblocks;
+    };
+    const Shape c3{811, 3461, 30}, c5{2061, 8061, 30};
+    const std::vector<RCase> rcases = {
+        {"res_c2_1shot", {c2}, 256, 2, 19, 17L * 30},
+        {"res_c2_2shots", {c2, c2}, 256, 2, 38, 2L * 17 * 15},
+        {"res_c1_3shots", {c1, c1, c1}, 256, 1, 12, 3L * 4 * 21},
+        {"res_c3_1shot", {c3}, 256, 4, 46, 55L * 18},
+        {"res_c5_too_big", {c5}, 256, 4, 0, 0},
+    };
+    for (const RCase& rc : rcases) {
+        std::vector<ShotDev> h(rc.shapes.size());
+        for (size_t k = 0; k < rc.shapes.size(); ++k) {
+            h[k] = ShotDev{};
+            h[k].nz = rc.shapes[k].nz;
+            h[k].nx = rc.shapes[k].nx;
+            h[k].npml = rc.shapes[k].np;
+        }
+        long n = 0;
+        const int bh = resident_rows(h, rc.ncu, rc.per_cu, &n);
+        if (bh == rc.bh && (bh == 0 || n == rc.nblocks)) {
+            printf("ok %s\n", rc.name);
+        } else {
+            printf("FAIL %s: block height %d, %ld blocks (expected %d, %ld)\n", rc.name, bh, n,
+                   rc.bh, rc.nblocks);
+            ++bad;
+        }
+    }
     for (const Case& cs : cases) {
         const std::string r = check(cs.name, cs.kernel, cs.rz, cs.brows, cs.strip_first, cs.shapes);
         if (r.empty()) {
