@@ -1,7 +1,7 @@
 /*
  * mnad_mex.c -- MATLAB MEX gateway over libmnad (include/mnad.h).
  *
- * Build inside MATLAB:
+ * Build inside MATLAB, with libmnad.so on the loader path at run time:
  *   mex -R2017b -I<repo>/include mnad_mex.c -L<repo>/mnad-modeling-rtm_amd/_build -lmnad
  *
  * Commands (first argument):

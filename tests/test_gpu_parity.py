@@ -1,7 +1,7 @@
 """HIP path (libmnad through its C-ABI) against the FP64 oracle and the golden
 fixtures.  Tolerances (relative L2 against FP64): gathers and final fields
-1e-4, RTM images 1e-3 -- the FP32 targets of the north-star metric; the
-snapshot storage format changes only the image error, stated per format."""
+1e-4, RTM images 1e-3 -- the FP32 path's targets; the snapshot storage
+format changes only the image error, stated per format."""
 import sys
 
 import numpy as np
