@@ -51,7 +51,8 @@ def kernel_name(t):
     return {0: f"mnad::k_step<{r}, {b}>", 1: "mnad::k_step_pf", 2: f"mnad::k_step_lds<{4 * r}>",
             3: "mnad::k_step2", 4: f"mnad::k_stepw<{b}, false>", 5: f"mnad::k_stepw<{b}, true>",
             6: f"mnad::k_stepx<{b}>", 7: f"mnad::k_step2w<{4 * r}, {b}>",
-            9: "mnad::k_res", 10: f"mnad::k_stepx2<{b}>"}.get(k, f"kernel {k}")
+            9: "mnad::k_res", 10: f"mnad::k_stepx2<{b}>",
+            11: "mnad::k_stepw2"}.get(k, f"kernel {k}")
 
 
 def c2_workload(nt, npml=30):

@@ -29,18 +29,26 @@ struct Variant {
     int kernel, rz, brows;
 };
 
-const Variant kVariants[] = {{4, 1, 4}, {4, 1, 1}, {0, 1, 4}, {6, 1, 2}, {6, 1, 4},
+const Variant kVariants[] = {{4, 1, 4}, {4, 1, 1}, {11, 2, 1}, {0, 1, 4}, {6, 1, 2}, {6, 1, 4},
                              {10, 1, 2}, {7, 3, 16}, {7, 5, 8}, {7, 7, 16}};
 
+// Snapshot period of an RTM forward on this model: T = floor(nt / nf / 2) with
+// nf = floor(nt vmin / (6 vmax)), about 3 vmax / vmin.
+int probe_period(const Model& m) { return std::max(3, (int)(3.0 * m.vmax / m.vmin)); }
+
 // Time `steps` steps of `b` forward shots (source at the centre, no
-// receivers, no snapshots) with the context's current tuning; returns
-// microseconds per shot-step of step-kernel time.
+// receivers) with the context's current tuning; returns microseconds per
+// shot-step of step-kernel time.  Like an RTM launch, about two shots in
+// three take f16 d2u snapshots every probe_period steps (the adjoint's
+// imaging term has the same shape), so that a variant whose snapshot or
+// imaging steps cost more is seen to.
 int probe(mnad_ctx* c, const Model& m, int b, int steps, double* us)
 {
     std::vector<double> w(steps, 0.0);
     for (int t = 0; t < steps; ++t) w[t] = std::sin(0.3 * t) * std::exp(-0.01 * t);
     std::vector<Shot> shots(b);
     std::vector<Shot*> run;
+    const int T = probe_period(m);
     for (int k = 0; k < b; ++k) {
         ShotSpec sp;
         sp.mode = MODE_FWD;
@@ -48,6 +56,12 @@ int probe(mnad_ctx* c, const Model& m, int b, int steps, double* us)
         sp.w = w.data();
         sp.zs = m.g.nz / 2;
         sp.xs = m.g.nx / 2;
+        if (k % 3 != 2 && steps >= T) {
+            sp.T = T;
+            sp.K = steps / T;
+            sp.ncol = sp.K;
+            sp.snap_fmt = MNAD_SNAP_F16;
+        }
         int rc = make_shot(c, m, sp, shots[k]);
         if (rc != MNAD_OK) return rc;
         run.push_back(&shots[k]);
@@ -75,7 +89,9 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
     const double pts = (double)g.nz * g.nx;
     const int steps = pts > 4e6 ? 2 * c->graph_chunk : 4 * c->graph_chunk;
     // shots per launch: powers of two up to max_shots, bounded by memory
-    const size_t per_shot = 6 * m.elems * 4 + 9 * ((size_t)g.nz * 2 * g.npml + (size_t)2 * g.npml * g.nx) * 4;
+    const size_t ni = (size_t)std::max(0, g.nz - 2 * g.npml) * std::max(0, g.nx - 2 * g.npml);
+    const size_t per_shot = 6 * m.elems * 4 + 9 * ((size_t)g.nz * 2 * g.npml + (size_t)2 * g.npml * g.nx) * 4 +
+                            (size_t)(steps / probe_period(m)) * ni * 2 + ni * 4;   // snapshots, illumination
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
         (void)hipGetLastError();
@@ -86,6 +102,17 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
 
     const int sk = c->kernel, sr = c->rz, sb = c->brows;
     const bool log = getenv("MNAD_AUTOTUNE_LOG") != nullptr;   // one stderr line per probe
+    // MNAD_AUTOTUNE_SKIP: comma-separated kernel numbers left out of the search
+    std::vector<int> skip;
+    if (const char* e = getenv("MNAD_AUTOTUNE_SKIP")) {
+        for (const char* q = e; *q;) {
+            char* end = nullptr;
+            const long k = strtol(q, &end, 10);
+            if (end == q) break;
+            skip.push_back((int)k);
+            q = (*end == ',') ? end + 1 : end;
+        }
+    }
     TunePick best{0, 1, 4, 1, 1e30, {}};
     constexpr int nv = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
     double curves[nv][8] = {};   // per variant: us per shot-step at 1, 2, 4, ... shots
@@ -94,6 +121,10 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
         double best_b = 1e30;
         for (int iv = 0; iv < nv; ++iv) {
             const Variant& v = kVariants[iv];
+            if (std::find(skip.begin(), skip.end(), v.kernel) != skip.end()) {
+                curves[iv][kb] = 1e30;
+                continue;
+            }
             c->kernel = v.kernel;
             c->rz = v.rz;
             c->brows = v.brows;

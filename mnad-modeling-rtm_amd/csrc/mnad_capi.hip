@@ -58,6 +58,7 @@ static bool tuning_ok(int kernel, int rows, int brows)
                (rows == 5 && (brows == 4 || brows == 8)) ||
                (rows == 7 && (brows == 8 || brows == 16));
     if (kernel == 9) return brows == 8 && rows >= 1 && rows <= 4;   // rows: blocks per CU
+    if (kernel == 11) return brows == 1 && rows == 2;                // two rows per wavefront
     return false;
 }
 

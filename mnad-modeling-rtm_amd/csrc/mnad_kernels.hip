@@ -791,6 +791,149 @@ __global__ __launch_bounds__(BX * BZR) void k_stepw(const ShotDev* __restrict__ 
     step_one<PF, false>(d, s, j, ti * BX, threadIdx.x);
 }
 
+// ------------------------------------------------ two rows per wavefront (kernel 11)
+// A wavefront owns 64 columns of rows j and j+1.  The rows the two updates
+// share are loaded once: U and Vs rows j-1 .. j+2, Ws rows j-2 .. j+3 (the
+// outer two at the centre column only), 14 row loads for two points against
+// 22 for two one-row waves, and the two increments' arithmetic interleaves.
+// The tail of each point (CPML memories, injections, receivers, stores,
+// snapshots / imaging) is step_one's.
+
+__device__ __forceinline__ void pair_tail(const ShotDev& d, int s, int j, int i0, unsigned lane,
+                                          const Nbh& u, const Nbh& v, const Nbh& w, float rx,
+                                          float pu, float pv, float pw, float du, float dv,
+                                          float dw, gfloat* PU, gfloat* PV, gfloat* PW)
+{
+    const int i = i0 + (int)lane;
+    const int np = d.npml;
+    const bool inx = (i < np) || (i >= d.nx - np);
+    const bool inz = (j < np) || (j >= d.nz - np);   // wave-uniform
+    int r = -1;
+    if (d.nrec > 0) {
+        const int off = __builtin_amdgcn_readfirstlane(d.rrow[j]);   // wave-uniform
+        if (off >= 0) r = uni(gp(d.rcol) + off + i0)[lane];
+    }
+    if (inx || inz) {
+        // The CPML memories see the pre-injection U (fwd_PDE_exact_MNAD.m:95-111).
+        float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
+        if (d.mode == MODE_FWD && d.src_j >= 0) {
+            const float a0 = d.src_amp[s];
+            if (j == d.src_j) {
+                if (i == d.src_i) uc -= a0;
+                if (i - 1 == d.src_i) uw -= a0;
+                if (i + 1 == d.src_i) ue -= a0;
+            }
+            if (i == d.src_i) {
+                if (j - 1 == d.src_j) un -= a0;
+                if (j + 1 == d.src_j) us -= a0;
+            }
+        }
+        if (inx) {
+            const long nsx = (long)d.nz * 2 * np, rowx = (long)j * 2 * np;
+            const unsigned bx = 4u * ((i < np) ? (unsigned)i : (unsigned)(i - (d.nx - 2 * np)));
+            PsiW mx;
+            psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
+            psi_step(mx, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+            psi_store(mx, gp(d.psx) + rowx, nsx, bx);
+        }
+        if (inz) {
+            const long nsz = (long)2 * np * d.pitch;
+            const long rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.pitch + i0;
+            PsiW mz;
+            psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
+            psi_step(mz, rx * d.kap, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+            psi_store(mz, gp(d.psz) + rowz, nsz, 4u * lane);
+        }
+    }
+    float nu = (2.0f * u.c + du) - pu;
+    const float nv = (2.0f * v.c + dv) - pv;
+    const float nw = (2.0f * w.c + dw) - pw;
+    if (d.mode == MODE_FWD) {
+        if (j == d.src_j && i == d.src_i) nu += d.src_amp[s + 1];
+        if (r >= 0) (gp(d.gather) + (long)s * d.nrec)[(unsigned)r] = u.c;
+    } else if (r >= 0) {
+        nu -= (gp(d.radj) + (long)s * d.nrec)[(unsigned)r];
+    }
+    PU[lane] = nu;
+    PV[lane] = nv;
+    PW[lane] = nw;
+    if (d.T > 0 && !inx && !inz)
+        snap_image_w(d, s, (long)(j - np) * (d.nx - 2 * np), (unsigned)(i - np), du, nu);
+}
+
+__device__ __forceinline__ void step_pair(const ShotDev& d, int s, int j, int i0, unsigned lane)
+{
+    const int i = i0 + (int)lane;
+    if (i >= d.nx) return;
+    const bool two = j + 1 < d.nz;   // wave-uniform; rows past the grid read zero ghost rows
+    const bool odd = s & 1;
+    const long P = d.pitch;
+    const long org = (long)(j + MNAD_ZO) * P + (i0 + MNAD_XO);   // storage of (j, i0)
+    const gfloat* U = gp(odd ? d.f[1][0] : d.f[0][0]) + org + lane;
+    const gfloat* V = gp(odd ? d.f[1][1] : d.f[0][1]) + org + lane;
+    const gfloat* W = gp(odd ? d.f[1][2] : d.f[0][2]) + org + lane;
+    // U rows j-1 .. j+2 and Vs rows j-1 .. j+2 at columns -1 .. +1 (Vs rows j, j+1
+    // at -2 .. +2), Ws rows j-1 .. j+2 at -1 .. +1 and rows j-2, j+3 at 0
+    const float u0w = U[-P - 1], u0c = U[-P], u0e = U[-P + 1];
+    const float u1w = U[-1], u1c = U[0], u1e = U[1];
+    const float u2w = U[P - 1], u2c = U[P], u2e = U[P + 1];
+    const float u3w = U[2 * P - 1], u3c = U[2 * P], u3e = U[2 * P + 1];
+    const float v0w = V[-P - 1], v0c = V[-P], v0e = V[-P + 1];
+    const float v1ww = V[-2], v1w = V[-1], v1c = V[0], v1e = V[1], v1ee = V[2];
+    const float v2ww = V[P - 2], v2w = V[P - 1], v2c = V[P], v2e = V[P + 1], v2ee = V[P + 2];
+    const float v3w = V[2 * P - 1], v3c = V[2 * P], v3e = V[2 * P + 1];
+    const float wm2 = W[-2 * P];
+    const float w0w = W[-P - 1], w0c = W[-P], w0e = W[-P + 1];
+    const float w1w = W[-1], w1c = W[0], w1e = W[1];
+    const float w2w = W[P - 1], w2c = W[P], w2e = W[P + 1];
+    const float w3w = W[2 * P - 1], w3c = W[2 * P], w3e = W[2 * P + 1];
+    const float wp3 = W[3 * P];
+    const gfloat* RX = gp(d.rx) + org + lane;
+    const float rxa = RX[0], rxb = RX[P];
+    gfloat* PU = gp(odd ? d.f[0][0] : d.f[1][0]) + org;
+    gfloat* PV = gp(odd ? d.f[0][1] : d.f[1][1]) + org;
+    gfloat* PW = gp(odd ? d.f[0][2] : d.f[1][2]) + org;
+    const float pua = PU[lane], pva = PV[lane], pwa = PW[lane];
+    const float pub = (PU + P)[lane], pvb = (PV + P)[lane], pwb = (PW + P)[lane];
+
+    Nbh ua, va, wa, ub, vb, wb;
+    ua.nw = u0w; ua.n = u0c; ua.ne = u0e; ua.w = u1w; ua.c = u1c; ua.e = u1e;
+    ua.sw = u2w; ua.s = u2c; ua.se = u2e;
+    ub.nw = u1w; ub.n = u1c; ub.ne = u1e; ub.w = u2w; ub.c = u2c; ub.e = u2e;
+    ub.sw = u3w; ub.s = u3c; ub.se = u3e;
+    va.nw = v0w; va.n = v0c; va.ne = v0e; va.w = v1w; va.c = v1c; va.e = v1e;
+    va.sw = v2w; va.s = v2c; va.se = v2e;
+    vb.nw = v1w; vb.n = v1c; vb.ne = v1e; vb.w = v2w; vb.c = v2c; vb.e = v2e;
+    vb.sw = v3w; vb.s = v3c; vb.se = v3e;
+    wa.nw = w0w; wa.n = w0c; wa.ne = w0e; wa.w = w1w; wa.c = w1c; wa.e = w1e;
+    wa.sw = w2w; wa.s = w2c; wa.se = w2e;
+    wb.nw = w1w; wb.n = w1c; wb.ne = w1e; wb.w = w2w; wb.c = w2c; wb.e = w2e;
+    wb.sw = w3w; wb.s = w3c; wb.se = w3e;
+    float dua, dva, dwa, dub, dvb, dwb;
+    mnad_incr(d.ic, rxa, ua, va, v1ww, v1ee, wa, wm2, w3c, dua, dva, dwa);
+    mnad_incr(d.ic, rxb, ub, vb, v2ww, v2ee, wb, w0c, wp3, dub, dvb, dwb);
+
+    pair_tail(d, s, j, i0, lane, ua, va, wa, rxa, pua, pva, pwa, dua, dva, dwa, PU, PV, PW);
+    if (two)
+        pair_tail(d, s, j + 1, i0, lane, ub, vb, wb, rxb, pub, pvb, pwb, dub, dvb, dwb, PU + P,
+                  PV + P, PW + P);
+}
+
+__global__ __launch_bounds__(BX) void k_stepw2(const ShotDev* __restrict__ shots,
+                                               const TileRef* __restrict__ tiles,
+                                               const int* __restrict__ base, int rel)
+{
+    // one exit test after one round of descriptor loads, as in k_stepw
+    const TileRef tr = tiles[blockIdx.x];
+    const ShotDev& d = shots[tr.shot < 0 ? 0 : tr.shot];
+    const int s = *base + rel - d.t0;
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int j = 2 * tj;
+    if ((tr.shot < 0) | (s < 0) | (s >= d.nt) | (j >= d.nz) | (d.rx == nullptr)) return;
+    step_pair(d, s, j, ti * BX, threadIdx.x);
+}
+
 // ------------------------------------------------ resident blocks (kernel 9)
 // One cooperative launch runs every step of a batch.  Workgroup b owns
 // blocks[b] for the whole launch: columns c0 .. c0+63 of one shot and rows
@@ -2029,6 +2172,8 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL(k_stepx<2>, g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
     else if (kernel == 6 && brows == 4 && rows == 1)
         hipLaunchKernelGGL(k_stepx<4>, g, b, 0, st, shots, tiles, base, rel);
+    else if (kernel == 11 && brows == 1 && rows == 2)
+        hipLaunchKernelGGL(k_stepw2, g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
     else if (kernel == 10 && brows == 1 && rows == 1)
         hipLaunchKernelGGL(k_stepx2<1>, g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
     else if (kernel == 10 && brows == 2 && rows == 1)

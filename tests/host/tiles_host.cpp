@@ -184,6 +184,7 @@ int main()
         {"k6_b2_c2", 6, 1, 2, 1, {c2}},
         {"k6_b4_mixed", 6, 1, 4, 1, {c1, c2, tiny, nocpml}},
         {"k10_b1_c2", 10, 1, 1, 1, {c2}},
+        {"k11_r2_mixed", 11, 2, 1, 1, {c2, c1, tiny, nocpml}},
         {"k10_b2_mixed", 10, 1, 2, 1, {c1, c2, tiny, nocpml}},
         {"k7_r5_b8_c2x3", 7, 5, 8, 1, {c2, c2, c2}},
         {"k7_r3_b16_mixed", 7, 3, 16, 1, {c1, tiny, c2}},

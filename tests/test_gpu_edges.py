@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
 KERNELS = [(4, 1, 1), (4, 1, 4), (5, 1, 4), (6, 1, 2), (7, 3, 16), (9, 1, 8), (9, 4, 8),
-           (10, 1, 1), (10, 1, 4)]
+           (10, 1, 1), (10, 1, 4), (11, 2, 1)]
 
 
 def _say(msg):
