@@ -184,6 +184,8 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  *   before creating shots;
  * kernel 10 = kernel 6 with two columns per lane in the interior (8-byte
  *   loads, 124 columns per wave, fewer registers): block_rows 1, 2 or 4, 1 row;
+ * kernel 11 = two grid rows per wavefront (the rows both updates need are
+ *   loaded once): block_rows 1, rows_per_thread 2;
  * kernel 9 = resident blocks: one cooperative launch runs every step of a
  *   batch; a workgroup of 8 wavefronts keeps a 64-column block of one shot in
  *   LDS for the whole launch and exchanges only its two-cell edges with its
