@@ -362,17 +362,19 @@ def test_pipelined_rtm_full_size_c2(mnad, ctx):
     assert np.array_equal(imgs[0][0][lit], imgs[1][0][lit])
 
 
-def test_resident_kernel_full_size_c2_rtm_bitwise(mnad):
+@pytest.mark.parametrize("kernel,rz,br", [(9, 2, 8), (11, 2, 1)])
+def test_full_size_c2_rtm_bitwise_as_default_kernel(mnad, kernel, rz, br):
     """The resident-block kernel (one cooperative launch for every step, LDS
-    blocks exchanging edge records) gives the default kernel's gathers, image
-    and illumination bitwise on a full-size C2 RTM shot (561 x 1061, nt 300:
-    two forwards in one launch, then the adjoint)."""
-    _say("resident kernel, full-size C2 RTM")
+    blocks exchanging edge records) and the two-row kernel (shared row loads)
+    evaluate every point's update exactly as the default kernel: gathers,
+    image and illumination equal bitwise on a full-size C2 RTM shot
+    (561 x 1061, nt 300: two forwards in one launch, then the adjoint)."""
+    _say(f"kernel {kernel} vs default, full-size C2 RTM")
     wl = _c2_inputs(300)
     v = wl["v"]
     g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
     out = []
-    for tun in ((4, 1, 1), (9, 2, 8)):
+    for tun in ((4, 1, 1), (kernel, rz, br)):
         ctx = mnad.Context(0)
         try:
             ctx.set_tuning(kernel=tun[0], rows_per_thread=tun[1], block_rows=tun[2])
