@@ -154,8 +154,9 @@ MNAD_API int mnad_fwd_shots(mnad_ctx *ctx, const mnad_grid *g, const double *v, 
 /* Step-kernel accounting since the last reset: GPU time of the step-kernel
  * launch sequences measured with HIP events on the context's stream (ms,
  * inter-launch gaps included), the number of step launches, and the
- * algorithmic HBM bytes those launches move (40 B per grid point, 72 B per
- * CPML strip point and direction, receivers, snapshots, image).  reset != 0
+ * algorithmic HBM bytes those launches move (40 B per grid point, 84 B per
+ * CPML strip point and direction: nine memories read and written, three
+ * damping coefficients read; receivers, snapshots, image).  reset != 0
  * clears the counters after reading. */
 MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches,
                          double *bytes);

@@ -601,13 +601,20 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
     }
 }
 
+// Unique HBM bytes a shot's nt steps must move:
+// - per padded point: U, Vs, Ws at both levels and rx read, the new level written (40 B);
+// - per CPML strip point and direction: nine memories (three fields x three levels) read
+//   and written (72 B) and the three damping coefficients c1, h*c2, c3 read (12 B);
+// - per receiver: its gather or injection value (4 B);
+// - every T steps: the snapshot plus illumination (forward) or image and compensation
+//   (adjoint) read and written.
 double algorithmic_bytes(const ShotDev& d)
 {
     const double N = (double)d.nz * d.nx, np2 = 2.0 * d.npml;
     const double strips = np2 * d.nz + np2 * d.nx;
     const double ni = (double)(d.nz - 2 * d.npml) * (d.nx - 2 * d.npml);
     const double esz = d.sfmt == SNAP_F32 ? 4.0 : 2.0;
-    double b = (double)d.nt * (40.0 * N + 72.0 * strips + 4.0 * d.nrec);
+    double b = (double)d.nt * (40.0 * N + 84.0 * strips + 4.0 * d.nrec);
     if (d.T > 0) b += (double)d.K * ni * (esz + (d.mode == MODE_FWD ? 8.0 : 16.0));
     return b;
 }
