@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -249,28 +250,106 @@ static int check_grid(const mnad_grid* g)
     return MNAD_OK;
 }
 
+// Validation of a velocity model (n >= 1 values): every value positive and
+// finite, and the range.  A positive finite double has bits in
+// [1, 0x7FEFFFFFFFFFFFFF], and on that range the order of the bit patterns is
+// the order of the values, so the pass runs on integers in independent lanes
+// that the compiler vectorises.  Returns n, or the index of the first bad value.
+static size_t scan_model(const double* v, size_t n, double* vmin, double* vmax)
+{
+    constexpr int L = 8;
+    int64_t lo[L], hi[L];
+    uint64_t bad = 0;
+    for (int i = 0; i < L; ++i) {
+        lo[i] = INT64_MAX;
+        hi[i] = 0;
+    }
+    size_t k = 0;
+    for (; k + L <= n; k += L)
+        for (int i = 0; i < L; ++i) {
+            int64_t b;
+            std::memcpy(&b, v + k + i, sizeof b);
+            bad |= (uint64_t)(b - 1) >= 0x7FEFFFFFFFFFFFFFull;
+            lo[i] = b < lo[i] ? b : lo[i];
+            hi[i] = b > hi[i] ? b : hi[i];
+        }
+    for (; k < n; ++k) {
+        int64_t b;
+        std::memcpy(&b, v + k, sizeof b);
+        bad |= (uint64_t)(b - 1) >= 0x7FEFFFFFFFFFFFFFull;
+        lo[0] = b < lo[0] ? b : lo[0];
+        hi[0] = b > hi[0] ? b : hi[0];
+    }
+    if (bad)
+        for (size_t q = 0; q < n; ++q)
+            if (!(v[q] > 0.0 && v[q] < INFINITY)) return q;
+    for (int i = 1; i < L; ++i) {
+        lo[0] = std::min(lo[0], lo[i]);
+        hi[0] = std::max(hi[0], hi[i]);
+    }
+    std::memcpy(vmin, &lo[0], sizeof *vmin);
+    std::memcpy(vmax, &hi[0], sizeof *vmax);
+    return n;
+}
+
+// 128-bit digest of a model's FP64 bit patterns, the model-cache key.  Two
+// multiply-xorshift chains (one fed by xor, one by addition), each in four
+// interleaved lanes that take two values a step as w0 + rotl(w1, 29); the
+// lanes are folded in order at the end.  Each step is a bijection of the lane
+// state for a given input and of either value for the other, so changing any
+// single value changes both halves of the digest.
+static void model_digest(const double* v, size_t n, uint64_t* da, uint64_t* db)
+{
+    constexpr int L = 4;
+    auto rotl = [](uint64_t x, int r) { return (x << r) | (x >> (64 - r)); };
+    auto step_a = [](uint64_t h, uint64_t w) {
+        h = (h ^ w) * 0x100000001B3ull;
+        return h ^ (h >> 29);
+    };
+    auto step_b = [](uint64_t h, uint64_t w) {
+        h = (h + w) * 0x9E3779B97F4A7C15ull;
+        return h ^ (h >> 31);
+    };
+    uint64_t ha[L], hb[L];
+    for (int i = 0; i < L; ++i) {
+        ha[i] = 0x243F6A8885A308D3ull ^ (0x9E3779B97F4A7C15ull * (uint64_t)(i + 1));
+        hb[i] = 0x13198A2E03707344ull + 0xC2B2AE3D27D4EB4Full * (uint64_t)(i + 1);
+    }
+    size_t k = 0;
+    for (; k + 2 * L <= n; k += 2 * L)
+        for (int i = 0; i < L; ++i) {
+            uint64_t w0, w1;
+            std::memcpy(&w0, v + k + i, sizeof w0);
+            std::memcpy(&w1, v + k + L + i, sizeof w1);
+            const uint64_t w = w0 + rotl(w1, 29);
+            ha[i] = step_a(ha[i], w);
+            hb[i] = step_b(hb[i], w);
+        }
+    for (; k < n; ++k) {
+        uint64_t w;
+        std::memcpy(&w, v + k, sizeof w);
+        ha[k % L] = step_a(ha[k % L], w);
+        hb[k % L] = step_b(hb[k % L], w);
+    }
+    for (int i = 1; i < L; ++i) {
+        ha[0] = step_a(ha[0], ha[i]);
+        hb[0] = step_b(hb[0], hb[i]);
+    }
+    *da = ha[0];
+    *db = hb[0];
+}
+
 int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m)
 {
     CHK(check_grid(g));
     if (!v) return fail(MNAD_ERR_ARG, "velocity is NULL");
     const size_t n = (size_t)g->nz * g->nx;
-    // validation and a 128-bit digest of the values (two independent
-    // multiply-xorshift chains over the FP64 bit patterns) in one pass
-    double vmin = v[0], vmax = v[0];
-    uint64_t ha = 0x243F6A8885A308D3ull, hb = 0x13198A2E03707344ull;
-    for (size_t k = 0; k < n; ++k) {
-        const double x = v[k];
-        if (!std::isfinite(x) || !(x > 0))
-            return fail(MNAD_ERR_ARG, "velocity must be finite and positive (v[%zu] = %g)", k, x);
-        vmin = std::min(vmin, x);
-        vmax = std::max(vmax, x);
-        uint64_t w;
-        std::memcpy(&w, &x, sizeof w);
-        ha = (ha ^ w) * 0x100000001B3ull;
-        ha ^= ha >> 29;
-        hb = (hb + w) * 0x9E3779B97F4A7C15ull;
-        hb ^= hb >> 31;
-    }
+    double vmin, vmax;
+    const size_t bad = scan_model(v, n, &vmin, &vmax);
+    if (bad < n)
+        return fail(MNAD_ERR_ARG, "velocity must be finite and positive (v[%zu] = %g)", bad, v[bad]);
+    uint64_t ha, hb;
+    model_digest(v, n, &ha, &hb);
     const double courant = vmax * g->dt / std::min(g->dx, g->dz);
     if (courant > 1.0)
         return fail(MNAD_ERR_ARG,
