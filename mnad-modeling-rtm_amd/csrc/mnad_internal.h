@@ -233,6 +233,11 @@ int copy_in(mnad_ctx* c, void* dst, const void* src, size_t bytes);
 // (stream idle).
 void release_batch_cache(mnad_ctx* c);
 int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m);
+// Host pass over a velocity model: the index of its first value that is not
+// positive and finite (n if none) and its range; and the 128-bit digest of
+// its values that keys the context's model cache.
+size_t scan_model(const double* v, size_t n, double* vmin, double* vmax);
+void model_digest(const double* v, size_t n, uint64_t* da, uint64_t* db);
 int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s);
 void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
 int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);

@@ -255,7 +255,7 @@ static int check_grid(const mnad_grid* g)
 // [1, 0x7FEFFFFFFFFFFFFF], and on that range the order of the bit patterns is
 // the order of the values, so the pass runs on integers in independent lanes
 // that the compiler vectorises.  Returns n, or the index of the first bad value.
-static size_t scan_model(const double* v, size_t n, double* vmin, double* vmax)
+size_t scan_model(const double* v, size_t n, double* vmin, double* vmax)
 {
     constexpr int L = 8;
     int64_t lo[L], hi[L];
@@ -298,7 +298,7 @@ static size_t scan_model(const double* v, size_t n, double* vmin, double* vmax)
 // lanes are folded in order at the end.  Each step is a bijection of the lane
 // state for a given input and of either value for the other, so changing any
 // single value changes both halves of the digest.
-static void model_digest(const double* v, size_t n, uint64_t* da, uint64_t* db)
+void model_digest(const double* v, size_t n, uint64_t* da, uint64_t* db)
 {
     constexpr int L = 4;
     auto rotl = [](uint64_t x, int r) { return (x << r) | (x >> (64 - r)); };

@@ -11,9 +11,17 @@
 //  - the estimated work per XCD differs by at most two tile weights, half a
 //    tile at either end of each XCD's range (two-step kernels: counts);
 //  - with strip_first, each XCD dispatches its CPML-strip tiles first;
-// and the block heights the resident kernel (9) picks for a few batches.
+// and the block heights the resident kernel (9) picks for a few batches;
+// and the host pass over a velocity model (scan_model, model_digest):
+//  - the first value that is not positive and finite (0, -0, negative, NaN,
+//    +inf; a subnormal is valid) is found wherever it sits, lanes and tail;
+//  - the range equals the one std::minmax_element gives;
+//  - the digest is the same for equal arrays, and a one-ulp change of any
+//    single value, or the swap of two neighbours, changes both halves.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -224,6 +232,58 @@ int main()
                    rc.bh, rc.nblocks);
             ++bad;
         }
+    }
+    {   // model scan
+        std::string why;
+        for (size_t n : {9u, 15u, 16u, 17u, 1001u}) {
+            std::vector<double> v(n);
+            for (size_t k = 0; k < n; ++k) v[k] = 1500.0 + 37.0 * (double)((k * 7919) % 101);
+            double lo = 0, hi = 0;
+            if (scan_model(v.data(), n, &lo, &hi) != n) why += " valid model rejected;";
+            const auto mm = std::minmax_element(v.begin(), v.end());
+            if (lo != *mm.first || hi != *mm.second) why += " wrong range;";
+            const double bads[] = {0.0, -0.0, -1500.0, std::numeric_limits<double>::quiet_NaN(),
+                                   std::numeric_limits<double>::infinity()};
+            for (size_t at : {(size_t)0, n / 2, n - 1})
+                for (double b : bads) {
+                    std::vector<double> w = v;
+                    w[at] = b;
+                    if (at + 1 < n) w[n - 1] = -1.0;   // a later bad value must not be reported
+                    if (scan_model(w.data(), n, &lo, &hi) != at) why += " bad value missed;";
+                }
+            std::vector<double> w = v;
+            w[n - 1] = std::numeric_limits<double>::denorm_min();
+            if (scan_model(w.data(), n, &lo, &hi) != n || lo != w[n - 1]) why += " subnormal;";
+        }
+        printf(why.empty() ? "ok %s\n" : "FAIL %s:%s\n", "model scan", why.c_str());
+        bad += !why.empty();
+    }
+    {   // model digest
+        std::string why;
+        for (size_t n : {9u, 15u, 16u, 17u, 1001u}) {
+            std::vector<double> v(n);
+            for (size_t k = 0; k < n; ++k) v[k] = 1500.0 + 37.0 * (double)((k * 7919) % 101);
+            uint64_t a0, b0, a1, b1;
+            model_digest(v.data(), n, &a0, &b0);
+            std::vector<double> c = v;
+            model_digest(c.data(), n, &a1, &b1);
+            if (a1 != a0 || b1 != b0) why += " equal arrays differ;";
+            for (size_t k = 0; k < n; ++k) {
+                c[k] = std::nextafter(v[k], 1e9);
+                model_digest(c.data(), n, &a1, &b1);
+                if (a1 == a0 || b1 == b0) why += " one-ulp change at " + std::to_string(k) + ";";
+                c[k] = v[k];
+            }
+            for (size_t k = 0; k + 1 < n; ++k) {
+                if (v[k] == v[k + 1]) continue;
+                std::swap(c[k], c[k + 1]);
+                model_digest(c.data(), n, &a1, &b1);
+                if (a1 == a0 || b1 == b0) why += " swap at " + std::to_string(k) + ";";
+                std::swap(c[k], c[k + 1]);
+            }
+        }
+        printf(why.empty() ? "ok %s\n" : "FAIL %s:%s\n", "model digest", why.substr(0, 300).c_str());
+        bad += !why.empty();
     }
     for (const Case& cs : cases) {
         const std::string r = check(cs.name, cs.kernel, cs.rz, cs.brows, cs.strip_first, cs.shapes);

@@ -3,7 +3,8 @@ launch (build_tiles in csrc/mnad_runtime.hip): tests/host/tiles_host.cpp is
 compiled against the runtime source alone, with the kernel launchers stubbed,
 and run on the host. Each case prints "ok <name>" or "FAIL <name>: <reason>";
 what it checks is listed at the top of that file (the tile tables of the
-one-step kernels and the block heights of the resident kernel)."""
+one-step kernels, the block heights of the resident kernel, and the host pass
+over a velocity model that validates it and keys the model cache)."""
 import os
 import shutil
 import subprocess
@@ -29,6 +30,6 @@ def test_tile_table_covers_balances_and_orders(tmp_path):
     assert b.returncode == 0, b.stderr[-4000:]
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     lines = r.stdout.strip().splitlines()
-    assert len(lines) == 18, r.stdout
+    assert len(lines) == 20, r.stdout
     bad = [ln for ln in lines if not ln.startswith("ok ")]
     assert not bad and r.returncode == 0, "\n".join(bad)
