@@ -344,6 +344,47 @@ def run_probe(args, mnad):
           flush=True)
 
 
+def run_compare(args, mnad):
+    """Step-kernel time per step (HIP events on the library stream, from
+    mnad_timing) of several kernel variants on forward batches: for each
+    padded grid of --grid (NZxNXxNPML, comma list), each variant of --compare
+    (KERNEL:ROWS:BLOCK_ROWS, comma list) and each shot count of --compare-shots,
+    one untimed call (set-up, graph capture) and --compare-calls timed calls of
+    fwd_shots with --nt steps, on a linear-gradient model (1500 to 3000 m/s),
+    sources at the centre of the first physical row (3 cells apart), a receiver
+    on every interior column of the next row.  --compare-calls 1
+    --compare-warm 0 makes one call per variant (for profiler runs)."""
+    dt, fc = 8e-4, 20.0
+    t = np.arange(args.nt) * dt
+    tau = np.pi * (t - 1.1 / fc) * fc
+    w = (1 - 2 * tau**2) * np.exp(-tau**2)
+    with mnad.Context(0) as ctx:
+        for gs in args.grid.split(","):
+            nz, nx, npml = (int(x) for x in gs.split("x"))
+            v = np.asfortranarray(1500.0 + 1500.0 * np.arange(nz)[:, None] / nz * np.ones((1, nx)))
+            g = mnad.Grid(nz, nx, npml, 10.0, 10.0, dt)
+            xr = np.arange(npml, nx - npml)
+            zr = np.full(xr.size, npml + 1)
+            for ks in args.compare.split(","):
+                k, r, b = (int(x) for x in ks.split(":"))
+                ctx.set_tuning(kernel=k, rows_per_thread=r, block_rows=b)
+                for ns in (int(x) for x in args.compare_shots.split(",")):
+                    xs = [nx // 2 + 3 * q for q in range(ns)]
+                    call = (g, v, w, [npml] * ns, xs, [(zr, xr)] * ns)
+                    try:
+                        for _ in range(args.compare_warm):
+                            ctx.fwd_shots(*call, batch=ns)
+                        ctx.timing(reset=True)
+                        for _ in range(args.compare_calls):
+                            ctx.fwd_shots(*call, batch=ns)
+                        ms, launches, _ = ctx.timing(reset=True)
+                        us = ms * 1e3 / (args.nt * args.compare_calls)
+                        print(f"{nz}x{nx} kernel {k},{r},{b} {ns} shot(s): {us:.2f} us per step, "
+                              f"{us / ns:.2f} us per shot-step ({launches} launches)", flush=True)
+                    except mnad.MnadError as e:
+                        print(f"{nz}x{nx} kernel {k},{r},{b} {ns} shot(s): {e}", flush=True)
+
+
 def _cpu_shot(args):
     """One oracle shot sample in a worker process: (points * steps, seconds)."""
     wl, steps = args
@@ -417,6 +458,13 @@ def main():
                     help="KERNEL,ROWS,BLOCK_ROWS[,SHOTS]: time SHOTS (default 2) forward shots "
                          "x 32 steps (twice) of the workload's grid with that step kernel "
                          "(for counter runs)")
+    ap.add_argument("--compare", default="",
+                    help="KERNEL:ROWS:BLOCK_ROWS list: step-kernel time per step of each variant "
+                         "on forward batches of the --grid grids (see run_compare)")
+    ap.add_argument("--grid", default="561x1061x30", help="--compare: padded NZxNXxNPML list")
+    ap.add_argument("--compare-shots", default="1,2", help="--compare: shots per launch list")
+    ap.add_argument("--compare-calls", type=int, default=1, help="--compare: timed calls")
+    ap.add_argument("--compare-warm", type=int, default=1, help="--compare: untimed calls")
     ap.add_argument("--tuning", default="",
                     help="C2 / C5 window: step kernel K,RZ,BR for the timed shots "
                          "(default: the library's)")
@@ -471,6 +519,11 @@ def main():
     mnad = load_package()
     if args.probe:
         run_probe(args, mnad)
+        return
+    if args.compare:
+        if args.nt == 4000:
+            args.nt = 1000
+        run_compare(args, mnad)
         return
     if args.workload == "c4":
         run_c4(args, mnad)

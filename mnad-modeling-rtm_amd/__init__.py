@@ -388,8 +388,10 @@ class Context:
         """Block until the request completes; returns (outputs, info).
         Raises MnadError with the request's status on failure/cancel."""
         info = ReqInfo()
+        info.status = 1   # mnad_wait fills it only for the call that takes the result
         rc = lib().mnad_wait(self._h, int(ticket), int(timeout_ms), C.byref(info))
-        if rc == ERR_TIMEOUT:
+        if rc == ERR_TIMEOUT or info.status == 1:
+            # still running, unknown, or taken by another waiter: the outputs stay
             raise MnadError(rc, lib().mnad_last_error().decode())
         outs = self._reqs.pop(int(ticket), None)
         _check(rc)

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "mnad.h"
@@ -88,17 +89,27 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
     // probe length: enough launches to amortise the graph chunking, fewer on big grids
     const double pts = (double)g.nz * g.nx;
     const int steps = pts > 4e6 ? 2 * c->graph_chunk : 4 * c->graph_chunk;
-    // shots per launch: powers of two up to max_shots, bounded by memory
+    // shots per launch: powers of two up to max_shots, bounded by memory.  A
+    // probe shot holds 3 nlev field planes and 9 (one-step kernels) or 18
+    // (two-step kernels: two sets) CPML memory planes of its strips, plus its
+    // snapshots and illumination.
     const size_t ni = (size_t)std::max(0, g.nz - 2 * g.npml) * std::max(0, g.nx - 2 * g.npml);
-    const size_t per_shot = 6 * m.elems * 4 + 9 * ((size_t)g.nz * 2 * g.npml + (size_t)2 * g.npml * g.nx) * 4 +
-                            (size_t)(steps / probe_period(m)) * ni * 2 + ni * 4;   // snapshots, illumination
+    const size_t strip = (size_t)g.nz * 2 * g.npml + (size_t)2 * g.npml * m.pitch;
+    auto per_shot = [&](int kernel) {
+        const size_t nlev = two_step_kernel(kernel) ? 4 : 2;
+        return 3 * nlev * m.elems * 4 + 9 * (nlev / 2) * strip * 4 +
+               (size_t)(steps / probe_period(m)) * ni * 2 + ni * 4 + 4096;
+    };
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
         (void)hipGetLastError();
-        freeb = per_shot;
+        freeb = 0;
     }
-    const int mem_cap = (int)std::max<size_t>(1, (size_t)(0.5 * (double)(freeb + c->pool.cached)) / per_shot);
-    max_shots = std::max(1, std::min(max_shots, mem_cap));
+    const double avail = 0.5 * (double)(freeb + c->pool.cached);
+    auto mem_cap = [&](int kernel) {
+        return (int)std::max<size_t>(1, (size_t)(avail / (double)per_shot(kernel)));
+    };
+    max_shots = std::max(1, std::min(max_shots, mem_cap(4)));
 
     const int sk = c->kernel, sr = c->rz, sb = c->brows;
     const bool log = getenv("MNAD_AUTOTUNE_LOG") != nullptr;   // one stderr line per probe
@@ -117,14 +128,15 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
     constexpr int nv = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
     double curves[nv][8] = {};   // per variant: us per shot-step at 1, 2, 4, ... shots
     int kbest = 0;
+    int ran = 0, first_rc = MNAD_OK;
+    std::string first_err;
     for (int b = 1, kb = 0; b <= max_shots && kb < 8; b *= 2, ++kb) {
         double best_b = 1e30;
         for (int iv = 0; iv < nv; ++iv) {
             const Variant& v = kVariants[iv];
-            if (std::find(skip.begin(), skip.end(), v.kernel) != skip.end()) {
-                curves[iv][kb] = 1e30;
-                continue;
-            }
+            curves[iv][kb] = 1e30;
+            if (std::find(skip.begin(), skip.end(), v.kernel) != skip.end()) continue;
+            if (b > mem_cap(v.kernel)) continue;   // this variant's shots do not fit
             c->kernel = v.kernel;
             c->rz = v.rz;
             c->brows = v.brows;
@@ -132,11 +144,17 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
             int rc = probe(c, m, b, steps, &us);   // warm-up: first touch, code load
             if (rc == MNAD_OK) rc = probe(c, m, b, steps, &us);
             if (rc != MNAD_OK) {
-                c->kernel = sk;
-                c->rz = sr;
-                c->brows = sb;
-                return rc;
+                // a variant that cannot run here (memory, launch limits) is left out
+                if (first_rc == MNAD_OK) {
+                    first_rc = rc;
+                    first_err = last_error();
+                }
+                if (log)
+                    fprintf(stderr, "mnad_autotune %dx%d: kernel %d rows %d block_rows %d, %d shots: "
+                            "skipped (%s)\n", g.nz, g.nx, v.kernel, v.rz, v.brows, b, last_error());
+                continue;
             }
+            ++ran;
             if (log)
                 fprintf(stderr, "mnad_autotune %dx%d: kernel %d rows %d block_rows %d, %d shots: %.3f us/shot-step\n",
                         g.nz, g.nx, v.kernel, v.rz, v.brows, b, us);
@@ -153,7 +171,11 @@ int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out)
     c->kernel = sk;
     c->rz = sr;
     c->brows = sb;
-    for (int k = 0; k < 8; ++k) best.curve[k] = curves[kbest][k];
+    if (ran == 0)
+        return fail(first_rc == MNAD_OK ? MNAD_ERR_ARG : first_rc,
+                    "autotune %dx%d: no step-kernel variant ran (%s)", g.nz, g.nx,
+                    first_rc == MNAD_OK ? "all skipped" : first_err.c_str());
+    for (int k = 0; k < 8; ++k) best.curve[k] = curves[kbest][k] < 1e29 ? curves[kbest][k] : 0.0;
     *out = best;
     c->tuned[std::make_tuple(g.nz, g.nx, g.npml)] = best;
     return MNAD_OK;

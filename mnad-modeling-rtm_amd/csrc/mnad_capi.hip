@@ -268,12 +268,21 @@ int mnad_born_adj(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, in
                     ss, k.nz, k.nx, k.npml, k.nt, k.T, k.K, g->nz, g->nx, g->npml, nt, sp.T, sp.K);
     Shot s;
     CHK(make_shot(ctx, m, sp, s));
+    // The kept source wavefield is lent to the adjoint and released only once
+    // the image is out: a failed adjoint leaves it in place for a retry, as the
+    // reference's UU_<ss>.mat survives a failed Born_adj_MNAD.
     attach_snaps(s, std::move(k.snaps), k.fmt, k.sscale);
+    int rc = fill_radj_host(ctx, s, nt, nr, data);
+    if (rc == MNAD_OK) rc = run_batch(ctx, std::vector<Shot*>{&s}, flags);
+    if (rc == MNAD_OK)
+        rc = download_interior(ctx, *g, s.img.as<float>(), s.imgc.as<float>(), (double)sp.T, Im);
+    if (rc != MNAD_OK) {
+        (void)hipStreamSynchronize(ctx->stream);   // nothing queued still reads it
+        k.snaps = std::move(s.snaps);
+        s.d.snaps = nullptr;
+        return rc;
+    }
     ctx->kept.erase(it);
-    CHK(fill_radj_host(ctx, s, nt, nr, data));
-    std::vector<Shot*> b{&s};
-    CHK(run_batch(ctx, b, flags));
-    CHK(download_interior(ctx, *g, s.img.as<float>(), s.imgc.as<float>(), (double)sp.T, Im));
     return MNAD_OK;
 }
 
@@ -313,8 +322,10 @@ int mnad_snap_save(mnad_ctx* ctx, int ss, const char* path)
     std::vector<char> buf(std::min<size_t>(bytes, (size_t)256 << 20));
     for (size_t off = 0; ok && off < bytes; off += buf.size()) {
         const size_t n = std::min(buf.size(), bytes - off);
-        if (hipMemcpy(buf.data(), static_cast<const char*>(k.snaps.p) + off, n,
-                      hipMemcpyDeviceToHost) != hipSuccess) {
+        // on the context stream, behind every kernel that may still write the block
+        if (hipMemcpyAsync(buf.data(), static_cast<const char*>(k.snaps.p) + off, n,
+                           hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
             fclose(f);
             return fail(MNAD_ERR_HIP, "copy of snapshots to the host failed");
         }
@@ -356,8 +367,11 @@ int mnad_snap_load(mnad_ctx* ctx, int ss, const char* path)
             fclose(f);
             return fail(MNAD_ERR_ARG, "%s is truncated", path);
         }
-        if (hipMemcpy(static_cast<char*>(k.snaps.p) + off, buf.data(), n, hipMemcpyHostToDevice) !=
-            hipSuccess) {
+        // on the context stream: the pool hands blocks back in stream order, so a
+        // recycled block may still be read by a kernel queued there
+        if (hipMemcpyAsync(static_cast<char*>(k.snaps.p) + off, buf.data(), n,
+                           hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
             fclose(f);
             return fail(MNAD_ERR_HIP, "copy of snapshots to the device failed");
         }

@@ -142,7 +142,11 @@ constexpr int kDefaultKernel = 4, kDefaultRows = 1, kDefaultBrows = 1;
 
 struct mnad_ctx {
     int device = 0;
-    mnad::Server* server = nullptr;   // serving loop (mnad_serve_start)
+    // serving loop (mnad_serve_start).  Every serving call takes its own
+    // reference under srv_mu, so the server and its requests outlive a
+    // concurrent mnad_serve_stop until that call returns.
+    std::shared_ptr<mnad::Server> server;
+    std::mutex srv_mu;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
     mnad::Pool pool;

@@ -269,14 +269,14 @@ size_t scan_model(const double* v, size_t n, double* vmin, double* vmax)
         for (int i = 0; i < L; ++i) {
             int64_t b;
             std::memcpy(&b, v + k + i, sizeof b);
-            bad |= (uint64_t)(b - 1) >= 0x7FEFFFFFFFFFFFFFull;
+            bad |= ((uint64_t)b - 1u) >= 0x7FEFFFFFFFFFFFFFull;
             lo[i] = b < lo[i] ? b : lo[i];
             hi[i] = b > hi[i] ? b : hi[i];
         }
     for (; k < n; ++k) {
         int64_t b;
         std::memcpy(&b, v + k, sizeof b);
-        bad |= (uint64_t)(b - 1) >= 0x7FEFFFFFFFFFFFFFull;
+        bad |= ((uint64_t)b - 1u) >= 0x7FEFFFFFFFFFFFFFull;
         lo[0] = b < lo[0] ? b : lo[0];
         hi[0] = b > hi[0] ? b : hi[0];
     }

@@ -56,6 +56,7 @@ struct Request {
     int rc = MNAD_OK;
     std::string err;
     bool cancel = false;
+    bool claimed = false;   // a mnad_wait has taken the result
     double t_submit = 0, t_admit = 0, t_done = 0;
     size_t bytes = 0;
     size_t points = 0;   // grid points stepping while the request is active
@@ -82,7 +83,8 @@ struct Server {
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queued;
     std::vector<Request*> active;
-    std::map<int64_t, std::unique_ptr<Request>> all;
+    // every request not yet waited for; a waiter holds its own reference
+    std::map<int64_t, std::shared_ptr<Request>> all;
     size_t budget = 0, reserved = 0;
     size_t max_points = 0, active_points = 0;
     int max_active = 64, chunk = 32;
@@ -392,9 +394,24 @@ void Server::run()
     cv_done.notify_all();
 }
 
+// The context's server, with a reference that keeps it alive for the caller.
+static std::shared_ptr<Server> get_server(mnad_ctx* ctx)
+{
+    std::lock_guard<std::mutex> lk(ctx->srv_mu);
+    return ctx->server;
+}
+
+// Stops the loop: new calls no longer find the server, the thread cancels
+// everything still queued or active and wakes every waiter, and the device
+// side goes away.  The Server object itself (with the requests not yet waited
+// for) is freed when the last caller still inside mnad_wait & co. returns.
 void server_destroy(mnad_ctx* ctx)
 {
-    Server* s = ctx->server;
+    std::shared_ptr<Server> s;
+    {
+        std::lock_guard<std::mutex> lk(ctx->srv_mu);
+        s.swap(ctx->server);
+    }
     if (!s) return;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -405,10 +422,8 @@ void server_destroy(mnad_ctx* ctx)
     s->dsh.reset();
     s->dtab.reset();
     s->dbase.reset();
-    s->all.clear();
     mnad_ctx_destroy(s->c);
-    delete s;
-    ctx->server = nullptr;
+    s->c = nullptr;
 }
 
 }  // namespace mnad
@@ -420,13 +435,10 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
 {
     if (!ctx) return fail(MNAD_ERR_ARG, "context is NULL");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    if (ctx->server) return fail(MNAD_ERR_STATE, "the server is already running");
-    auto* s = new Server();
+    if (get_server(ctx)) return fail(MNAD_ERR_STATE, "the server is already running");
+    auto s = std::make_shared<Server>();
     int rc = mnad_ctx_create(ctx->device, &s->c);
-    if (rc != MNAD_OK) {
-        delete s;
-        return rc;
-    }
+    if (rc != MNAD_OK) return rc;
     // the server steps one time level per launch (admission at any step), so
     // the two-step and the whole-batch kernels fall back to the default
     const bool other = two_step_kernel(ctx->kernel) || resident_kernel(ctx->kernel);
@@ -448,11 +460,12 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
     if (rc != MNAD_OK) {
         s->dbase.reset();
         mnad_ctx_destroy(s->c);
-        delete s;
         return rc;
     }
-    ctx->server = s;
-    s->th = std::thread([s] { s->run(); });
+    Server* raw = s.get();
+    s->th = std::thread([raw] { raw->run(); });
+    std::lock_guard<std::mutex> slk(ctx->srv_mu);
+    ctx->server = std::move(s);
     return MNAD_OK;
 }
 
@@ -467,7 +480,7 @@ extern "C" int mnad_serve_stop(mnad_ctx* ctx)
 extern "C" int mnad_submit(mnad_ctx* ctx, const mnad_request* q, int64_t* ticket)
 {
     if (!ctx || !q || !ticket) return fail(MNAD_ERR_ARG, "NULL argument");
-    Server* s = ctx->server;
+    std::shared_ptr<Server> s = get_server(ctx);
     if (!s) return fail(MNAD_ERR_STATE, "the server is not running (mnad_serve_start)");
     if (q->kind != MNAD_REQ_MODEL && q->kind != MNAD_REQ_RTM)
         return fail(MNAD_ERR_ARG, "unknown request kind %d", q->kind);
@@ -502,7 +515,7 @@ extern "C" int mnad_submit(mnad_ctx* ctx, const mnad_request* q, int64_t* ticket
     for (int r = 0; r < q->nr; ++r)
         if (q->zr[r] < 0 || q->zr[r] >= g.nz || q->xr[r] < 0 || q->xr[r] >= g.nx)
             return fail(MNAD_ERR_ARG, "receiver %d outside the grid", r);
-    auto r = std::make_unique<Request>();
+    auto r = std::make_shared<Request>();
     r->spec = *q;
     r->v.assign(q->v, q->v + n);
     if (q->kind == MNAD_REQ_RTM) r->vs.assign(q->v_smooth, q->v_smooth + n);
@@ -526,6 +539,7 @@ extern "C" int mnad_submit(mnad_ctx* ctx, const mnad_request* q, int64_t* ticket
     r->bytes = request_bytes(r->spec);
     r->points = (size_t)g.nz * g.nx * (q->kind == MNAD_REQ_RTM ? 2 : 1);
     std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return fail(MNAD_ERR_STATE, "the server is stopping");
     if (r->bytes > s->budget)
         return fail(MNAD_ERR_BUDGET, "request needs %zu bytes, the HBM budget is %zu", r->bytes,
                     s->budget);
@@ -541,18 +555,24 @@ extern "C" int mnad_submit(mnad_ctx* ctx, const mnad_request* q, int64_t* ticket
 extern "C" int mnad_wait(mnad_ctx* ctx, int64_t ticket, int timeout_ms, mnad_req_info* info)
 {
     if (!ctx) return fail(MNAD_ERR_ARG, "context is NULL");
-    Server* s = ctx->server;
+    std::shared_ptr<Server> s = get_server(ctx);
     if (!s) return fail(MNAD_ERR_STATE, "the server is not running");
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->all.find(ticket);
     if (it == s->all.end()) return fail(MNAD_ERR_STATE, "unknown ticket %lld", (long long)ticket);
-    Request* r = it->second.get();
-    auto ready = [&] { return r->state == DONE; };
+    std::shared_ptr<Request> r = it->second;   // stays valid whatever another waiter does
+    // a stopping server marks every request done before it wakes the waiters
+    auto ready = [&] { return r->state == DONE || r->claimed; };
     if (timeout_ms < 0) {
         s->cv_done.wait(lk, ready);
     } else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
         return fail(MNAD_ERR_TIMEOUT, "ticket %lld still running", (long long)ticket);
     }
+    if (r->claimed)
+        return fail(MNAD_ERR_STATE, "ticket %lld was already waited for", (long long)ticket);
+    r->claimed = true;
+    s->all.erase(ticket);
+    s->cv_done.notify_all();   // other waiters on this ticket return MNAD_ERR_STATE
     if (info) {
         info->status = r->rc;
         info->submit_s = r->t_submit;
@@ -562,7 +582,6 @@ extern "C" int mnad_wait(mnad_ctx* ctx, int64_t ticket, int timeout_ms, mnad_req
     }
     const int rc = r->rc;
     std::string err = r->err;
-    s->all.erase(it);
     if (rc != MNAD_OK) return fail(rc, "ticket %lld: %s", (long long)ticket, err.c_str());
     return MNAD_OK;
 }
@@ -570,7 +589,7 @@ extern "C" int mnad_wait(mnad_ctx* ctx, int64_t ticket, int timeout_ms, mnad_req
 extern "C" int mnad_cancel(mnad_ctx* ctx, int64_t ticket)
 {
     if (!ctx) return fail(MNAD_ERR_ARG, "context is NULL");
-    Server* s = ctx->server;
+    std::shared_ptr<Server> s = get_server(ctx);
     if (!s) return fail(MNAD_ERR_STATE, "the server is not running");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->all.find(ticket);
@@ -594,7 +613,7 @@ extern "C" int mnad_serve_status(mnad_ctx* ctx, int* queued, int* active, uint64
                                  uint64_t* budget, int64_t* steps)
 {
     if (!ctx) return fail(MNAD_ERR_ARG, "context is NULL");
-    Server* s = ctx->server;
+    std::shared_ptr<Server> s = get_server(ctx);
     if (!s) return fail(MNAD_ERR_STATE, "the server is not running");
     std::lock_guard<std::mutex> lk(s->mu);
     if (queued) *queued = (int)s->queued.size();

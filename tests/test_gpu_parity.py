@@ -575,3 +575,31 @@ def test_full_size_mirror_symmetry(mnad, ctx):
     r = ctx.fwd(g, v, wl["w"], wl["zs"], xc, zr, xr, want_state=False)
     G = r["gather"]
     assert rel_l2(G[:, ::-1], G) < 1e-6
+
+
+def test_failed_adjoint_keeps_the_source_wavefield(mnad, ctx):
+    """A Born_adj_MNAD call that fails after the kept snapshots were handed to
+    it (here: scattered data with a NaN, rejected on the host) leaves them in
+    place, as the reference's ./Wavefields/UU_<ss>.mat survives a failed
+    adjoint; the retry with valid data gives bitwise the image of a call that
+    never failed."""
+    _say("failed adjoint keeps the kept snapshots")
+    f = golden("rtm_rect")
+    g = _grid(mnad, f, "vs")
+    w, zs, xs, zr, xr = f["w"], int(f["zs0"]), int(f["xs0"]), f["zr0"], f["xr0"]
+    obs = ctx.fwd(g, f["v"], w, zs, xs, zr, xr, want_state=False)["gather"]
+    syn = ctx.fwd(g, f["vs"], w, zs, xs, zr, xr, ss=9, indx=1, snap_fmt=mnad.SNAP_F16,
+                  want_state=False)
+    data = obs - syn["gather"]
+    ref = ctx.born_adj(g, f["vs"], data, zr, xr, ss=9)
+    ctx.fwd(g, f["vs"], w, zs, xs, zr, xr, ss=9, indx=1, snap_fmt=mnad.SNAP_F16, want_state=False)
+    bad = data.copy()
+    bad[bad.shape[0] // 2, 0] = np.nan
+    with pytest.raises(mnad.MnadError) as ei:
+        ctx.born_adj(g, f["vs"], bad, zr, xr, ss=9)
+    assert ei.value.code == mnad.ERR_ARG
+    got = ctx.born_adj(g, f["vs"], data, zr, xr, ss=9)
+    assert np.array_equal(got, ref)
+    with pytest.raises(mnad.MnadError) as ei:   # consumed by the successful call
+        ctx.born_adj(g, f["vs"], data, zr, xr, ss=9)
+    assert ei.value.code == mnad.ERR_STATE

@@ -182,3 +182,51 @@ def test_server_on_whole_batch_or_two_step_tuning(mnad, kernel, rz, br):
             ctx.close()
     assert rel_l2(out[0], f["gather"]) < 1e-4
     assert np.array_equal(out[0], out[1])
+
+
+def test_stop_and_double_wait_while_waiting(mnad):
+    """Waiters blocked in mnad_wait while mnad_serve_stop runs wake with the
+    request's final status (done, or cancelled by the stop) and never touch
+    freed server state; two waiters on one ticket get the result exactly
+    once, the other MNAD_ERR_STATE."""
+    import threading
+    _say("serving: stop while waiting, double wait")
+    ctx = mnad.Context(0)
+    try:
+        ctx.serve_start(hbm_budget=0, max_active=4, chunk_steps=16, max_points=0)
+        f, r = _model_req(mnad, "fwd_rect")
+        # two waiters on one ticket
+        t = _submit(ctx, r)
+        got = []
+
+        def wait_one(tk):
+            try:
+                out, info = ctx.wait(tk)
+                got.append(("ok", out))
+            except mnad.MnadError as e:
+                got.append(("err", e.code))
+
+        ths = [threading.Thread(target=wait_one, args=(t,)) for _ in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join(60)
+        assert sorted(k for k, _ in got) == ["err", "ok"], got
+        assert [c for k, c in got if k == "err"] == [mnad.ERR_STATE]
+        out = [o for k, o in got if k == "ok"][0]
+        assert rel_l2(out["gather"], f["gather"]) < 1e-4
+        # a queue longer than max_active, a waiter on the last ticket, then stop
+        tickets = [_submit(ctx, r) for _ in range(12)]
+        got.clear()
+        th = threading.Thread(target=wait_one, args=(tickets[-1],))
+        th.start()
+        time.sleep(0.05)
+        ctx.serve_stop()
+        th.join(60)
+        assert not th.is_alive()
+        assert got and (got[0][0] == "ok" or got[0][1] == mnad.ERR_CANCELLED), got
+        with pytest.raises(mnad.MnadError) as ei:
+            ctx.wait(tickets[0])
+        assert ei.value.code == mnad.ERR_STATE   # no server any more
+    finally:
+        ctx.close()

@@ -16,13 +16,21 @@ from conftest import ROOT
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-def test_tile_table_covers_balances_and_orders(tmp_path):
+# The host code also runs under the undefined-behaviour sanitizer (host pass
+# only; no GPU code is built with it): signed overflow in the model scan,
+# shifts and out-of-range conversions abort the run.
+UBSAN = ["-Xarch_host", "-fsanitize=undefined", "-Xarch_host", "-fno-sanitize-recover=undefined"]
+
+
+@pytest.mark.parametrize("san", ["plain", "ubsan"])
+def test_tile_table_covers_balances_and_orders(tmp_path, san):
     if not (os.path.exists(HIPCC) or shutil.which(HIPCC)):
         pytest.skip("hipcc not found")
     exe = str(tmp_path / "tiles_host")
     csrc = os.path.join(ROOT, "mnad-modeling-rtm_amd", "csrc")
     cmd = [HIPCC, "-O1", "-std=c++17", "--offload-arch=gfx950",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           *(UBSAN if san == "ubsan" else []),
            "-I", os.path.join(ROOT, "include"), "-I", csrc,
            os.path.join(ROOT, "tests", "host", "tiles_host.cpp"),
            os.path.join(csrc, "mnad_runtime.hip"), "-o", exe]
