@@ -46,10 +46,19 @@ extern "C" {
 #define MNAD_ERR_NOGPU (-7)     /* no HIP device */
 #define MNAD_ERR_TIMEOUT (-8)   /* wait timed out (request still running) */
 
-/* Storage format of the saved source-side d2u snapshots. */
+/* Storage of the source wavefield an RTM forward keeps for its adjoint.
+ * F32 / F16 / BF16: the d2u snapshots of fwd_PDE_exact_MNAD.m:126-143 every T
+ * steps, as the reference's UU (K x interior points).
+ * RING_F16 / RING_BF16: the forward's last two levels plus (U, dU/dx, dU/dz)
+ * on a ring of two cells around the interior every step, coded as 16-bit
+ * residuals of a closed-loop predictor; the adjoint recomputes the source
+ * wavefield backwards inside the CPML and takes d2u from it.  About
+ * 12 (nx + nz) bytes per step instead of 2 nx nz / T. */
 #define MNAD_SNAP_F32 0
 #define MNAD_SNAP_F16 1   /* FP16 with a per-shot power-of-two scale */
 #define MNAD_SNAP_BF16 2
+#define MNAD_SNAP_RING_F16 3
+#define MNAD_SNAP_RING_BF16 4
 
 /* Flags */
 #define MNAD_FLAG_NO_GRAPH 1   /* plain launches, one step per launch */

@@ -32,11 +32,14 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_STATE = -1, -2, -3, -4
 ERR_CANCELLED, ERR_BUDGET, ERR_NOGPU, ERR_TIMEOUT = -5, -6, -7, -8
 SNAP_F32, SNAP_F16, SNAP_BF16 = 0, 1, 2
+# saved source-wavefield boundary ring + reverse reconstruction (16-bit codes)
+SNAP_RING_F16, SNAP_RING_BF16 = 3, 4
 FLAG_NO_GRAPH = 1
 FLAG_SERIAL = 2
 FLAG_AUTOTUNE = 4
 REQ_MODEL, REQ_RTM = 0, 1
-SNAP_FORMATS = {"f32": SNAP_F32, "f16": SNAP_F16, "bf16": SNAP_BF16}
+SNAP_FORMATS = {"f32": SNAP_F32, "f16": SNAP_F16, "bf16": SNAP_BF16,
+                "ring_f16": SNAP_RING_F16, "ring_bf16": SNAP_RING_BF16}
 
 # Every symbol include/mnad.h declares (checked by the CPU test suite).
 EXPORTS = (

@@ -107,6 +107,7 @@ int mnad_ctx_create(int device, mnad_ctx** out)
     if (const char* s = getenv("MNAD_STRIP_WEIGHT")) c->strip_weight = std::max(1.0, atof(s));
     if (const char* s = getenv("MNAD_X4_WEIGHT")) c->x4_weight = std::max(1.0, atof(s));
     if (const char* s = getenv("MNAD_AUTOTUNE")) c->autotune_on_use = atoi(s) != 0;
+    if (const char* s = getenv("MNAD_RING_ORDER")) c->ring_order = std::max(2, std::min(4, atoi(s)));
     *out = c;
     return MNAD_OK;
 }
@@ -194,8 +195,7 @@ int mnad_fwd(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, const d
 {
     ENTER(ctx);
     if (indx != 0 && indx != 1) return fail(MNAD_ERR_ARG, "indx must be 0 or 1");
-    if (snap_fmt < MNAD_SNAP_F32 || snap_fmt > MNAD_SNAP_BF16)
-        return fail(MNAD_ERR_ARG, "unknown snapshot format %d", snap_fmt);
+    if (!snap_format_ok(snap_fmt)) return fail(MNAD_ERR_ARG, "unknown snapshot format %d", snap_fmt);
     if (nr < 0 || (nr > 0 && (!zr || !xr))) return fail(MNAD_ERR_ARG, "bad receiver arrays");
     Model m;
     CHK(make_model(ctx, g, v, m));
@@ -208,11 +208,17 @@ int mnad_fwd(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, const d
     sp.nr = nr;
     sp.zr = zr;
     sp.xr = xr;
+    const bool ring = indx == 1 && ring_format(snap_fmt);
     if (indx == 1) {
         CHK(schedule(nt, m.vmin, m.vmax, nullptr, &sp.T, &sp.K, &sp.ncol));
         sp.snap_fmt = snap_fmt;
+        sp.ring_order = ctx->ring_order;
     }
-    Shot s;
+    // the ring code runs on the one-step kernels 4, 5, 6, 10, 11
+    TunePick ring_pick{kDefaultKernel, kDefaultRows, kDefaultBrows, 1, 0.0, {}};
+    TuningGuard tg(ctx, ring && !ring_kernel(ctx->kernel) ? &ring_pick : nullptr);
+    auto sp_shot = std::make_shared<Shot>();
+    Shot& s = *sp_shot;
     CHK(make_shot(ctx, m, sp, s));
     std::vector<Shot*> b{&s};
     CHK(run_batch(ctx, b, flags));
@@ -233,6 +239,19 @@ int mnad_fwd(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, const d
         k.T = sp.T;
         k.K = sp.K;
         k.ncol = sp.ncol;
+        k.dx = g->dx;
+        k.dz = g->dz;
+        k.dt = g->dt;
+        if (ring) {   // keep the forward's last two levels, source and ring only
+            s.psx.reset();
+            s.psz.reset();
+            s.gather.reset();
+            s.rrow.reset();
+            s.rcol.reset();
+            s.illum.reset();
+            for (int q = 6; q < 12; ++q) s.state[q].reset();
+            k.fwd = sp_shot;
+        }
         ctx->kept[ss] = std::move(k);
     }
     return MNAD_OK;
@@ -266,6 +285,49 @@ int mnad_born_adj(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, in
                     "shot %d: kept snapshots (grid %dx%d, npml %d, nt %d, T %d, K %d) do not "
                     "match this call (grid %dx%d, npml %d, nt %d, T %d, K %d)",
                     ss, k.nz, k.nx, k.npml, k.nt, k.T, k.K, g->nz, g->nx, g->npml, nt, sp.T, sp.K);
+    if (k.fwd) {   // ring storage: reverse reconstruction beside the adjoint
+        TunePick ring_pick{kDefaultKernel, kDefaultRows, kDefaultBrows, 1, 0.0, {}};
+        TuningGuard tg(ctx, ring_kernel(ctx->kernel) ? nullptr : &ring_pick);
+        Shot s;
+        CHK(make_shot(ctx, m, sp, s));
+        Shot& f = *k.fwd;
+        f.d.rx = m.rx();   // the migration model of this call (the forward's, by contract)
+        // The reconstruction runs the kept levels and ring state backwards; a
+        // copy taken first restores them if the adjoint fails, so the source
+        // wavefield survives for a retry as the reference's UU_<ss>.mat does.
+        DevBuf bk[7];
+        for (int q = 0; q < 6; ++q) {
+            CHK(dalloc(ctx, bk[q], f.state[q].n, false));
+            HIPCHK(hipMemcpyAsync(bk[q].p, f.state[q].p, f.state[q].n, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        }
+        CHK(dalloc(ctx, bk[6], f.rst.n, false));
+        HIPCHK(hipMemcpyAsync(bk[6].p, f.rst.p, f.rst.n, hipMemcpyDeviceToDevice, ctx->stream));
+        Shot rec;
+        int rc = make_rec_shot(ctx, f, s, rec);
+        if (rc == MNAD_OK) rc = fill_radj_host(ctx, s, nt, nr, data);
+        if (rc == MNAD_OK) rc = run_batch(ctx, std::vector<Shot*>{&s, &rec}, flags);
+        if (rc == MNAD_OK)
+            rc = download_interior(ctx, *g, s.img.as<float>(), s.imgc.as<float>(), (double)sp.T, Im);
+        if (rc != MNAD_OK) {
+            (void)hipStreamSynchronize(ctx->stream);
+            for (int q = 0; q < 6; ++q)
+                if (rec.state[q].p) f.state[q] = std::move(rec.state[q]);
+            if (rec.amp.p) f.amp = std::move(rec.amp);
+            if (rec.rq.p) f.rq = std::move(rec.rq);
+            if (rec.rst.p) f.rst = std::move(rec.rst);
+            f.d.rq = f.rq.as<unsigned short>();
+            f.d.rst = f.rst.as<long long>();
+            for (int q = 0; q < 6; ++q)
+                (void)hipMemcpyAsync(f.state[q].p, bk[q].p, f.state[q].n, hipMemcpyDeviceToDevice,
+                                     ctx->stream);
+            (void)hipMemcpyAsync(f.rst.p, bk[6].p, f.rst.n, hipMemcpyDeviceToDevice, ctx->stream);
+            (void)hipStreamSynchronize(ctx->stream);
+            return rc;
+        }
+        ctx->kept.erase(it);
+        return MNAD_OK;
+    }
     Shot s;
     CHK(make_shot(ctx, m, sp, s));
     // The kept source wavefield is lent to the adjoint and released only once
@@ -301,6 +363,120 @@ struct SnapHeader {
     int32_t pad;
     uint64_t bytes;
 };
+
+// Ring storage: the forward's last two levels (6 field planes), source
+// amplitudes, ring codes and predictor state.
+struct RingHeader {
+    char magic[8];   // "MNADRNG1"
+    int32_t nz, nx, npml, nt, T, K, ncol, fmt;
+    int32_t rord, nring, rfmt, src_j, src_i, pitch, pad0, pad1;
+    double dx, dz, dt, rsc, rcs;
+    uint64_t plane_bytes, amp_bytes, rq_bytes, rst_bytes;
+};
+
+// Device <-> file through a bounded host buffer, on the context stream.
+int write_dev(mnad_ctx* ctx, FILE* f, const void* p, size_t bytes)
+{
+    std::vector<char> buf(std::min<size_t>(bytes, (size_t)256 << 20));
+    for (size_t off = 0; off < bytes; off += buf.size()) {
+        const size_t n = std::min(buf.size(), bytes - off);
+        if (hipMemcpyAsync(buf.data(), static_cast<const char*>(p) + off, n, hipMemcpyDeviceToHost,
+                           ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            return fail(MNAD_ERR_HIP, "copy of the source wavefield to the host failed");
+        if (fwrite(buf.data(), 1, n, f) != n) return fail(MNAD_ERR_ARG, "file write failed");
+    }
+    return MNAD_OK;
+}
+
+int read_dev(mnad_ctx* ctx, FILE* f, void* p, size_t bytes, const char* path)
+{
+    std::vector<char> buf(std::min<size_t>(bytes, (size_t)256 << 20));
+    for (size_t off = 0; off < bytes; off += buf.size()) {
+        const size_t n = std::min(buf.size(), bytes - off);
+        if (fread(buf.data(), 1, n, f) != n) return fail(MNAD_ERR_ARG, "%s is truncated", path);
+        if (hipMemcpyAsync(static_cast<char*>(p) + off, buf.data(), n, hipMemcpyHostToDevice,
+                           ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            return fail(MNAD_ERR_HIP, "copy of the source wavefield to the device failed");
+    }
+    return MNAD_OK;
+}
+
+int ring_save(mnad_ctx* ctx, const Kept& k, const char* path)
+{
+    const Shot& s = *k.fwd;
+    RingHeader h{};
+    memcpy(h.magic, "MNADRNG1", 8);
+    h.nz = k.nz; h.nx = k.nx; h.npml = k.npml; h.nt = k.nt; h.T = k.T; h.K = k.K;
+    h.ncol = k.ncol; h.fmt = k.fmt;
+    h.rord = s.d.rord; h.nring = s.d.nring; h.rfmt = s.d.rfmt;
+    h.src_j = s.d.src_j; h.src_i = s.d.src_i; h.pitch = s.d.pitch;
+    h.dx = k.dx; h.dz = k.dz; h.dt = k.dt; h.rsc = s.d.rsc; h.rcs = s.d.rcs;
+    h.plane_bytes = s.state[0].n; h.amp_bytes = s.amp.n; h.rq_bytes = s.rq.n; h.rst_bytes = s.rst.n;
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(MNAD_ERR_ARG, "cannot open %s for writing", path);
+    int rc = fwrite(&h, sizeof(h), 1, f) == 1 ? MNAD_OK : fail(MNAD_ERR_ARG, "write to %s failed", path);
+    for (int q = 0; q < 6 && rc == MNAD_OK; ++q) rc = write_dev(ctx, f, s.state[q].p, s.state[q].n);
+    if (rc == MNAD_OK) rc = write_dev(ctx, f, s.amp.p, s.amp.n);
+    if (rc == MNAD_OK) rc = write_dev(ctx, f, s.rq.p, s.rq.n);
+    if (rc == MNAD_OK) rc = write_dev(ctx, f, s.rst.p, s.rst.n);
+    if (fclose(f) != 0 && rc == MNAD_OK) rc = fail(MNAD_ERR_ARG, "write to %s failed", path);
+    return rc;
+}
+
+int ring_load(mnad_ctx* ctx, FILE* f, const char* path, Kept& k)
+{
+    RingHeader h{};
+    memcpy(h.magic, "MNADRNG1", 8);
+    if (fread((char*)&h + 8, sizeof(h) - 8, 1, f) != 1)
+        return fail(MNAD_ERR_ARG, "%s is not a snapshot file", path);
+    mnad_grid g{h.nz, h.nx, h.npml, h.dx, h.dz, h.dt};
+    const int pitch = ((MNAD_XO + h.nx + 2) + 31) / 32 * 32;
+    const size_t plane = ((size_t)(h.nz + 2 * MNAD_ZO + MNAD_ZPAD) * pitch * 4 + 255) & ~(size_t)255;
+    const size_t nr = (h.npml >= 2 && h.nz > 2 * h.npml && h.nx > 2 * h.npml)
+                          ? (size_t)ring_cells(h.nz, h.nx, h.npml) : 0;
+    auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    if (!ring_format(h.fmt) || h.nt < 1 || h.rord < 2 || h.rord > RING_MAX_ORDER || nr == 0 ||
+        (int)nr != h.nring || h.pitch != pitch || h.plane_bytes != plane ||
+        h.amp_bytes != r256((size_t)(h.nt + 1) * 4) || h.rq_bytes != r256((size_t)h.nt * 3 * nr * 2) ||
+        h.rst_bytes != r256((size_t)h.rord * 3 * nr * 8) || h.T < 1 || h.K < 1 ||
+        !(h.rsc > 0) || !(h.rcs > 0))
+        return fail(MNAD_ERR_ARG, "%s: inconsistent header", path);
+    auto s = std::make_shared<Shot>();
+    base_desc(g, pitch, h.nt, MODE_FWD, 2, s->d);
+    ShotDev& d = s->d;
+    for (int q = 0; q < 6; ++q) {
+        CHK(dalloc(ctx, s->state[q], plane, false));
+        CHK(read_dev(ctx, f, s->state[q].p, plane, path));
+        d.f[q / 3][q % 3] = s->state[q].as<float>();
+    }
+    CHK(dalloc(ctx, s->amp, h.amp_bytes, false));
+    CHK(read_dev(ctx, f, s->amp.p, h.amp_bytes, path));
+    CHK(dalloc(ctx, s->rq, h.rq_bytes, false));
+    CHK(read_dev(ctx, f, s->rq.p, h.rq_bytes, path));
+    CHK(dalloc(ctx, s->rst, h.rst_bytes, false));
+    CHK(read_dev(ctx, f, s->rst.p, h.rst_bytes, path));
+    d.src_j = h.src_j;
+    d.src_i = h.src_i;
+    d.src_amp = s->amp.as<float>();
+    d.T = h.T;
+    d.K = h.K;
+    d.shift = h.ncol - h.K;
+    d.rord = h.rord;
+    d.nring = h.nring;
+    d.rfmt = h.rfmt;
+    d.rq = s->rq.as<unsigned short>();
+    d.rst = s->rst.as<long long>();
+    d.rsc = h.rsc;
+    d.rsi = 1.0 / h.rsc;
+    d.rcs = h.rcs;
+    d.rci = 1.0 / h.rcs;
+    k.fmt = h.fmt; k.nz = h.nz; k.nx = h.nx; k.npml = h.npml; k.nt = h.nt; k.T = h.T;
+    k.K = h.K; k.ncol = h.ncol; k.dx = h.dx; k.dz = h.dz; k.dt = h.dt;
+    k.fwd = s;
+    return MNAD_OK;
+}
 }  // namespace
 
 int mnad_snap_save(mnad_ctx* ctx, int ss, const char* path)
@@ -310,6 +486,7 @@ int mnad_snap_save(mnad_ctx* ctx, int ss, const char* path)
     auto it = ctx->kept.find(ss);
     if (it == ctx->kept.end()) return fail(MNAD_ERR_STATE, "no snapshots kept for shot %d", ss);
     const Kept& k = it->second;
+    if (k.fwd) return ring_save(ctx, k, path);
     const size_t ni = (size_t)(k.nz - 2 * k.npml) * (k.nx - 2 * k.npml);
     const size_t bytes = (size_t)k.K * ni * (k.fmt == MNAD_SNAP_F32 ? 4 : 2);
     SnapHeader h{};
@@ -342,8 +519,22 @@ int mnad_snap_load(mnad_ctx* ctx, int ss, const char* path)
     if (!path) return fail(MNAD_ERR_ARG, "path is NULL");
     FILE* f = fopen(path, "rb");
     if (!f) return fail(MNAD_ERR_STATE, "cannot open %s", path);
+    char magic[8];
+    if (fread(magic, 8, 1, f) != 1) {
+        fclose(f);
+        return fail(MNAD_ERR_ARG, "%s is not a snapshot file", path);
+    }
+    if (memcmp(magic, "MNADRNG1", 8) == 0) {
+        Kept k;
+        const int rc = ring_load(ctx, f, path, k);
+        fclose(f);
+        if (rc != MNAD_OK) return rc;
+        ctx->kept[ss] = std::move(k);
+        return MNAD_OK;
+    }
     SnapHeader h{};
-    if (fread(&h, sizeof(h), 1, f) != 1 || memcmp(h.magic, "MNADSNP1", 8) != 0 || h.K < 0 ||
+    memcpy(h.magic, magic, 8);
+    if (fread((char*)&h + 8, sizeof(h) - 8, 1, f) != 1 || memcmp(h.magic, "MNADSNP1", 8) != 0 || h.K < 0 ||
         h.fmt < MNAD_SNAP_F32 || h.fmt > MNAD_SNAP_BF16 || h.nz <= 2 * h.npml || h.nx <= 2 * h.npml) {
         fclose(f);
         return fail(MNAD_ERR_ARG, "%s is not a snapshot file", path);
@@ -484,8 +675,10 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
 {
     ENTER(ctx);
     CHK(check_shots(sh));
-    if (snap_fmt < MNAD_SNAP_F32 || snap_fmt > MNAD_SNAP_BF16)
-        return fail(MNAD_ERR_ARG, "unknown snapshot format %d", snap_fmt);
+    if (!snap_format_ok(snap_fmt)) return fail(MNAD_ERR_ARG, "unknown snapshot format %d", snap_fmt);
+    const bool ring = ring_format(snap_fmt);
+    if (ring && g && g->npml < 2)
+        return fail(MNAD_ERR_ARG, "ring storage needs npml >= 2 (npml = %d)", g->npml);
     auto sit = ctx->stacks.find(stack_id);
     if (sit == ctx->stacks.end()) return fail(MNAD_ERR_STATE, "unknown stack %d", stack_id);
     Stack& st = sit->second;
@@ -505,8 +698,14 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
     // runs the group's 2B forwards, the next its B adjoints.  Pipelined: one
     // sequence runs the group's forwards together with the previous group's
     // adjoints (3B propagations), so two groups' snapshots are alive at once.
-    const size_t per_ord = 3 * shot_bytes(mt, nt, maxnr) + (size_t)K * ni * esz + 4 * ni * 4;
-    const size_t per_pip = per_ord + (size_t)K * ni * esz;
+    // Ring storage: the adjoints' launches also run the B reverse
+    // reconstructions, which keep the forwards' fields, ring and a slot.
+    const size_t keep = ring ? (size_t)nt * 3 * ring_cells(g->nz, g->nx, g->npml) * 2 +
+                                   (size_t)ctx->ring_order * 3 * ring_cells(g->nz, g->nx, g->npml) * 8 +
+                                   ni * 4
+                             : (size_t)K * ni * esz;
+    const size_t per_ord = 3 * shot_bytes(mt, nt, maxnr) + keep + 4 * ni * 4;
+    const size_t per_pip = per_ord + keep;
     const TunePick* tp = nullptr;
     if (!serial) {
         tp = find_tuned(ctx, *g);
@@ -516,7 +715,13 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
             tp = find_tuned(ctx, *g);
         }
     }
+    // the ring code and the reverse reconstruction run on the one-step kernels
+    // 4, 5, 6, 10, 11; a tuned pick of another family gives way to kernel 4
+    TunePick ring_pick{kDefaultKernel, kDefaultRows, kDefaultBrows, 1, 0.0, {}};
+    if (ring && tp && !ring_kernel(tp->kernel)) tp = nullptr;
+    if (ring && !tp && !ring_kernel(ctx->kernel)) tp = &ring_pick;
     TuningGuard tg(ctx, tp);
+    if (tp == &ring_pick) tp = nullptr;   // no time-per-batch curve for the schedule
     bool pipelined = !serial;
     if (!serial && batch == 0 && tp) {
         // the tuned kernel's time per shot-step against shots per launch:
@@ -547,6 +752,7 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
     struct Group {
         int k0 = 0, nb = 0;
         std::vector<Shot> fw, ad;   // fw[2b]: observed (true model), fw[2b+1]: background
+        std::vector<Shot> rc;       // ring storage: reverse reconstructions of fw[2b+1]
     };
     // forward shots of shots [k0, k0 + nb)
     auto forwards = [&](Group& gr) -> int {
@@ -567,19 +773,21 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
             sp.K = K;
             sp.ncol = ncol;
             sp.snap_fmt = snap_fmt;
-            CHK(make_shot(ctx, ms, sp, gr.fw[2 * b + 1]));      // background + snapshots
+            sp.ring_order = ctx->ring_order;
+            CHK(make_shot(ctx, ms, sp, gr.fw[2 * b + 1]));      // background + snapshots / ring
         }
         return MNAD_OK;
     };
     // after the forwards ran: gathers out, d_obs - d_syn and the adjoint shots
     auto adjoints = [&](Group& gr) -> int {
         gr.ad.resize(gr.nb);
+        gr.rc.resize(ring ? gr.nb : 0);
         for (int b = 0; b < gr.nb; ++b) {
             const int k = gr.k0 + b, r0 = sh->rec_off[k], nr = sh->rec_off[k + 1] - r0;
             if (dobs) CHK(download_gather(ctx, gr.fw[2 * b], nt, nr, dobs + (size_t)r0 * nt));
             if (dsyn) CHK(download_gather(ctx, gr.fw[2 * b + 1], nt, nr, dsyn + (size_t)r0 * nt));
             gr.fw[2 * b].release_state();
-            gr.fw[2 * b + 1].release_state();
+            if (!ring) gr.fw[2 * b + 1].release_state();   // ring: the reconstruction starts there
             ShotSpec sp;
             sp.mode = MODE_ADJ;
             sp.nt = nt;
@@ -595,8 +803,11 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
                                              gr.fw[2 * b + 1].gather.as<float>(),
                                              gr.ad[b].arec.as<float>(), nt, gr.ad[b].nu,
                                              gr.ad[b].radj.as<float>(), ctx->stream));
-            attach_snaps(gr.ad[b], std::move(gr.fw[2 * b + 1].snaps), snap_fmt,
-                         gr.fw[2 * b + 1].d.sscale);
+            if (ring)
+                CHK(make_rec_shot(ctx, gr.fw[2 * b + 1], gr.ad[b], gr.rc[b]));
+            else
+                attach_snaps(gr.ad[b], std::move(gr.fw[2 * b + 1].snaps), snap_fmt,
+                             gr.fw[2 * b + 1].d.sscale);
         }
         return MNAD_OK;
     };
@@ -627,10 +838,15 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
             }
             CHK(adjoints(gr));
             run.clear();
-            for (Shot& s1 : gr.ad) run.push_back(&s1);
-            if (serial) {
-                for (Shot* s1 : run) CHK(run_batch(ctx, std::vector<Shot*>{s1}, flags));
+            if (serial) {   // an adjoint and its reconstruction share their launches
+                for (int b = 0; b < gr.nb; ++b) {
+                    std::vector<Shot*> one{&gr.ad[b]};
+                    if (ring) one.push_back(&gr.rc[b]);
+                    CHK(run_batch(ctx, one, flags));
+                }
             } else {
+                for (Shot& s1 : gr.ad) run.push_back(&s1);
+                for (Shot& s1 : gr.rc) run.push_back(&s1);
                 CHK(run_batch(ctx, run, flags));
             }
             CHK(stack(gr));
@@ -651,6 +867,7 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
         if (prev) {
             CHK(adjoints(*prev));
             for (Shot& s1 : prev->ad) run.push_back(&s1);
+            for (Shot& s1 : prev->rc) run.push_back(&s1);
         }
         CHK(run_batch(ctx, run, flags));
         if (prev) CHK(stack(*prev));

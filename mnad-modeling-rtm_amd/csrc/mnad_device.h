@@ -12,6 +12,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "mnad_ring.h"
 #include "mnad_stencil.h"
 
 #define MNAD_ZO 2    // ghost rows above the grid
@@ -20,8 +21,14 @@
 
 namespace mnad {
 
-enum SnapFmt : int { SNAP_F32 = 0, SNAP_F16 = 1, SNAP_BF16 = 2 };
-enum Mode : int { MODE_FWD = 0, MODE_ADJ = 1 };
+// Source-side d2u storage of a shot.  F32 / F16 / BF16: K snapshots kept
+// (forward writes, adjoint reads); SLOT: one FP32 snapshot handed from a
+// reverse-reconstruction shot to its adjoint in the same launch sequence.
+enum SnapFmt : int { SNAP_F32 = 0, SNAP_F16 = 1, SNAP_BF16 = 2, SNAP_SLOT = 3 };
+// FWD: forward modelling; ADJ: receiver back-propagation with imaging; REC:
+// reverse reconstruction of a forward's source wavefield inside the CPML
+// (from its last two levels and its saved boundary ring, mnad_ring.h).
+enum Mode : int { MODE_FWD = 0, MODE_ADJ = 1, MODE_REC = 2 };
 
 // Everything one shot needs inside the step kernel.  Lives in device memory;
 // one entry per shot of a batch.
@@ -67,6 +74,18 @@ struct ShotDev {
     float* illum;                   // [nzi * nxi]
     float* img;                     // [nzi * nxi] Kahan sum
     float* imgc;                    // [nzi * nxi] Kahan compensation
+    // Saved boundary ring (mnad_ring.h).  FWD with rord > 0 codes the new
+    // level of every ring cell; REC decodes it back, one level per step.
+    // REC step s works on the forward's levels: current ntf - 1 - s, the
+    // next one (read as "previous"), and writes level ntf - 2 - s.
+    int rord;                       // predictor order P (2..4); 0: no ring
+    int nring;                      // ring cells per field
+    int rfmt;                       // code format: SNAP_F16 or SNAP_BF16
+    int ntf;                        // REC: the forward's number of steps
+    unsigned short* rq;             // [ntf][3][nring] codes; level L at row L - 1
+    long long* rst;                 // [P][3][nring] decoded values, level L in slot L % P
+    double rsc, rsi;                // fixed-point unit 2^-E: rsc = 2^E, rsi = 2^-E
+    double rcs, rci;                // code scale (a power of two) and its inverse
 };
 
 static_assert(offsetof(ShotDev, rcol) + sizeof(void*) <= 192, "hot descriptor fields");

@@ -57,12 +57,18 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
-// Source wavefield kept between mnad_fwd(indx = 1) and mnad_born_adj.
+struct Shot;
+
+// Source wavefield kept between mnad_fwd(indx = 1) and mnad_born_adj: the
+// d2u snapshots, or (ring formats) the forward shot itself with its last two
+// levels, source amplitudes and saved boundary ring.
 struct Kept {
     DevBuf snaps;
     int fmt = 0;
     float sscale = 1.0f;
     int nz = 0, nx = 0, npml = 0, nt = 0, T = 0, K = 0, ncol = 0;
+    double dx = 0, dz = 0, dt = 0;
+    std::shared_ptr<Shot> fwd;
 };
 
 // Autotuner result for one grid shape.
@@ -163,6 +169,7 @@ struct mnad_ctx {
     int rz = mnad::kDefaultRows;       // rows per thread
     int brows = mnad::kDefaultBrows;   // thread rows per block (tile = 64 x brows*rz)
     int strip_first = 1;        // dispatch CPML-strip tiles first
+    int ring_order = 3;         // predictor order of the saved boundary ring (2..4)
     // XCD balance of one-step kernels: estimated work of a CPML-strip tile and
     // of a four-column tile, in interior one-point tiles
     double strip_weight = 2.0;
@@ -209,13 +216,20 @@ struct ShotSpec {
     const int* zr = nullptr;
     const int* xr = nullptr;
     int T = 0, K = 0, ncol = 0;  // snapshots (forward) / imaging (adjoint); T = 0: off
-    int snap_fmt = MNAD_SNAP_F32;
+    int snap_fmt = MNAD_SNAP_F32;   // forward: MNAD_SNAP_RING_* saves the boundary ring
+    int ring_order = 3;             // predictor order of the ring code (2..4)
 };
+
+inline bool ring_format(int fmt) { return fmt == MNAD_SNAP_RING_F16 || fmt == MNAD_SNAP_RING_BF16; }
+inline bool snap_format_ok(int fmt) { return fmt >= MNAD_SNAP_F32 && fmt <= MNAD_SNAP_RING_BF16; }
 
 struct Shot {
     ShotDev d{};
-    long long t0 = 0;                           // server: global step of local step 0
+    // global step of local step 0: the server's step counter, or the launch
+    // of a run_batch sequence at which the shot starts
+    long long t0 = 0;
     DevBuf state[12], psx, psz, amp, rrow, rcol, gather, radj, arec, snaps, illum, img, imgc;
+    DevBuf rq, rst;                             // saved boundary ring: codes, predictor state
     std::vector<int> uid;                       // receiver r -> unique point u
     std::vector<std::pair<int, int>> upts;      // unique points (j, i)
     std::vector<double> arec_h;                 // (v dt)^2 at unique points
@@ -243,6 +257,19 @@ int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m);
 size_t scan_model(const double* v, size_t n, double* vmin, double* vmax);
 void model_digest(const double* v, size_t n, uint64_t* da, uint64_t* db);
 int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s);
+// Grid part of a shot descriptor (no buffers).
+void base_desc(const mnad_grid& g, int pitch, int nt, int mode, int nlev, ShotDev& d);
+// Reverse reconstruction of a finished ring-saving forward `fwd` (its last
+// two levels, ring, source) as a REC shot paired with the adjoint `adj` of
+// the same shot: REC hands d2u to `adj` through a one-snapshot FP32 slot,
+// and their launch origins are set so that it does so one launch before the
+// adjoint's imaging step reads it.  Takes over fwd's state and ring.
+int make_rec_shot(mnad_ctx* c, Shot& fwd, Shot& adj, Shot& rec);
+// Kernels that implement the ring code and the reverse reconstruction.
+inline bool ring_kernel(int kernel)
+{
+    return kernel == 4 || kernel == 5 || kernel == 6 || kernel == 10 || kernel == 11;
+}
 void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
 int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);
 int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags);

@@ -603,7 +603,14 @@ __device__ __forceinline__ void snap_image_w(const ShotDev& d, int s, long qrow,
     // hipcc compiled the bfloat16 store path so that the illumination update
     // after it read its column offset from a register no path had written.
     const long ni = (long)(d.nz - 2 * d.npml) * (d.nx - 2 * d.npml);
-    const bool f32 = d.sfmt == SNAP_F32, f16 = d.sfmt == SNAP_F16;
+    // the one-slot hand-off is FP32 like the F32 snapshots
+    const bool f32 = d.sfmt == SNAP_F32 || d.sfmt == SNAP_SLOT, f16 = d.sfmt == SNAP_F16;
+    if (d.mode == MODE_REC) {
+        // d2u of the forward's step ntf - s (1-based); on a snapshot step it
+        // goes to the slot, which the adjoint reads one launch later
+        if ((d.ntf - s) % d.T == 0) (gp(static_cast<float*>(d.snaps)) + qrow)[q] = du * d.inv_dt2;
+        return;
+    }
     if (d.mode == MODE_FWD) {
         if ((s + 1) % d.T == 0) {
             // d2u = (nU - 2U + pU)/dt^2 = du/dt^2; snapshot k = (s+1)/T - 1.
@@ -611,12 +618,14 @@ __device__ __forceinline__ void snap_image_w(const ShotDev& d, int s, long qrow,
             const float d2u = du * d.inv_dt2;
             gfloat* il = gp(d.illum) + qrow;
             il[q] += d2u * d2u;
-            if (f32) {
-                (gp(static_cast<float*>(d.snaps)) + k0)[q] = d2u;
-            } else {
-                const unsigned short h16 = __half_as_ushort(__float2half(d2u * d.sscale));
-                const unsigned short b16 = __bfloat16_as_ushort(__float2bfloat16(d2u));
-                ((gu16*)d.snaps + k0)[q] = f16 ? h16 : b16;
+            if (d.snaps) {   // a forward that saves its boundary ring keeps no snapshots
+                if (f32) {
+                    (gp(static_cast<float*>(d.snaps)) + k0)[q] = d2u;
+                } else {
+                    const unsigned short h16 = __half_as_ushort(__float2half(d2u * d.sscale));
+                    const unsigned short b16 = __bfloat16_as_ushort(__float2bfloat16(d2u));
+                    ((gu16*)d.snaps + k0)[q] = f16 ? h16 : b16;
+                }
             }
         }
     } else {
@@ -625,7 +634,7 @@ __device__ __forceinline__ void snap_image_w(const ShotDev& d, int s, long qrow,
             // UU(:, ncol - ttt + 1) = snapshot column shift + tt/T (1-based).
             const int col = d.shift + tt / d.T;
             if (col <= d.K) {
-                const long k0 = (long)(col - 1) * ni + qrow;
+                const long k0 = (d.sfmt == SNAP_SLOT ? 0L : (long)(col - 1) * ni) + qrow;
                 float sn;
                 if (f32) {
                     sn = (gp(static_cast<const float*>(d.snaps)) + k0)[q];
@@ -645,6 +654,147 @@ __device__ __forceinline__ void snap_image_w(const ShotDev& d, int s, long qrow,
             }
         }
     }
+}
+
+// ------------------------------------------------ saved boundary ring
+typedef __attribute__((address_space(1))) long long gi64;
+
+// Forward, ring cell ri: code level L of (U, Vs, Ws) (mnad_ring.h).
+template <int P>
+__device__ __forceinline__ void ring_put_p(const ShotDev& d, int L, unsigned ri, float nu,
+                                           float nv, float nw)
+{
+    const long nr = d.nring;
+    gi64* st = (gi64*)d.rst;
+    gu16* rq = (gu16*)d.rq;
+    const float x[3] = {nu, nv, nw};
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const long cell = f * nr + ri;
+        int64_t g[RING_MAX_ORDER] = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 1; i <= P; ++i) g[i - 1] = st[(long)(((L - i) % P + P) % P) * 3 * nr + cell];
+        int64_t gL;
+        const unsigned short code = ring_encode(P, g, x[f], d.rsc, d.rcs, d.rci, d.rfmt, &gL);
+        st[(long)(L % P) * 3 * nr + cell] = gL;
+        rq[(long)(L - 1) * 3 * nr + cell] = code;
+    }
+}
+
+__device__ __forceinline__ void ring_put(const ShotDev& d, int L, unsigned ri, float nu, float nv,
+                                         float nw)
+{
+    if (d.rord == 2) ring_put_p<2>(d, L, ri, nu, nv, nw);
+    else if (d.rord == 3) ring_put_p<3>(d, L, ri, nu, nv, nw);
+    else ring_put_p<4>(d, L, ri, nu, nv, nw);
+}
+
+// Reverse, ring cell ri: with the decoded window G^(L) .. G^(L-P+1) and the
+// code of level L, decode G^(L-P) into the slot of G^(L) and return level
+// L - 2 (the level the reverse step writes).
+template <int P>
+__device__ __forceinline__ void ring_get_p(const ShotDev& d, int L, unsigned ri, float* out)
+{
+    const long nr = d.nring;
+    gi64* st = (gi64*)d.rst;
+    const gu16* rq = (const gu16*)d.rq;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const long cell = f * nr + ri;
+        const long top = (long)(L % P) * 3 * nr + cell;
+        const int64_t gL = st[top];
+        int64_t g[RING_MAX_ORDER] = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 1; i < P; ++i) g[i - 1] = st[(long)(((L - i) % P + P) % P) * 3 * nr + cell];
+        const int64_t back = ring_decode(P, gL, g, rq[(long)(L - 1) * 3 * nr + cell], d.rfmt, d.rci);
+        st[top] = back;
+        out[f] = (float)((double)(P == 2 ? back : g[1]) * d.rsi);
+    }
+}
+
+__device__ __forceinline__ void ring_get(const ShotDev& d, int L, unsigned ri, float* out)
+{
+    if (d.rord == 2) ring_get_p<2>(d, L, ri, out);
+    else if (d.rord == 3) ring_get_p<3>(d, L, ri, out);
+    else ring_get_p<4>(d, L, ri, out);
+}
+
+// Neighbourhoods of U, Vs, Ws around (j, i0 + lane): row origins U, V, W.
+__device__ __forceinline__ void load_nbh(const gfloat* U, const gfloat* V, const gfloat* W, long P,
+                                         unsigned lane, Nbh& u, Nbh& v, Nbh& w, float& vww,
+                                         float& vee, float& wnn, float& wss)
+{
+    const gfloat* a = (U - P) + lane;
+    u.nw = a[-1]; u.n = a[0]; u.ne = a[1];
+    a = U + lane;
+    u.w = a[-1];  u.c = a[0]; u.e = a[1];
+    a = (U + P) + lane;
+    u.sw = a[-1]; u.s = a[0]; u.se = a[1];
+    a = (V - P) + lane;
+    v.nw = a[-1]; v.n = a[0]; v.ne = a[1];
+    a = V + lane;
+    vww = a[-2];  v.w = a[-1]; v.c = a[0]; v.e = a[1]; vee = a[2];
+    a = (V + P) + lane;
+    v.sw = a[-1]; v.s = a[0]; v.se = a[1];
+    wnn = ((W - 2 * P) + lane)[0];
+    a = (W - P) + lane;
+    w.nw = a[-1]; w.n = a[0]; w.ne = a[1];
+    a = W + lane;
+    w.w = a[-1];  w.c = a[0]; w.e = a[1];
+    a = (W + P) + lane;
+    w.sw = a[-1]; w.s = a[0]; w.se = a[1];
+    wss = ((W + 2 * P) + lane)[0];
+}
+
+// Reverse reconstruction (MODE_REC), one point per lane of row j: step s
+// turns levels c = ntf-1-s (current) and c+1 into level c-1, written over
+// c+1.  Inside the reconstruction region R the update runs backwards,
+//     F(c-1) = 2 F(c) + incr(F(c)) - F(c+1)   (+ a_src w at the source,
+// the forward's injection into level c+1 taken back out); R holds no CPML
+// memories.  Ring cells decode level c-1 from the saved ring; every other
+// point is left alone (R never reads it).  On the forward's snapshot steps
+// d2u = incr/dt^2 goes to the adjoint's slot.  SKIPX4: points of the
+// multi-column region belong to step_xn.
+template <bool SKIPX4>
+__device__ __forceinline__ void rec_one(const ShotDev& d, int s, int j, int i0, unsigned lane)
+{
+    const int i = i0 + (int)lane;
+    if (i >= d.nx) return;
+    const int np = d.npml;
+    const int ri = ring_index(j, i, d.nz, d.nx, np);
+    if (ri < 0) {
+        if (!in_rec_region(j, i, d.nz, d.nx, np)) return;
+        if (SKIPX4 && i < d.x4hi) return;
+    }
+    const int c = d.ntf - 1 - s;
+    const bool odd = c & 1;
+    const long P = d.pitch;
+    const long org = (long)(j + MNAD_ZO) * P + (i0 + MNAD_XO);
+    gfloat* NU = gp(odd ? d.f[0][0] : d.f[1][0]) + org;
+    gfloat* NV = gp(odd ? d.f[0][1] : d.f[1][1]) + org;
+    gfloat* NW = gp(odd ? d.f[0][2] : d.f[1][2]) + org;
+    if (ri >= 0) {
+        float o[3];
+        ring_get(d, c + 1, (unsigned)ri, o);
+        NU[lane] = o[0];
+        NV[lane] = o[1];
+        NW[lane] = o[2];
+        return;
+    }
+    Nbh u, v, w;
+    float vww, vee, wnn, wss;
+    load_nbh(gp(odd ? d.f[1][0] : d.f[0][0]) + org, gp(odd ? d.f[1][1] : d.f[0][1]) + org,
+             gp(odd ? d.f[1][2] : d.f[0][2]) + org, P, lane, u, v, w, vww, vee, wnn, wss);
+    const float rx = (gp(d.rx) + org)[lane];
+    const float pu = NU[lane], pv = NV[lane], pw = NW[lane];
+    float du, dv, dw;
+    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    float nu = (2.0f * u.c + du) - pu;
+    if (j == d.src_j && i == d.src_i) nu += d.src_amp[c + 1];
+    NU[lane] = nu;
+    NV[lane] = (2.0f * v.c + dv) - pv;
+    NW[lane] = (2.0f * w.c + dw) - pw;
+    if (d.T > 0) snap_image_w(d, s, (long)(j - np) * (d.nx - 2 * np), (unsigned)(i - np), du, nu);
 }
 
 // One point per lane: the wave owns columns i0 .. i0+63 of row j.
@@ -767,6 +917,10 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
     PU[lane] = nu;
     PV[lane] = nv;
     PW[lane] = nw;
+    if (d.rord > 0 && (inx || inz)) {   // ring cells lie on the CPML strips
+        const int ri = ring_index(j, i, d.nz, d.nx, np);
+        if (ri >= 0) ring_put(d, s + 1, (unsigned)ri, nu, nv, nw);
+    }
 
     if (d.T > 0 && !inx && !inz)
         snap_image_w(d, s, (long)(j - np) * (d.nx - 2 * np), (unsigned)(i - np), du, nu);
@@ -788,7 +942,8 @@ __global__ __launch_bounds__(BX * BZR) void k_stepw(const ShotDev* __restrict__ 
     const int ti = tr.tile - tj * d.ntx;
     const int j = tj * BZR + __builtin_amdgcn_readfirstlane((int)threadIdx.y);   // wave row
     if ((tr.shot < 0) | (s < 0) | (s >= d.nt) | (j >= d.nz) | (d.rx == nullptr)) return;
-    step_one<PF, false>(d, s, j, ti * BX, threadIdx.x);
+    if (d.mode == MODE_REC) rec_one<false>(d, s, j, ti * BX, threadIdx.x);
+    else step_one<PF, false>(d, s, j, ti * BX, threadIdx.x);
 }
 
 // ------------------------------------------------ two rows per wavefront (kernel 11)
@@ -857,6 +1012,10 @@ __device__ __forceinline__ void pair_tail(const ShotDev& d, int s, int j, int i0
     PU[lane] = nu;
     PV[lane] = nv;
     PW[lane] = nw;
+    if (d.rord > 0 && (inx || inz)) {
+        const int ri = ring_index(j, i, d.nz, d.nx, np);
+        if (ri >= 0) ring_put(d, s + 1, (unsigned)ri, nu, nv, nw);
+    }
     if (d.T > 0 && !inx && !inz)
         snap_image_w(d, s, (long)(j - np) * (d.nx - 2 * np), (unsigned)(i - np), du, nu);
 }
@@ -919,7 +1078,7 @@ __device__ __forceinline__ void step_pair(const ShotDev& d, int s, int j, int i0
                   PV + P, PW + P);
 }
 
-__global__ __launch_bounds__(BX) void k_stepw2(const ShotDev* __restrict__ shots,
+__global__ __launch_bounds__(BX) __attribute__((amdgpu_waves_per_eu(6))) void k_stepw2(const ShotDev* __restrict__ shots,
                                                const TileRef* __restrict__ tiles,
                                                const int* __restrict__ base, int rel)
 {
@@ -931,6 +1090,11 @@ __global__ __launch_bounds__(BX) void k_stepw2(const ShotDev* __restrict__ shots
     const int ti = tr.tile - tj * d.ntx;
     const int j = 2 * tj;
     if ((tr.shot < 0) | (s < 0) | (s >= d.nt) | (j >= d.nz) | (d.rx == nullptr)) return;
+    if (d.mode == MODE_REC) {   // reverse reconstruction: one row at a time
+        rec_one<false>(d, s, j, ti * BX, threadIdx.x);
+        if (j + 1 < d.nz) rec_one<false>(d, s, j + 1, ti * BX, threadIdx.x);
+        return;
+    }
     step_pair(d, s, j, ti * BX, threadIdx.x);
 }
 
@@ -1399,7 +1563,10 @@ __device__ __forceinline__ void step_xn(const ShotDev& d, int s, int j, int i0, 
 {
     typedef typename VecN<NC>::T VT;
     typedef typename VecN<NC>::G GVT;
-    const int cb = s & 1;
+    // current level: s, or ntf-1-s in a reverse reconstruction (rec_one)
+    const bool rec = d.mode == MODE_REC;
+    const int lev = rec ? d.ntf - 1 - s : s;
+    const int cb = lev & 1;
     const long P = d.pitch;
     const long org = (long)(j + MNAD_ZO) * P + (i0 + MNAD_XO);   // storage of (j, i0)
     const gfloat* U = gp(d.f[cb][0]) + org;
@@ -1455,10 +1622,10 @@ __device__ __forceinline__ void step_xn(const ShotDev& d, int s, int j, int i0, 
     // owned columns: [lo, hi) of this tile, lanes 1 .. 62
     const int c0 = i0 + NC * (int)lane;
     if (lane < 1 || lane > BX - 2 || c0 + NC - 1 < lo || c0 >= hi) return;
-    if (d.mode == MODE_FWD && j == d.src_j) {
+    if ((d.mode == MODE_FWD || rec) && j == d.src_j) {
 #pragma unroll
         for (int e = 0; e < NC; ++e)
-            if (c0 + e == d.src_i) nu[e] += d.src_amp[s + 1];
+            if (c0 + e == d.src_i) nu[e] += d.src_amp[lev + 1];
     }
     if (d.nrec > 0) {
         const int off = __builtin_amdgcn_readfirstlane(d.rrow[j]);   // wave-uniform
@@ -1533,7 +1700,8 @@ __global__ __launch_bounds__(BX * BZR) void k_stepx(const ShotDev* __restrict__ 
         const int ti = tr.tile - tj * d.ntx;
         const int j = tj * BZR + wy;
         if (j >= d.nz) return;
-        step_one<false, true>(d, s, j, ti * BX, threadIdx.x);
+        if (d.mode == MODE_REC) rec_one<true>(d, s, j, ti * BX, threadIdx.x);
+        else step_one<false, true>(d, s, j, ti * BX, threadIdx.x);
     } else {
         const int u = tr.tile - d.nt1;
         const int tb = u / d.nx4;
@@ -1566,7 +1734,8 @@ __global__ __launch_bounds__(BX * BZR) void k_stepx2(const ShotDev* __restrict__
         const int ti = tr.tile - tj * d.ntx;
         const int j = tj * BZR + wy;
         if (j >= d.nz) return;
-        step_one<false, true>(d, s, j, ti * BX, threadIdx.x);
+        if (d.mode == MODE_REC) rec_one<true>(d, s, j, ti * BX, threadIdx.x);
+        else step_one<false, true>(d, s, j, ti * BX, threadIdx.x);
     } else {
         const int u = tr.tile - d.nt1;
         const int tb = u / d.nx4;

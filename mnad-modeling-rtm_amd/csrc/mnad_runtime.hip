@@ -445,24 +445,29 @@ static int build_receivers(mnad_ctx* c, const Model& m, int nr, const int* zr, c
     return MNAD_OK;
 }
 
+void base_desc(const mnad_grid& g, int pitch, int nt, int mode, int nlev, ShotDev& d)
+{
+    d = ShotDev{};
+    d.nz = g.nz;
+    d.nx = g.nx;
+    d.pitch = pitch;
+    d.npml = g.npml;
+    d.nt = nt;
+    d.mode = mode;
+    d.kap = (float)((g.dx * g.dx) / (g.dz * g.dz));
+    incr_consts((g.dx * g.dx) / (g.dz * g.dz), d.ic);
+    d.inv_dt2 = (float)(1.0 / (g.dt * g.dt));
+    d.ntx = (g.nx + 63) / 64;
+    d.src_j = d.src_i = -1;
+    d.nlev = nlev;
+}
+
 int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
 {
     const mnad_grid& g = m.g;
     if (sp.nt < 1) return fail(MNAD_ERR_ARG, "nt must be >= 1");
-    s.d = ShotDev{};
-    s.d.nz = g.nz;
-    s.d.nx = g.nx;
-    s.d.pitch = m.pitch;
-    s.d.npml = g.npml;
-    s.d.nt = sp.nt;
-    s.d.mode = sp.mode;
-    s.d.kap = (float)((g.dx * g.dx) / (g.dz * g.dz));
-    incr_consts((g.dx * g.dx) / (g.dz * g.dz), s.d.ic);
-    s.d.inv_dt2 = (float)(1.0 / (g.dt * g.dt));
+    base_desc(g, m.pitch, sp.nt, sp.mode, two_step_kernel(c->kernel) ? 4 : 2, s.d);
     s.d.rx = m.rx();
-    s.d.ntx = (g.nx + 63) / 64;
-    s.d.src_j = s.d.src_i = -1;
-    s.d.nlev = two_step_kernel(c->kernel) ? 4 : 2;
     for (int k = 0; k < 3 * s.d.nlev; ++k) {
         CHK(dalloc(c, s.state[k], m.elems * sizeof(float), true));
         s.d.f[k / 3][k % 3] = s.state[k].as<float>();
@@ -503,7 +508,37 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
             CHK(dalloc(c, s.gather, (size_t)sp.nt * s.nu * sizeof(float), true));
             s.d.gather = s.gather.as<float>();
         }
-        if (sp.T > 0) {
+        if (sp.T > 0 && ring_format(sp.snap_fmt)) {
+            // illumination every T steps, boundary ring every step, no snapshots
+            if (np < 2) return fail(MNAD_ERR_ARG, "ring storage needs npml >= 2 (npml = %d)", np);
+            if (sp.ring_order < 2 || sp.ring_order > RING_MAX_ORDER)
+                return fail(MNAD_ERR_ARG, "ring predictor order %d not in 2..%d", sp.ring_order,
+                            RING_MAX_ORDER);
+            s.d.T = sp.T;
+            s.d.K = sp.K;
+            s.d.shift = sp.ncol - sp.K;
+            s.d.sfmt = SNAP_F32;
+            s.d.snaps = nullptr;
+            CHK(dalloc(c, s.illum, ni * sizeof(float), true));
+            s.d.illum = s.illum.as<float>();
+            const int P = sp.ring_order, nr = ring_cells(g.nz, g.nx, np);
+            s.d.rord = P;
+            s.d.nring = nr;
+            s.d.rfmt = sp.snap_fmt == MNAD_SNAP_RING_F16 ? SNAP_F16 : SNAP_BF16;
+            // fixed-point unit 2^-E = 2^-40 of the source amplitude a_src max|w|;
+            // the field stays far below 2^22 times that (|G^| < 2^62)
+            const double amax = asrc * std::max(wmax, 1e-300);
+            const int E = 40 - (int)std::ceil(std::log2(amax));
+            s.d.rsc = std::ldexp(1.0, E);
+            s.d.rsi = std::ldexp(1.0, -E);
+            // F16 codes: a residual of 2^P times the source amplitude maps to 2^14
+            s.d.rcs = s.d.rfmt == SNAP_F16 ? std::ldexp(1.0, -(26 + P)) : 1.0;
+            s.d.rci = 1.0 / s.d.rcs;
+            CHK(dalloc(c, s.rq, (size_t)sp.nt * 3 * nr * sizeof(uint16_t), false));
+            CHK(dalloc(c, s.rst, (size_t)P * 3 * nr * sizeof(int64_t), true));
+            s.d.rq = s.rq.as<unsigned short>();
+            s.d.rst = s.rst.as<long long>();
+        } else if (sp.T > 0) {
             s.d.T = sp.T;
             s.d.K = sp.K;
             s.d.shift = sp.ncol - sp.K;
@@ -535,6 +570,56 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
             s.d.img = s.img.as<float>();
             s.d.imgc = s.imgc.as<float>();
         }
+    }
+    return MNAD_OK;
+}
+
+int make_rec_shot(mnad_ctx* c, Shot& fwd, Shot& adj, Shot& rec)
+{
+    const ShotDev& f = fwd.d;
+    if (f.mode != MODE_FWD || f.rord <= 0 || f.T <= 0)
+        return fail(MNAD_ERR_STATE, "reverse reconstruction needs a forward that saved its ring");
+    if (f.nlev != 2)
+        return fail(MNAD_ERR_STATE, "reverse reconstruction runs on one-step kernels only");
+    rec.d = f;
+    ShotDev& d = rec.d;
+    d.mode = MODE_REC;
+    d.ntf = f.nt;
+    // the adjoint images at tt = m T with snapshot column shift + m <= K, m >= 1:
+    // the earliest forward step needed is (1 + shift) T
+    const int first = (1 + f.shift) * f.T;
+    d.nt = std::max(1, f.nt - first + 1);
+    d.nrec = 0;
+    d.rrow = nullptr;
+    d.rcol = nullptr;
+    d.gather = nullptr;
+    d.psx = d.psz = nullptr;
+    d.illum = nullptr;
+    d.img = d.imgc = nullptr;
+    d.sfmt = SNAP_SLOT;
+    for (int k = 0; k < 6; ++k) rec.state[k] = std::move(fwd.state[k]);
+    fwd.psx.reset();
+    fwd.psz.reset();
+    rec.amp = std::move(fwd.amp);
+    rec.rq = std::move(fwd.rq);
+    rec.rst = std::move(fwd.rst);
+    fwd.d.rq = nullptr;
+    fwd.d.rst = nullptr;
+    const size_t ni = (size_t)(f.nz - 2 * f.npml) * (f.nx - 2 * f.npml);
+    CHK(dalloc(c, rec.snaps, ni * sizeof(float), true));
+    d.snaps = rec.snaps.p;
+    // the adjoint reads the slot one launch after REC writes it:
+    // t0(rec) - t0(adj) = shift T - 1
+    adj.d.snaps = rec.snaps.p;
+    adj.d.sfmt = SNAP_SLOT;
+    adj.d.sscale = adj.d.sinv = 1.0f;
+    const long long lag = (long long)f.shift * f.T - 1;
+    if (lag >= 0) {
+        adj.t0 = 0;
+        rec.t0 = lag;
+    } else {
+        adj.t0 = -lag;
+        rec.t0 = 0;
     }
     return MNAD_OK;
 }
@@ -628,13 +713,17 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
         h[k].ntx = ntx;
         x4_layout(h[k], x4 ? nc : 4, ntx * ntz);
         const int nz = h[k].nz, nx = h[k].nx, x4hi = x4 ? h[k].x4hi : np;
+        const bool rec = h[k].mode == MODE_REC;
         for (int t = 0; t < ntx * ntz; ++t) {
             const int tj = t / ntx, ti = t - tj * ntx;
             const int r0 = tj * th, r1 = std::min(nz, r0 + th), c0 = ti * tw,
                       c1 = std::min(nx, c0 + tw);
             // kernels 6, 10: one-point tiles lying wholly in the interior are not listed
             if (x4 && r0 >= np && r1 <= nz - np && c0 >= np && c1 <= x4hi) continue;
-            const bool st = c0 < np || c1 > nx - np || r0 < np || r1 > nz - np;
+            // a reverse reconstruction touches the interior and its two-cell ring only
+            if (rec && (r1 <= np - 2 || r0 >= nz - np + 2 || c1 <= np - 2 || c0 >= nx - np + 2))
+                continue;
+            const bool st = !rec && (c0 < np || c1 > nx - np || r0 < np || r1 > nz - np);
             list.push_back(TileRef{k, t});
             strip.push_back(st);
             work.push_back(st ? sw : 1.0);
@@ -687,14 +776,27 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
 // - per receiver: its gather or injection value (4 B);
 // - every T steps: the snapshot plus illumination (forward) or image and compensation
 //   (adjoint) read and written.
+// - saved boundary ring, per ring cell, field and step: the predictor state read
+//   and written (8 (P + 1) B) and the 16-bit code written (forward) or read
+//   (reverse reconstruction);
+// - reverse reconstruction: 40 B per interior point (the CPML strips are not
+//   touched), plus the d2u slot written every T steps.
 double algorithmic_bytes(const ShotDev& d)
 {
     const double N = (double)d.nz * d.nx, np2 = 2.0 * d.npml;
     const double strips = np2 * d.nz + np2 * d.nx;
     const double ni = (double)(d.nz - 2 * d.npml) * (d.nx - 2 * d.npml);
-    const double esz = d.sfmt == SNAP_F32 ? 4.0 : 2.0;
-    double b = (double)d.nt * (40.0 * N + 84.0 * strips + 4.0 * d.nrec);
-    if (d.T > 0) b += (double)d.K * ni * (esz + (d.mode == MODE_FWD ? 8.0 : 16.0));
+    const double esz = (d.sfmt == SNAP_F32 || d.sfmt == SNAP_SLOT) ? 4.0 : 2.0;
+    const double ring = d.rord > 0 ? 3.0 * d.nring * (8.0 * (d.rord + 1) + 2.0) : 0.0;
+    if (d.mode == MODE_REC) {
+        double b = (double)d.nt * (40.0 * ni + ring + 12.0 * d.nring);   // + decoded ring written
+        if (d.T > 0) b += std::floor((double)d.nt / d.T) * ni * 4.0;
+        return b;
+    }
+    double b = (double)d.nt * (40.0 * N + 84.0 * strips + 4.0 * d.nrec + ring);
+    if (d.T > 0)
+        b += (double)d.K * ni * ((d.snaps || d.mode == MODE_ADJ ? esz : 0.0) +
+                                 (d.mode == MODE_FWD ? 8.0 : 16.0));
     return b;
 }
 
@@ -822,11 +924,16 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     const int kern = c->kernel, rz = c->rz, br = c->brows;
     for (int k = 0; k < n; ++k) {
         h[k] = shots[k]->d;
-        h[k].t0 = 0;
-        nsteps = std::max(nsteps, h[k].nt);
+        if (shots[k]->t0 < 0 || shots[k]->t0 > (1 << 20))
+            return fail(MNAD_ERR_STATE, "shot launch origin %lld out of range", shots[k]->t0);
+        h[k].t0 = (int)shots[k]->t0;
+        nsteps = std::max(nsteps, h[k].t0 + h[k].nt);
         if (h[k].nlev != (two_step_kernel(kern) ? 4 : 2))
             return fail(MNAD_ERR_STATE, "shot built for %d time levels cannot run on step kernel %d",
                         h[k].nlev, kern);
+        if ((h[k].rord > 0 || h[k].mode == MODE_REC) && !ring_kernel(kern))
+            return fail(MNAD_ERR_STATE, "step kernel %d does not save or replay the boundary ring "
+                        "(kernels 4, 5, 6, 10, 11 do)", kern);
     }
     if (resident_kernel(kern)) {
         double bytes = 0.0;
