@@ -1,15 +1,18 @@
 """Benchmark of the MNAD hot path on one MI355X.
 
-Workload (default, the configuration the headline metric is quoted on):
-single-shot FP32 modelling plus one-shot RTM on a 1001 x 501 grid
-(dx = dz = 10 m, 30 CPML cells -> 561 x 1061 padded), seeded layered model
-1500-4000 m/s with 5 % Gaussian-correlated perturbations (50 m), migration
-model = true model smoothed with a 100 m Gaussian, Ricker 20 Hz at the centre
-of the first physical row, 1001 receivers on the second, dt = 0.8 ms,
-nt = 4000.  One step = one shot: a forward on the true model (observed
-gather) and a forward on the migration model with d2u snapshots (both in one
-batched launch sequence), the scattered data, and the adjoint with the
-imaging condition.  Useful updates per step = 3 * N * nt (N padded points).
+Default workload (the largest single-GPU configuration, C5): batched RTM on
+the 8001 x 2001 model (dx = dz = 12.5 m, 30 CPML cells -> 2061 x 8061 padded),
+seeded layered + faulted + high-velocity-body synthetic, migration model = a
+100 m Gaussian smoothing, Ricker 30 Hz, dt = 1 ms, nt = 12000, the 64-shot
+layout of evenly spaced surface sources with a +-2 km split spread; the
+source wavefield is kept as the BF16 boundary ring the configuration names.
+One step = one shot of the batched job: a forward on the true model
+(observed gather) and a forward on the migration model (both in the shared
+forward launches), the scattered data, and the adjoint with the imaging
+condition (beside the reverse reconstruction of the source wavefield).
+Useful updates per step = 3 * N * nt (N padded points); the reconstruction
+is overhead and not counted.  --workload c2 runs the C2 single-shot RTM
+instead; c3 / c5all / c4 the many-shot and serving forms.
 
 Prints one JSON line (metric, value in Gpt-updates/s, roofline of the step
 kernel from HIP events on the library's stream, CPU baseline of the FP64
@@ -203,46 +206,225 @@ def run_multi(args, mnad, wl, label):
     print(json.dumps(out), flush=True)
 
 
-def run_c4(args, mnad):
-    """Serving: Poisson arrivals at --qps of modelling (70 %) and RTM (30 %)
-    requests; physical grids U[256,2048] x U[256,1024] (dx = 10 m, 20 CPML
-    cells), layered + random models 1500-4500 m/s, Ricker U[10,30] Hz,
-    nt U[1000,8000], dt = 0.8*0.65*dx/vmax (main_Acoustic_Data.m:71,77).
-    Reports p50/p99 latency (submit -> result) and aggregate Gpt-updates/s."""
-    import threading
-    from __graft_entry__ import load_package
-    load_package()
-    import importlib
-    synth = importlib.import_module("mnad_amd.synth")
-    rng = np.random.default_rng(args.seed)
-    npml, dx = 20, 10.0
-    NZ, NX = 1024 + 2 * npml, 2048 + 2 * npml
-    noise = synth.correlated_noise((NZ, NX), 50.0, dx, rng)
-    noise_s = synth.gaussian_smooth(noise, 100.0, dx)
-    n = args.requests
-    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, n))
-    specs = []
-    for k in range(n):
-        nxp, nzp = int(rng.integers(256, 2049)), int(rng.integers(256, 1025))
-        specs.append(dict(nxp=nxp, nzp=nzp, fc=float(rng.uniform(10, 30)),
-                          nt=int(rng.integers(1000, 8001)), rtm=bool(rng.random() < 0.3),
-                          oz=int(rng.integers(0, NZ - nzp - 2 * npml + 1)),
-                          ox=int(rng.integers(0, NX - nxp - 2 * npml + 1)),
-                          xs=int(rng.integers(0, nxp))))
+def run_c5_headline(args, mnad):
+    """The headline line: the C5 batched RTM, one step = one shot of the
+    batched job (the reference's parfor over shots, main_Acoustic_RTM.m:
+    223-226,250-254, as the batch axis).  The shots are the C5 layout's 64
+    sources in order (cycled past 64); --warmup shots run first in one batched
+    call, then --steps shots in one timed batched call (autotuned kernel and
+    shots per launch, hipGraph chunks).  Storage of the source wavefield:
+    --snap (default: the BF16 boundary ring the configuration names).  Extra
+    keys: the same job with f16 snapshots, the plain path (one shot and one
+    propagation per launch sequence, no graph), the C2 single-shot RTM."""
+    nt = 12000 if args.nt == 4000 else args.nt
+    wl = c5_workload(nt, 64)
+    v = wl["v"]
+    g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    N = v.shape[0] * v.shape[1]
+    fmt = mnad.SNAP_FORMATS[args.snap]
+    nsh = len(wl["xs"])
 
-    def build(sp):
+    def pick(k0, n):
+        idx = [(k0 + q) % nsh for q in range(n)]
+        return ([wl["zs"][q] for q in idx], [wl["xs"][q] for q in idx], [wl["recs"][q] for q in idx])
+
+    def job(ctx, k0, n, sf, batch=0, flags=0):
+        sid = ctx.stack_create(g)
+        zs, xs, recs = pick(k0, n)
+        ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], zs, xs, recs, snap_fmt=sf, batch=batch,
+                      flags=flags)
+        img, ill, _ = ctx.stack_image(sid, g)
+        ctx.stack_destroy(sid)
+        return img, ill
+
+    print(f"# C5 {v.shape[0]}x{v.shape[1]} nt={nt} snap={args.snap}", file=sys.stderr, flush=True)
+    extra = {}
+    with mnad.Context(0) as ctx:
+        t0 = time.perf_counter()
+        if args.tuning:   # a fixed kernel variant and --batch shots per group (no autotune)
+            k, rz, br = (int(x) for x in args.tuning.split(","))
+            ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
+            tuning, per_launch, us = ctx.tuning(), args.batch, 0.0
+        else:
+            tuning, per_launch, us = ctx.autotune(g, v, max_shots=32)
+        tune_s = time.perf_counter() - t0
+        print(f"# tuning {tune_s:.1f} s: {tuning}, {per_launch} shots/launch, "
+              f"{us:.2f} us/shot-step", file=sys.stderr, flush=True)
+        if args.warmup > 0:
+            job(ctx, 0, args.warmup, fmt, batch=args.batch)
+            print(f"# warmup: {args.warmup} shots done", file=sys.stderr, flush=True)
+        ctx.synchronize()
+        ctx.timing(reset=True)
+        t0 = time.perf_counter()
+        img, ill = job(ctx, args.warmup, args.steps, fmt, batch=args.batch)
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        k_ms, k_launches, k_bytes = ctx.timing(reset=True)
+        print(f"# timed: {args.steps} shots in {el:.1f} s", file=sys.stderr, flush=True)
+        if not args.no_plain:
+            t1 = time.perf_counter()
+            job(ctx, 0, 1, fmt, batch=1, flags=mnad.FLAG_SERIAL | mnad.FLAG_NO_GRAPH)
+            plain_s = time.perf_counter() - t1
+            extra["plain_ms_per_step"] = round(plain_s * 1e3, 1)
+            extra["speedup_vs_plain"] = round(plain_s / (el / args.steps), 3)
+            print(f"# plain path: {plain_s:.2f} s per shot", file=sys.stderr, flush=True)
+        if args.compare_snap and args.snap != "f16":
+            n2 = 4
+            t1 = time.perf_counter()
+            job(ctx, args.warmup, n2, mnad.SNAP_F16)
+            s2 = (time.perf_counter() - t1) / n2
+            extra["snapshots_f16"] = {
+                "ms_per_step": round(s2 * 1e3, 1),
+                "value": round(3.0 * N * nt / s2 / 1e9, 3), "shots": n2,
+                "bytes_per_shot_source_wavefield": int(
+                    snap_bytes(mnad, g, wl["vs"], nt, mnad.SNAP_F16))}
+            print(f"# f16 snapshots: {s2:.2f} s per shot", file=sys.stderr, flush=True)
+        extra["bytes_per_shot_source_wavefield"] = int(snap_bytes(mnad, g, wl["vs"], nt, fmt))
+        if not args.no_c2:
+            extra["c2_single_shot"] = c2_single_shot(args, mnad)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    kname = kernel_name(tuning)
+    out = {
+        "metric": METRIC, "value": round(3.0 * N * nt * args.steps / el / 1e9, 3),
+        "unit": "Gpt-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded layered + faulted + high-velocity-body model)",
+        "config": {"workload": (f"C5: 8001x2001 (+30 CPML) dx=dz=12.5 m, nt={nt}, dt=1 ms, Ricker "
+                                "30 Hz, 64-shot layout (split spread +-2 km), batched RTM, one "
+                                f"step = one shot; source wavefield: {args.snap}"),
+                   "grid_padded": [int(v.shape[0]), int(v.shape[1])], "nt": nt,
+                   "source_wavefield": args.snap, "tuning": tuning,
+                   "shots_per_launch_tuned": per_launch,
+                   "updates_per_step": 3 * N * nt},
+        "shots_per_hour": round(3600.0 * args.steps / el, 1),
+        "autotune_s": round(tune_s, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get(("c5", kname)),
+                     "traffic_unit": "bytes per launch (rocprofv3 counter runs)",
+                     "algorithmic_bytes_per_launch": round(k_bytes / max(k_launches, 1)),
+                     "kernel": kname, "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 3),
+                     "launches": int(k_launches)},
+        # img = Im ./ illum is 0/0 where no shot of the stack reached (far parts of
+        # the 100 km line); it must be finite everywhere else
+        "image_finite_where_illuminated": bool(np.isfinite(img[ill > 0]).all()),
+        "illuminated_fraction": round(float(np.mean(ill > 0)), 4),
+    }
+    out.update(extra)
+    if not args.no_cpu and args.cpu_steps > 0:
+        print("# cpu baseline (FP64 oracle, C5 window) ...", file=sys.stderr, flush=True)
+        cw = c5_workload(nt, 1, crop=True)
+        out["cpu_baseline"] = cpu_baseline(cw, min(args.cpu_steps, nt), args.cpu_procs,
+                                           label="C5 1001x501 window")
+    print(json.dumps(out), flush=True)
+
+
+def snap_bytes(mnad, g, vs, nt, fmt):
+    """Device bytes one shot keeps for its adjoint: K d2u snapshots of the
+    interior (fwd_PDE_exact_MNAD.m:137-142), or the boundary ring (three
+    fields on 4 (nx + nz - 4 npml + 4) cells every step, 16-bit) plus the last
+    two levels of the three fields."""
+    nzi, nxi = g.nz - 2 * g.npml, g.nx - 2 * g.npml
+    _, T, K, _ = mnad.snap_schedule(nt, float(np.min(vs)), float(np.max(vs)))
+    if fmt in (mnad.SNAP_RING_F16, mnad.SNAP_RING_BF16):
+        ring = 4 * ((nxi + 4) + nzi)
+        return nt * 3 * ring * 2 + 6 * g.nz * g.nx * 4
+    return K * nzi * nxi * (4 if fmt == mnad.SNAP_F32 else 2)
+
+
+def c2_single_shot(args, mnad, shots=3):
+    """C2 single-shot RTM (the observed-data forward and the migration-model
+    forward in one batched launch sequence, then the adjoint): seconds per
+    shot with --snap storage and on the plain path."""
+    wl = c2_workload(4000)
+    v = wl["v"]
+    g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    N = v.shape[0] * v.shape[1]
+    res = {}
+    with mnad.Context(0) as ctx:
+        for name, fmt in ((args.snap, mnad.SNAP_FORMATS[args.snap]), ("f16", mnad.SNAP_F16)):
+            sid = ctx.stack_create(g)
+            call = (sid, g, v, wl["vs"], wl["w"], [wl["zs"]], [wl["xs"]], [(wl["zr"], wl["xr"])])
+            ctx.rtm_shots(*call, snap_fmt=fmt, batch=1)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(shots):
+                ctx.rtm_shots(*call, snap_fmt=fmt, batch=1)
+            ctx.synchronize()
+            s = (time.perf_counter() - t0) / shots
+            t0 = time.perf_counter()
+            ctx.rtm_shots(*call, snap_fmt=fmt, batch=1, flags=mnad.FLAG_SERIAL | mnad.FLAG_NO_GRAPH)
+            plain = time.perf_counter() - t0
+            ctx.stack_destroy(sid)
+            res[name] = {"ms_per_shot": round(s * 1e3, 2), "value": round(3.0 * N * 4000 / s / 1e9, 3),
+                         "plain_ms_per_shot": round(plain * 1e3, 2),
+                         "speedup_vs_plain": round(plain / s, 3)}
+            if name == "f16":
+                break
+    print(f"# C2 single shot: {res}", file=sys.stderr, flush=True)
+    return res
+
+
+class C4Requests:
+    """The C4 serving workload's requests, seeded: physical grids
+    U[256,2048] x U[256,1024] (dx = 10 m, 20 CPML cells), layered + random
+    models 1500-4500 m/s cut from one correlated-noise field, Ricker
+    U[10,30] Hz, nt U[1000,8000], dt = 0.8*0.65*dx/vmax
+    (main_Acoustic_Data.m:71,77), 70 % modelling / 30 % RTM, Poisson
+    arrivals at `qps`.  build(k) returns request k's arrays."""
+
+    npml, dx = 20, 10.0
+
+    def __init__(self, n, seed, qps=10.0):
+        from __graft_entry__ import load_package
+        load_package()
+        import importlib
+        self.synth = importlib.import_module("mnad_amd.synth")
+        rng = np.random.default_rng(seed)
+        npml, dx = self.npml, self.dx
+        NZ, NX = 1024 + 2 * npml, 2048 + 2 * npml
+        self.noise = self.synth.correlated_noise((NZ, NX), 50.0, dx, rng)
+        self.noise_s = self.synth.gaussian_smooth(self.noise, 100.0, dx)
+        self.arrivals = np.cumsum(rng.exponential(1.0 / qps, n))
+        self.specs = []
+        for _ in range(n):
+            nxp, nzp = int(rng.integers(256, 2049)), int(rng.integers(256, 1025))
+            self.specs.append(dict(nxp=nxp, nzp=nzp, fc=float(rng.uniform(10, 30)),
+                                   nt=int(rng.integers(1000, 8001)), rtm=bool(rng.random() < 0.3),
+                                   oz=int(rng.integers(0, NZ - nzp - 2 * npml + 1)),
+                                   ox=int(rng.integers(0, NX - nxp - 2 * npml + 1)),
+                                   xs=int(rng.integers(0, nxp))))
+
+    def build(self, k):
+        """(nz, nx, npml, dx, dt, v, vs, w, zs, xs, zr, xr) of request k;
+        vs is None for a modelling request."""
+        sp, npml, dx = self.specs[k], self.npml, self.dx
         nz, nx = sp["nzp"] + 2 * npml, sp["nxp"] + 2 * npml
         z = np.arange(nz) * dx
         base = 1500.0 + 2700.0 * (z / z[-1])[:, None]
         crop = np.s_[sp["oz"]:sp["oz"] + nz, sp["ox"]:sp["ox"] + nx]
-        v = np.clip(base * (1 + 0.03 * noise[crop]), 1500.0, 4500.0)
-        vs = np.clip(base * (1 + 0.03 * noise_s[crop]), 1500.0, 4500.0) if sp["rtm"] else None
+        v = np.asfortranarray(np.clip(base * (1 + 0.03 * self.noise[crop]), 1500.0, 4500.0))
+        vs = (np.asfortranarray(np.clip(base * (1 + 0.03 * self.noise_s[crop]), 1500.0, 4500.0))
+              if sp["rtm"] else None)
         dt = 0.8 * 0.65 * dx / 4500.0
-        w = synth.ricker(sp["fc"], dt, sp["nt"])
+        w = self.synth.ricker(sp["fc"], dt, sp["nt"])
         xr = npml + np.arange(0, sp["nxp"], 4)
         zr = np.full(xr.size, npml + 1)
-        g = mnad.Grid(nz, nx, npml, dx, dx, dt)
-        return g, v, vs, w, npml, npml + sp["xs"], zr, xr, nz * nx
+        return nz, nx, npml, dx, dt, v, vs, w, npml, npml + sp["xs"], zr, xr
+
+
+def run_c4(args, mnad):
+    """Serving: Poisson arrivals at --qps of the C4 requests (C4Requests).
+    Reports p50/p99 latency (submit -> result) and aggregate Gpt-updates/s."""
+    import threading
+    reqs = C4Requests(args.requests, args.seed, args.qps)
+    n = args.requests
+    arrivals, specs = reqs.arrivals, reqs.specs
+
+    def build(k):
+        nz, nx, npml, dx, dt, v, vs, w, zs, xs, zr, xr = reqs.build(k)
+        return mnad.Grid(nz, nx, npml, dx, dx, dt), v, vs, w, zs, xs, zr, xr, nz * nx
 
     lat, results, errors = {}, {}, []
     tickets = []
@@ -277,7 +459,7 @@ def run_c4(args, mnad):
         th.start()
         t0 = time.perf_counter()
         for k, sp in enumerate(specs):
-            g, v, vs, w, zs, xs, zr, xr, N = build(sp)
+            g, v, vs, w, zs, xs, zr, xr, N = build(k)
             delay = arrivals[k] - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
@@ -396,17 +578,28 @@ def _cpu_shot(args):
     return v.size * steps, time.perf_counter() - t0
 
 
-def cpu_baseline(wl, steps, procs=0):
+def host_cpus():
+    """CPUs of the host (nproc) and those this process may run on."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    return os.cpu_count() or 1, aff
+
+
+def cpu_baseline(wl, steps, procs=0, label=""):
     """FP64 oracle on the first `steps` forward steps of the true-model shot of
     the same workload: once on one core, then one such shot per core on `procs`
     cores at once (the reference's parfor over shots); the aggregate rate is
-    the value.  procs = 0: the host's cores, at most 16 (one GPU's share)."""
+    the value.  procs = 0: every CPU this process may run on, at most 16 (the
+    CPU share a one-GPU job gets on the GPU hosts; nproc is reported beside)."""
     from multiprocessing import get_context
     sample = dict(v=wl["v"], w=wl["w"], npml=wl["npml"], dx=wl["dx"], dt=wl["dt"],
                   zs=wl["zs"], xs=wl["xs"], zr=wl["zr"], xr=wl["xr"])
     n1, t1 = _cpu_shot((sample, steps))
     single = n1 / t1 / 1e9
-    procs = procs or min(16, os.cpu_count() or 1)
+    nproc, aff = host_cpus()
+    procs = procs or min(16, aff)
     if procs > 1:
         with get_context("spawn").Pool(procs) as pool:
             t0 = time.perf_counter()
@@ -415,17 +608,20 @@ def cpu_baseline(wl, steps, procs=0):
         value = sum(r[0] for r in res) / el / 1e9
     else:
         value, el = single, t1
+    where = f" of the {label}" if label else ""
     return dict(value=value, unit="Gpt-updates/s", cores=procs, kind="port",
-                single_core_value=single,
-                sample=f"FP64 oracle forward, true model, first {steps} of {wl['w'].size} steps; "
-                       f"1 shot on 1 core ({t1:.1f} s), then {procs} such shots on {procs} "
+                single_core_value=single, host_nproc=nproc, host_cpus_allowed=aff,
+                sample=f"FP64 oracle forward{where}, true model, first {steps} of {wl['w'].size} "
+                       f"steps; 1 shot on 1 core ({t1:.1f} s), then {procs} such shots on {procs} "
                        f"cores at once ({el:.1f} s)")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5crop"])
+    ap.add_argument("--workload", default="c5", choices=["c2", "c3", "c4", "c5", "c5all", "c5crop"],
+                    help="c5 (default): the C5 batched RTM, one step = one shot; c5all: all 64 "
+                         "C5 shots as one step; c2: the C2 single-shot RTM")
     ap.add_argument("--requests", type=int, default=1000, help="C4: number of requests")
     ap.add_argument("--qps", type=float, default=10.0, help="C4: Poisson arrival rate")
     ap.add_argument("--chunk", type=int, default=32, help="C4: steps between admissions")
@@ -447,9 +643,15 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=4000)
-    ap.add_argument("--snap", default="f16", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--snap", default="",
+                    choices=["", "f32", "f16", "bf16", "ring_f16", "ring_bf16"],
+                    help="source-wavefield storage (default: ring_bf16 for C5, f16 otherwise)")
+    ap.add_argument("--no-compare-snap", dest="compare_snap", action="store_false",
+                    help="C5: skip the f16-snapshot comparison")
+    ap.add_argument("--no-c2", action="store_true", help="C5: skip the C2 single-shot extra keys")
     ap.add_argument("--npml", type=int, default=30, help="C2: CPML cells (30 in the workload)")
-    ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="CPU baseline sample steps (0: 300 on the C5 window, 400 on C2)")
     ap.add_argument("--cpu-procs", type=int, default=0,
                     help="CPU baseline processes (0 = host cores, at most 16)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
@@ -528,7 +730,14 @@ def main():
     if args.workload == "c4":
         run_c4(args, mnad)
         return
+    if not args.snap:
+        args.snap = "ring_bf16" if args.workload == "c5" else "f16"
+    if args.cpu_steps == 0:
+        args.cpu_steps = 300 if args.workload == "c5" else 400
     if args.workload == "c5":
+        run_c5_headline(args, mnad)
+        return
+    if args.workload == "c5all":
         nt = 12000 if args.nt == 4000 else args.nt
         run_multi(args, mnad, c5_workload(nt, args.shots),
                   f"C5: 8001x2001 (+30 CPML) dx=12.5 m, {args.shots} shots batched RTM, nt={nt}")

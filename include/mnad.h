@@ -84,6 +84,9 @@ MNAD_API int mnad_device_count(void);
  * device stream, the memory pool and the source-wavefield handles. */
 MNAD_API int mnad_ctx_create(int device, mnad_ctx **out);
 MNAD_API void mnad_ctx_destroy(mnad_ctx *ctx);
+/* Waits until all work queued on the context's stream has finished (every
+ * other entry point returns with its results on the host already). */
+MNAD_API int mnad_ctx_sync(mnad_ctx *ctx);
 
 /* Forward shot.  Replaces
  *   data = fwd_PDE_exact_MNAD(dx,xs,xr,dz,zs,zr,v,dt,nt,w,d_pml,ss,Auu,...,Aww,indx)

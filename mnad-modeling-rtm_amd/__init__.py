@@ -19,9 +19,11 @@ without the built library every call raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import threading
+import weakref
 
 import numpy as np
 
@@ -49,7 +51,7 @@ EXPORTS = (
     "mnad_stack_image", "mnad_rtm_shots", "mnad_fwd_shots", "mnad_mem_info",
     "mnad_timing", "mnad_get_tuning", "mnad_set_tuning", "mnad_autotune",
     "mnad_serve_start", "mnad_serve_stop", "mnad_submit", "mnad_wait", "mnad_cancel",
-    "mnad_serve_status", "mnad_snap_save", "mnad_snap_load",
+    "mnad_serve_status", "mnad_snap_save", "mnad_snap_load", "mnad_ctx_sync",
 )
 
 
@@ -110,6 +112,7 @@ def lib():
             L.mnad_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
             L.mnad_ctx_destroy.argtypes = [vp]
             L.mnad_ctx_destroy.restype = None
+            L.mnad_ctx_sync.argtypes = [vp]
             L.mnad_fwd.argtypes = [vp, gp, dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, ip, ip,
                                    C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp]
             L.mnad_born_adj.argtypes = [vp, gp, dp, C.c_int, C.c_int, ip, ip, dp, C.c_int,
@@ -189,6 +192,21 @@ def snap_schedule(nt, vmin, vmax):
     return a.value, b.value, c.value, d.value
 
 
+# Contexts still open.  They are closed by an exit handler, while the
+# interpreter and the HIP runtime are both intact, rather than by __del__ during
+# interpreter teardown (after which the runtime's own exit handlers run).
+_OPEN = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_contexts():
+    for c in list(_OPEN):
+        try:
+            c.close()
+        except Exception:   # noqa: BLE001 - best effort at exit
+            pass
+
+
 def _shots_struct(zs, xs, recs):
     """recs: list of (zr, xr) per shot (0-based). Keeps arrays alive."""
     zs = _i32(zs)
@@ -218,11 +236,18 @@ class Context:
     def __init__(self, device: int = 0):
         self._h = C.c_void_p()
         _check(lib().mnad_ctx_create(int(device), C.byref(self._h)))
+        _OPEN.add(self)
 
     def close(self):
         if self._h:
             lib().mnad_ctx_destroy(self._h)
             self._h = C.c_void_p()
+        _OPEN.discard(self)
+
+    def synchronize(self):
+        """Wait for the context's stream (every call already returns with its
+        results on the host)."""
+        _check(lib().mnad_ctx_sync(self._h))
 
     def __del__(self):
         try:

@@ -130,6 +130,13 @@ void mnad_ctx_destroy(mnad_ctx* ctx)
     delete ctx;
 }
 
+int mnad_ctx_sync(mnad_ctx* ctx)
+{
+    ENTER(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MNAD_OK;
+}
+
 int mnad_snap_schedule(int nt, double vmin, double vmax, int* nf, int* T, int* K, int* ncol)
 {
     if (!(vmin > 0) || !(vmax >= vmin)) return fail(MNAD_ERR_ARG, "bad velocity range");

@@ -257,6 +257,11 @@ int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m);
 size_t scan_model(const double* v, size_t n, double* vmin, double* vmax);
 void model_digest(const double* v, size_t n, uint64_t* da, uint64_t* db);
 int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s);
+// Unique receivers of a shot and the kernel's row/column lookup tables (host).
+int receiver_tables(const mnad_grid& g, int nr, const int* zr, const int* xr, Shot& s,
+                    std::vector<int>& rrow, std::vector<int>& rcol);
+// Grid checks of make_model (no device work).
+int check_grid(const mnad_grid* g);
 // Grid part of a shot descriptor (no buffers).
 void base_desc(const mnad_grid& g, int pitch, int nt, int mode, int nlev, ShotDev& d);
 // Reverse reconstruction of a finished ring-saving forward `fwd` (its last

@@ -231,7 +231,7 @@ static int ensure_batch_buf(mnad_ctx* c, DevBuf& b, size_t bytes)
 }
 
 // ----------------------------------------------------------------- model
-static int check_grid(const mnad_grid* g)
+int check_grid(const mnad_grid* g)
 {
     if (!g) return fail(MNAD_ERR_ARG, "grid is NULL");
     if (g->nz < 3 || g->nx < 3 || g->npml < 0)
@@ -397,11 +397,12 @@ int make_model(mnad_ctx* c, const mnad_grid* g, const double* v, Model& m)
 // ------------------------------------------------------------------ shot
 static size_t snap_elem(int fmt) { return fmt == MNAD_SNAP_F32 ? 4 : 2; }
 
-// Unique receiver points and the row/column lookup tables of the kernel.
-static int build_receivers(mnad_ctx* c, const Model& m, int nr, const int* zr, const int* xr,
-                           Shot& s)
+// Unique receiver points of a shot (s.uid, s.upts, s.nu) and the row/column
+// lookup tables of the kernel (rrow [nz], rcol [rows with receivers * nx]);
+// host only.
+int receiver_tables(const mnad_grid& g, int nr, const int* zr, const int* xr, Shot& s,
+                    std::vector<int>& rrow, std::vector<int>& rcol)
 {
-    const mnad_grid& g = m.g;
     s.uid.assign(nr, -1);
     s.upts.clear();
     std::map<std::pair<int, int>, int> seen;
@@ -422,16 +423,28 @@ static int build_receivers(mnad_ctx* c, const Model& m, int nr, const int* zr, c
     }
     s.nu = (int)s.upts.size();
     s.d.nrec = s.nu;
+    rrow.assign(g.nz, -1);
+    rcol.clear();
     if (s.nu == 0) return MNAD_OK;
-    std::vector<int> rrow(g.nz, -1);
     std::vector<int> rows;
     for (auto& pt : s.upts)
         if (rrow[pt.first] < 0) {
             rrow[pt.first] = (int)rows.size() * g.nx;
             rows.push_back(pt.first);
         }
-    std::vector<int> rcol(rows.size() * (size_t)g.nx, -1);
+    rcol.assign(rows.size() * (size_t)g.nx, -1);
     for (int u = 0; u < s.nu; ++u) rcol[rrow[s.upts[u].first] + s.upts[u].second] = u;
+    return MNAD_OK;
+}
+
+// receiver_tables on the device, and a = (v dt)^2 at each unique receiver.
+static int build_receivers(mnad_ctx* c, const Model& m, int nr, const int* zr, const int* xr,
+                           Shot& s)
+{
+    const mnad_grid& g = m.g;
+    std::vector<int> rrow, rcol;
+    CHK(receiver_tables(g, nr, zr, xr, s, rrow, rcol));
+    if (s.nu == 0) return MNAD_OK;
     CHK(upload(c, s.rrow, rrow.data(), rrow.size() * sizeof(int)));
     CHK(upload(c, s.rcol, rcol.data(), rcol.size() * sizeof(int)));
     s.d.rrow = s.rrow.as<int>();
