@@ -163,6 +163,22 @@ MNAD_API int mnad_fwd_shots(mnad_ctx *ctx, const mnad_grid *g, const double *v, 
                    const double *w, const mnad_shots *shots, int batch,
                    int flags, double *datau);
 
+/* Forward shot without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same computation as mnad_fwd(..., indx = 0, flags = MNAD_FLAG_NO_GRAPH)
+ * and the same results; every device buffer beyond the caller's inputs and
+ * outputs comes from a fixed 64 MiB array in device memory that each GPU the
+ * library runs on holds until the process ends (about a single 1001 x 501
+ * shot with 30 CPML cells, 4000 steps and 1001 receivers; MNAD_ERR_ARG, with
+ * the bytes needed, beyond that).  Calls on one GPU share that array: within
+ * a process they are serialised, and each call clears what must start at
+ * zero.  Inputs and outputs are host arrays as in mnad_fwd, copied on the
+ * context's stream; U, V, W, datau may be NULL. */
+MNAD_API int mnad_fwd_noalloc(mnad_ctx *ctx, const mnad_grid *g, const double *v, int nt,
+                              const double *w, int zs, int xs, int nr, const int *zr,
+                              const int *xr, double *datau, double *U, double *V, double *W);
+
 /* Step-kernel accounting since the last reset: GPU time of the step-kernel
  * launch sequences measured with HIP events on the context's stream (ms,
  * inter-launch gaps included), the number of step launches, and the

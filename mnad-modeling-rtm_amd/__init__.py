@@ -52,6 +52,7 @@ EXPORTS = (
     "mnad_timing", "mnad_get_tuning", "mnad_set_tuning", "mnad_autotune",
     "mnad_serve_start", "mnad_serve_stop", "mnad_submit", "mnad_wait", "mnad_cancel",
     "mnad_serve_status", "mnad_snap_save", "mnad_snap_load", "mnad_ctx_sync",
+    "mnad_fwd_noalloc",
 )
 
 
@@ -115,6 +116,8 @@ def lib():
             L.mnad_ctx_sync.argtypes = [vp]
             L.mnad_fwd.argtypes = [vp, gp, dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, ip, ip,
                                    C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp]
+            L.mnad_fwd_noalloc.argtypes = [vp, gp, dp, C.c_int, dp, C.c_int, C.c_int, C.c_int,
+                                           ip, ip, dp, dp, dp, dp]
             L.mnad_born_adj.argtypes = [vp, gp, dp, C.c_int, C.c_int, ip, ip, dp, C.c_int,
                                         C.c_int, dp]
             L.mnad_release.argtypes = [vp, C.c_int]
@@ -285,6 +288,24 @@ class Context:
                               _ip(zr), _ip(xr), int(ss), int(indx), int(snap_fmt), int(flags),
                               _dp(gather), _dp(U), _dp(V), _dp(W), _dp(illum)))
         return dict(gather=gather, U=U, V=V, W=W, illum=illum)
+
+    def fwd_noalloc(self, grid: Grid, v, w, zs, xs, zr, xr, want_state=True):
+        """mnad_fwd_noalloc: the forward shot of fwd(indx=0) with every device
+        buffer taken from the library's fixed scratch array (no allocation)."""
+        v = _f64(v)
+        w = np.ascontiguousarray(np.asarray(w, dtype=np.float64).ravel())
+        zr, xr = _i32(zr), _i32(xr)
+        if zr.size != xr.size:
+            raise MnadError(ERR_ARG, "zr and xr differ in length")
+        nt, nr = w.size, zr.size
+        gather = np.zeros((nt, nr), order="F")
+        U = V = W = None
+        if want_state:
+            U, V, W = (np.zeros((grid.nz, grid.nx), order="F") for _ in range(3))
+        _check(lib().mnad_fwd_noalloc(self._h, C.byref(grid), _dp(v), nt, _dp(w), int(zs),
+                                      int(xs), nr, _ip(zr), _ip(xr), _dp(gather), _dp(U),
+                                      _dp(V), _dp(W)))
+        return dict(gather=gather, U=U, V=V, W=W)
 
     def born_adj(self, grid: Grid, v, data, zr, xr, ss=1, flags=0):
         """RTM adjoint of shot ss; returns Im [nzi*nxi] (column-major interior)."""

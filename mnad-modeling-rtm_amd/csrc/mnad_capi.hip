@@ -94,6 +94,11 @@ int mnad_ctx_create(int device, mnad_ctx** out)
         delete c;
         return fail(MNAD_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
+    if (int rc = noalloc_init_device(device); rc != MNAD_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     if (const char* s = getenv("MNAD_GRAPH_CHUNK")) c->graph_chunk = std::max(1, atoi(s));
     if (const char* s = getenv("MNAD_KERNEL")) c->kernel = atoi(s);
     if (const char* s = getenv("MNAD_RZ")) c->rz = atoi(s);

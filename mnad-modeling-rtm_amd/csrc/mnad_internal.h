@@ -260,6 +260,9 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s);
 // Unique receivers of a shot and the kernel's row/column lookup tables (host).
 int receiver_tables(const mnad_grid& g, int nr, const int* zr, const int* xr, Shot& s,
                     std::vector<int>& rrow, std::vector<int>& rcol);
+// Device address of the allocation-free entry point's scratch array on
+// `device` (current), taken once per device; mnad_ctx_create calls it.
+int noalloc_init_device(int device);
 // Grid checks of make_model (no device work).
 int check_grid(const mnad_grid* g);
 // Grid part of a shot descriptor (no buffers).
