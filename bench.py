@@ -45,6 +45,9 @@ MEASURED_TRAFFIC = {
     # C2 RTM shot, nt = 300: forward (2 shots) 69.68 MB, adjoint 35.58 MB,
     # equal launch counts -> 52.63 MB per launch (profiles/k_stepw_c2_rtm_shot.md)
     ("c2", "mnad::k_stepw<1, false>"): 52.63e6,
+    # C5, 20 shots, BF16 ring, groups of 10, pipelined: 2.0 x 6,480,359 +
+    # 5,777,001 KiB per launch (profiles/k_stepx_c5_ring_bf16_batched.md)
+    ("c5", "mnad::k_stepx<4>", "ring_bf16", 20, 10, True): 19.187e9,
 }
 
 
@@ -260,6 +263,7 @@ def run_c5_headline(args, mnad):
         ctx.synchronize()
         el = time.perf_counter() - t0
         k_ms, k_launches, k_bytes = ctx.timing(reset=True)
+        group, pipelined = ctx.rtm_schedule()
         print(f"# timed: {args.steps} shots in {el:.1f} s", file=sys.stderr, flush=True)
         if not args.no_plain:
             t1 = time.perf_counter()
@@ -296,12 +300,15 @@ def run_c5_headline(args, mnad):
                    "grid_padded": [int(v.shape[0]), int(v.shape[1])], "nt": nt,
                    "source_wavefield": args.snap, "tuning": tuning,
                    "shots_per_launch_tuned": per_launch,
+                   "shots_per_group": group,
+                   "schedule": "pipelined" if pipelined else "in order",
                    "updates_per_step": 3 * N * nt},
         "shots_per_hour": round(3600.0 * args.steps / el, 1),
         "autotune_s": round(tune_s, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get(("c5", kname)),
+                     "traffic": MEASURED_TRAFFIC.get(("c5", kname, args.snap, args.steps, group,
+                                                      pipelined)),
                      "traffic_unit": "bytes per launch (rocprofv3 counter runs)",
                      "algorithmic_bytes_per_launch": round(k_bytes / max(k_launches, 1)),
                      "kernel": kname, "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 3),

@@ -156,6 +156,12 @@ MNAD_API int mnad_rtm_shots(mnad_ctx *ctx, int stack_id, const mnad_grid *g,
                    const double *w, const mnad_shots *shots, int snap_fmt,
                    int batch, int flags, double *dobs, double *dsyn);
 
+/* Schedule the last mnad_rtm_shots call on this context used: shots per
+ * group (one launch sequence runs a group's forwards, the next its adjoints)
+ * and whether a group's forwards shared launches with the previous group's
+ * adjoints (pipelined = 1) or ran in order (0). */
+MNAD_API int mnad_rtm_schedule(mnad_ctx *ctx, int *shots_per_group, int *pipelined);
+
 /* Forward modelling of several shots on one model, batched in one launch
  * sequence (the parfor of main_Acoustic_Data.m:169-171).  datau receives the
  * gathers concatenated per shot ([nt x nr_k] column-major each). */

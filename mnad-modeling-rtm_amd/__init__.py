@@ -52,7 +52,7 @@ EXPORTS = (
     "mnad_timing", "mnad_get_tuning", "mnad_set_tuning", "mnad_autotune",
     "mnad_serve_start", "mnad_serve_stop", "mnad_submit", "mnad_wait", "mnad_cancel",
     "mnad_serve_status", "mnad_snap_save", "mnad_snap_load", "mnad_ctx_sync",
-    "mnad_fwd_noalloc",
+    "mnad_fwd_noalloc", "mnad_rtm_schedule",
 )
 
 
@@ -114,6 +114,7 @@ def lib():
             L.mnad_ctx_destroy.argtypes = [vp]
             L.mnad_ctx_destroy.restype = None
             L.mnad_ctx_sync.argtypes = [vp]
+            L.mnad_rtm_schedule.argtypes = [vp, ip, ip]
             L.mnad_fwd.argtypes = [vp, gp, dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, ip, ip,
                                    C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp]
             L.mnad_fwd_noalloc.argtypes = [vp, gp, dp, C.c_int, dp, C.c_int, C.c_int, C.c_int,
@@ -365,6 +366,12 @@ class Context:
         split = lambda a: [a[off[k] * nt:off[k + 1] * nt].reshape(nt, -1, order="F")
                            for k in range(len(off) - 1)]
         return split(dobs), split(dsyn)
+
+    def rtm_schedule(self):
+        """(shots per group, pipelined) of the last rtm_shots call."""
+        a, b = C.c_int(), C.c_int()
+        _check(lib().mnad_rtm_schedule(self._h, C.byref(a), C.byref(b)))
+        return a.value, bool(b.value)
 
     def fwd_shots(self, grid: Grid, v, w, zs, xs, recs, batch=0, flags=0):
         """Batched forward modelling; returns the list of gathers [nt x nr_k]."""

@@ -757,6 +757,8 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
     }
     if (serial) batch = 1;
     const int B = pick_batch(ctx, pipelined ? per_pip : per_ord, sh->nshots, batch);
+    ctx->last_group = B;
+    ctx->last_pipelined = pipelined ? 1 : 0;
     if (getenv("MNAD_AUTOTUNE_LOG"))
         fprintf(stderr, "mnad_rtm_shots %dx%d: %d shots, %s, %d per group\n", g->nz, g->nx,
                 sh->nshots, serial ? "serial" : (pipelined ? "pipelined" : "in order"), B);
@@ -885,6 +887,14 @@ extern "C" int mnad_rtm_shots(mnad_ctx* ctx, int stack_id, const mnad_grid* g,
         if (prev) CHK(stack(*prev));
         prev = std::move(cur);
     }
+    return MNAD_OK;
+}
+
+extern "C" int mnad_rtm_schedule(mnad_ctx* ctx, int* shots_per_group, int* pipelined)
+{
+    ENTER(ctx);
+    if (shots_per_group) *shots_per_group = ctx->last_group;
+    if (pipelined) *pipelined = ctx->last_pipelined;
     return MNAD_OK;
 }
 

@@ -170,6 +170,7 @@ struct mnad_ctx {
     int brows = mnad::kDefaultBrows;   // thread rows per block (tile = 64 x brows*rz)
     int strip_first = 1;        // dispatch CPML-strip tiles first
     int ring_order = 3;         // predictor order of the saved boundary ring (2..4)
+    int last_group = 0, last_pipelined = 0;   // schedule of the last mnad_rtm_shots
     // XCD balance of one-step kernels: estimated work of a CPML-strip tile and
     // of a four-column tile, in interior one-point tiles
     double strip_weight = 2.0;
