@@ -24,12 +24,15 @@ def ricker(fc: float, dt: float, nt: int) -> np.ndarray:
     return (1 - 2 * tau**2) * np.exp(-tau**2)
 
 
-def reference_dx_dt(vmin: float, vmax: float, fc: float, T_max: float):
+def reference_dx_dt(vmin: float, vmax: float, fc: float, T_max: float, with_end: bool = False):
     """main_Acoustic_Data.m:66-86: dx = vmin/(2.5*3fc); dt = 0.8*0.65*dx/vmax
-    computed from the un-floored dx; then dx = floor(dx); nt = length(0:dt:T_max-dt)."""
+    computed from the un-floored dx; then dx = floor(dx); nt = length(0:dt:T_max-dt).
+    with_end: the example drivers' time axis t = 0:dt:T_max
+    (examples/modeling/main_Acoustic_Data.m:77-79), nt = length(0:dt:T_max)."""
     dx = vmin / (2.5 * 3 * fc)
     dt = 0.8 * ((0.65 * dx) / vmax)
-    nt = int(np.floor((T_max - dt) / dt + 1e-10)) + 1
+    last = T_max if with_end else T_max - dt
+    nt = int(np.floor(last / dt + 1e-10)) + 1
     return float(np.floor(dx)), dt, nt
 
 

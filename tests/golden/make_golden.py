@@ -71,7 +71,7 @@ def build_ops(dx, dz, dt, v):
         ((2, nz), (1, nx - 1), -1, 1, lambda j, i: B(j, i) / (2 * dx**2 * dz**2)),
         ((1, nz - 1), (1, nx - 1), 1, 1, lambda j, i: B(j, i) / (2 * dx**2 * dz**2)),
     ])
-    # Auv_gen.m:116-170
+    # Auv_gen.m:24-78
     ops["Auv"] = _assemble(nz, nx, [
         ((1, nz), (2, nx), 0, -1, lambda j, i: (-20 / 33) * A(j, i) / dx + B(j, i) * (47 / (66 * dx**3) + 1 / (6 * dx * dz**2))),
         ((1, nz), (1, nx - 1), 0, 1, lambda j, i: (20 / 33) * A(j, i) / dx - B(j, i) * (47 / (66 * dx**3) + 1 / (6 * dx * dz**2))),

@@ -50,12 +50,18 @@ def test_golden_final_fields(mnad, ctx, name):
         assert rel_l2(r[k], f[k]) < GATHER_TOL, k
 
 
-@pytest.mark.parametrize("fmt,tol", [("f32", 1e-4), ("f16", IMAGE_TOL), ("bf16", 2e-2)])
+# BF16 storage of the source wavefield is the boundary ring (ring_bf16), held
+# to the image bound like every other format.  BF16 *snapshots* (8 mantissa
+# bits of d2u) are kept only as a comparison format: they measure 0.9-1.2e-3 on
+# these fixtures and 2.2e-3 on the full-size C2 image, so they do not meet the
+# bound, and their test states that error (2e-3 guard) instead.
+@pytest.mark.parametrize("fmt,tol", [("f32", 1e-4), ("f16", IMAGE_TOL), ("ring_bf16", IMAGE_TOL),
+                                     ("ring_f16", IMAGE_TOL), ("bf16", 2e-3)])
 @pytest.mark.parametrize("name", ["rtm_two_shot", "rtm_rect"])
 def test_golden_rtm_matlab_signature(mnad, name, fmt, tol):
     """fwd(indx=1) on the smooth model, scattered data, Born_adj_MNAD: the
-    per-shot image against the sparse transliteration.  bf16 keeps 8 mantissa
-    bits, so its stated error is 2e-2; f32 and f16 meet the 1e-3 target."""
+    per-shot image against the sparse transliteration, for every storage of
+    the source wavefield."""
     _say(f"golden rtm {name} {fmt}")
     f = golden(name)
     dx, dz, dt, d = float(f["dx"]), float(f["dz"]), float(f["dt"]), int(f["npml"])
@@ -242,8 +248,9 @@ def test_launch_shapes_agree(mnad, ctx, oracle, kernel, rz, br, strip_first):
                                           (9, 4, 8), (10, 1, 1), (10, 1, 4), (11, 2, 1)])
 def test_kernel_families_rtm_and_multi_tile(mnad, kernel, rz, br):
     """Each step-kernel family through the whole RTM (snapshots, imaging,
-    receiver injection; f32, f16 and bf16 snapshots) against the stacked fixture
-    image, and on C1 (241 x 241: several tile columns, so the left and right
+    receiver injection; f32, f16 and bf16 snapshots and the bf16 boundary ring,
+    which kernels without the ring code hand to kernel 4) against the stacked
+    fixture image, and on C1 (241 x 241: several tile columns, so the left and right
     CPML strips sit in different tiles) against the default kernel."""
     _say(f"kernel family kernel={kernel} rz={rz} br={br}")
     f = golden("rtm_two_shot")
@@ -259,7 +266,8 @@ def test_kernel_families_rtm_and_multi_tile(mnad, kernel, rz, br):
     base = ctx.fwd(gc, v, w, npml, xsc, zr, xr)
     try:
         ctx.set_tuning(kernel=kernel, rows_per_thread=rz, block_rows=br)
-        for fmt, tol in ((mnad.SNAP_F32, 1e-4), (mnad.SNAP_F16, IMAGE_TOL), (mnad.SNAP_BF16, 2e-2)):
+        for fmt, tol in ((mnad.SNAP_F32, 1e-4), (mnad.SNAP_F16, IMAGE_TOL), (mnad.SNAP_BF16, IMAGE_TOL),
+                         (mnad.SNAP_RING_BF16, IMAGE_TOL)):
             sid = ctx.stack_create(g)
             gat = ctx.rtm_shots(sid, g, f["v"], f["vs"], f["w"], zs, xs, recs, snap_fmt=fmt,
                                 want_gathers=True)

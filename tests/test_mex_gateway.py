@@ -64,3 +64,29 @@ def test_valid_call_without_gpu_raises_nogpu(mex, mnad):
     with pytest.raises(MexError) as e:
         mex(*_fwd_args(f))
     assert e.value.ident == "mnad:nogpu"
+
+
+def test_batched_and_serving_argument_validation(mex):
+    """The batched and serving commands check their arguments before any
+    device work: receiver counts that do not add up, v and vs of different
+    sizes, a wrong arity."""
+    from mexgw import MexError
+    f = golden("rtm_two_shot")
+    dx, dz, dt, d = float(f["dx"]), float(f["dz"]), float(f["dt"]), int(f["npml"])
+    w = f["w"]
+    xs, zs = [int(f["xs0"]) + 1, int(f["xs1"]) + 1], [int(f["zs0"]) + 1, int(f["zs1"]) + 1]
+    xr = np.concatenate([f["xr0"], f["xr1"]]) + 1
+    zr = np.concatenate([f["zr0"], f["zr1"]]) + 1
+    with pytest.raises(MexError) as e:   # sum(nrec) != length(xr)
+        mex("fwd_shots", dx, dz, dt, d, f["v"], w, w.size, xs, zs, xr, zr, [3, 3], 0)
+    assert e.value.ident == "mnad:arg"
+    with pytest.raises(MexError) as e:   # vs of another size
+        mex("rtm_shots", dx, dz, dt, d, f["v"], f["vs"][:-1], w, w.size, xs, zs, xr, zr,
+            [f["xr0"].size, f["xr1"].size], 0)
+    assert e.value.ident == "mnad:arg"
+    with pytest.raises(MexError) as e:
+        mex("submit", 0, dx, dz, dt)
+    assert e.value.ident == "mnad:arg"
+    with pytest.raises(MexError) as e:
+        mex("wait", 1)
+    assert e.value.ident == "mnad:arg"

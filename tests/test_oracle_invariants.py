@@ -98,6 +98,82 @@ def test_cpml_coefficients_off_strip_and_profile(oracle):
     assert np.all(np.diff(l3[:, :npml], axis=1) < 0)        # strongest at the edge
 
 
+@pytest.mark.parametrize("R", [1e-4, 1e-3])
+def test_cpml_coefficient_values_from_the_profile(oracle, R):
+    """C_cof.m's three coefficients at every strip point of every side are
+    pinned by what the damping profile must satisfy, not by retyping it:
+      - c1 = (1 - s)/(1 + s) and c3 = 2 s/(1 + s) with s = sigma dt / 2, so
+        c1 + c3 = 1 exactly;
+      - c2 / c3 = sigma' / sigma = 2 / x for the quadratic profile
+        sigma = sigma0 x^2, with x the distance into the layer counted from
+        the interior (one cell h at the first strip cell, D = npml h at the
+        outer one, C_cof.m:6,38);
+      - the profile absorbs to the target reflection R for a normal-incidence
+        wave: (2 / v) sum_k sigma(x_k) h = ln(1/R) (2 n^3 + 3 n^2 + n) / (2 n^3),
+        the discrete form of (2/v) int_0^D sigma dx = ln(1/R) with n = npml
+        (sigma0 = 3 v ln(1/R) / (2 D^3)).
+    A wrong power, scale, sign of x or an off-by-one strip index fails one."""
+    nz, nx, npml = 26, 34, 7
+    dx, dz, dt = 11.0, 9.0, 1e-3
+    g = oracle.grid(nz, nx, npml, dx, dz, dt)
+    rng = np.random.default_rng(3)
+    v = 1500.0 + 2000.0 * rng.random((nz, nx))
+    n = npml
+    target = np.log(1 / R) * (2 * n**3 + 3 * n**2 + n) / (2 * n**3)
+    for side in range(4):
+        c1, c2, c3 = oracle.cpml_coeffs(g, v, side, R=R)
+        h = dx if side < 2 else dz
+        # distance into the layer of each strip cell, and the strip's cells
+        if side == 0:
+            x = (n - np.arange(n)) * h
+            sl = lambda a: a[:, :n]                         # noqa: E731
+        elif side == 1:
+            x = (np.arange(n) + 1) * h
+            sl = lambda a: a[:, nx - n:]                    # noqa: E731
+        elif side == 2:
+            x = (n - np.arange(n)) * h
+            sl = lambda a: a[:n, :].T                       # noqa: E731
+        else:
+            x = (np.arange(n) + 1) * h
+            sl = lambda a: a[nz - n:, :].T                  # noqa: E731
+        a1, a2, a3 = sl(c1), sl(c2), sl(c3)
+        assert np.allclose(a1 + a3, 1.0, rtol=0, atol=4e-16)
+        assert np.allclose(a2 * x[None, :], 2.0 * a3, rtol=1e-13, atol=0)
+        sigma = a3 / (dt * (1.0 - a3 / 2.0))                 # c3 = sigma dt / (1 + sigma dt / 2)
+        vv = sl(v)
+        assert np.allclose((2.0 / vv * sigma * h).sum(axis=1), target, rtol=1e-12)
+
+
+def test_arrival_times_follow_the_velocity(oracle):
+    """In a homogeneous 2000 m/s medium the wave reaches receivers 300, 600
+    and 900 m from the source along x and along z with lags (r2 - r1) / v
+    between them (cross-correlation peak of the traces, refined to a
+    fraction of a sample): a wrong velocity scale, a swapped axis or a
+    wrong dx/dz in the operators moves them.  Dispersion at 12 points per
+    dominant wavelength keeps the error near 3e-4 (bound 2e-3)."""
+    nz, nx, npml = 221, 221, 20
+    dx, dt, v0, fc, nt = 10.0, 1e-3, 2000.0, 15.0, 800
+    g = oracle.grid(nz, nx, npml, dx, dx, dt)
+    v = np.full((nz, nx), v0)
+    c = nz // 2
+    offs = np.array([30, 60, 90])
+    zr = np.concatenate([np.full(3, c), c + offs])
+    xr = np.concatenate([c + offs, np.full(3, c)])
+    r = oracle.forward(g, v, _ricker(fc, dt, nt), c, c, zr, xr, want_state=False)["gather"]
+
+    def lag(a, b):
+        cc = np.correlate(b, a, mode="full")
+        k = int(np.argmax(cc))
+        y0, y1, y2 = cc[k - 1], cc[k], cc[k + 1]
+        return (k - (a.size - 1)) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)
+
+    for base in (0, 3):   # along x, along z
+        for q in (1, 2):
+            want = (offs[q] - offs[0]) * dx / v0 / dt
+            got = lag(r[:, base], r[:, base + q])
+            assert abs(got - want) < 2e-3 * want, (base, q, got, want)
+
+
 def _ricker(fc, dt, nt):
     t = np.arange(nt) * dt
     tau = np.pi * (t - 1.1 / fc) * fc
