@@ -59,6 +59,10 @@ static bool tuning_ok(int kernel, int rows, int brows)
                (rows == 7 && (brows == 8 || brows == 16));
     if (kernel == 9) return brows == 8 && rows >= 1 && rows <= 4;   // rows: blocks per CU
     if (kernel == 11) return brows == 1 && rows == 2;                // two rows per wavefront
+    if (kernel == 12)   // z-marching two-step strips: segments of block_rows * rows rows
+        return (brows == 4 || brows == 8) && rows >= 2 && rows <= 512;
+    if (kernel == 13)   // register-queue two-step strips: segments of 8 * rows rows
+        return (brows == 1 || brows == 2 || brows == 4) && rows >= 1 && rows <= 256;
     return false;
 }
 
@@ -130,6 +134,12 @@ void mnad_ctx_destroy(mnad_ctx* ctx)
         ctx->models.clear();
         release_batch_cache(ctx);
         ctx->pool.trim();
+        if (ctx->side) {
+            (void)hipStreamSynchronize(ctx->side);
+            (void)hipStreamDestroy(ctx->side);
+            (void)hipEventDestroy(ctx->fork);
+            (void)hipEventDestroy(ctx->join);
+        }
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

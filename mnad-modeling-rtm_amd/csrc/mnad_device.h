@@ -90,6 +90,15 @@ struct ShotDev {
 
 static_assert(offsetof(ShotDev, rcol) + sizeof(void*) <= 192, "hot descriptor fields");
 
+// Step kernel 13: a tile of `w` columns from x0 and `seg` rows from zA runs the
+// interior path when every load of its march lies in the grid (four cells of
+// halo) and every level-s+1 halo point (two cells) lies outside the CPML.
+MNAD_HD inline bool q_interior_tile(int nx, int nz, int npml, int x0, int zA, int w, int seg)
+{
+    const int bw = npml + 2 > 4 ? npml + 2 : 4;
+    return x0 >= bw && x0 + w <= nx - bw && zA >= bw && zA + seg <= nz - bw;
+}
+
 struct TileRef {
     int shot;   // -1: padding entry, the block exits
     int tile;   // tile index within the shot: tj * ntx + ti

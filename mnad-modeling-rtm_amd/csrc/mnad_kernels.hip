@@ -1517,16 +1517,19 @@ __global__ __launch_bounds__(BX * RES_NW) void k_res(const ShotDev* __restrict__
 // multiple of NC, so every row load is aligned to its size.
 constexpr int X4W = 4 * (BX - 2);   // columns updated per wave, four columns per lane
 
+// bound_ctrl set: a lane whose source lane is outside the wave reads 0, so
+// the shift needs no preloaded "old" value (one instruction, and the DPP can
+// fold into the instruction that uses it)
 __device__ __forceinline__ float lane_prev(float x)   // lane l-1's x (0 in lane 0)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x),
-                                                                 0x138, 0xf, 0xf, false));
+                                                                 0x138, 0xf, 0xf, true));
 }
 
 __device__ __forceinline__ float lane_next(float x)   // lane l+1's x (0 in lane 63)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x),
-                                                                 0x130, 0xf, 0xf, false));
+                                                                 0x130, 0xf, 0xf, true));
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -2224,6 +2227,769 @@ __global__ __launch_bounds__(BX * NW) void k_step2w(const ShotDev* __restrict__ 
     }
 }
 
+// ------------------------------------ two steps per launch, marching down z (kernel 12)
+// Snapshot / imaging work of step s (snap_image_w), decided once per block
+// instead of once per row: kind 0 none, 1 forward d2u (illumination, and the
+// snapshot when the shot keeps them), 2 adjoint imaging, 3 reconstruction
+// hand-off slot.  `plane` is the snapshot (or slot) plane of step s.
+struct SnapStep {
+    int kind;
+    void* plane;
+};
+
+__device__ __forceinline__ SnapStep snap_step(const ShotDev& d, int s)
+{
+    SnapStep o{0, nullptr};
+    if (d.T <= 0) return o;
+    const long ni = (long)(d.nz - 2 * d.npml) * (d.nx - 2 * d.npml);
+    const bool f32 = d.sfmt == SNAP_F32 || d.sfmt == SNAP_SLOT;
+    if (d.mode == MODE_REC) {
+        if ((d.ntf - s) % d.T == 0) o = {3, d.snaps};
+    } else if (d.mode == MODE_FWD) {
+        if ((s + 1) % d.T == 0) {
+            const long k0 = (long)((s + 1) / d.T - 1) * ni;
+            o.kind = 1;
+            if (d.snaps)
+                o.plane = f32 ? (void*)((float*)d.snaps + k0) : (void*)((unsigned short*)d.snaps + k0);
+        }
+    } else {
+        const int tt = d.nt - s;
+        if (tt % d.T == 0) {
+            const int col = d.shift + tt / d.T;
+            if (col <= d.K) {
+                const long k0 = d.sfmt == SNAP_SLOT ? 0L : (long)(col - 1) * ni;
+                o = {2, f32 ? (void*)((float*)d.snaps + k0) : (void*)((unsigned short*)d.snaps + k0)};
+            }
+        }
+    }
+    return o;
+}
+
+// snap_image_w's per-point work for a step decided by snap_step
+__device__ __forceinline__ void snap_apply(const ShotDev& d, const SnapStep& st, long qrow,
+                                           unsigned q, float du, float nu)
+{
+    const bool f32 = d.sfmt == SNAP_F32 || d.sfmt == SNAP_SLOT, f16 = d.sfmt == SNAP_F16;
+    if (st.kind == 3) {
+        (gp(static_cast<float*>(st.plane)) + qrow)[q] = du * d.inv_dt2;
+    } else if (st.kind == 1) {
+        const float d2u = du * d.inv_dt2;
+        gfloat* il = gp(d.illum) + qrow;
+        il[q] += d2u * d2u;
+        if (st.plane) {
+            if (f32) {
+                (gp(static_cast<float*>(st.plane)) + qrow)[q] = d2u;
+            } else {
+                const unsigned short h16 = __half_as_ushort(__float2half(d2u * d.sscale));
+                const unsigned short b16 = __bfloat16_as_ushort(__float2bfloat16(d2u));
+                ((gu16*)st.plane + qrow)[q] = f16 ? h16 : b16;
+            }
+        }
+    } else if (st.kind == 2) {
+        float sn;
+        if (f32) {
+            sn = (gp(static_cast<const float*>(st.plane)) + qrow)[q];
+        } else {
+            const unsigned short h = ((const gu16*)st.plane + qrow)[q];
+            sn = f16 ? __half2float(__ushort_as_half(h)) * d.sinv
+                     : __bfloat162float(__ushort_as_bfloat16(h));
+        }
+        const float x = sn * nu;
+        gfloat* im = gp(d.img) + qrow;
+        gfloat* ic = gp(d.imgc) + qrow;
+        const float sum = im[q], c = ic[q];
+        const float y = x - c;
+        const float t = sum + y;
+        ic[q] = (t - sum) - y;
+        im[q] = t;
+    }
+}
+
+// A workgroup of NW wavefronts owns a strip of X2W = 60 output columns and a
+// segment of `seg` rows, and marches down the segment in chunks of NW rows,
+// two time steps per launch.  Rolling LDS windows hold the rows the next
+// chunk still needs, so each level-s row is loaded from HBM once per strip
+// and each level-s+1 row computed once: only the strip's x edges are
+// recomputed (level s+1 on 64 columns for 60 outputs, level s staged on 68),
+// and only 4 halo rows per segment.  Per chunk z0:
+//   stage level s rows z0+4 .. z0+NW+3   (one row per wavefront),
+//   level s+1 on rows z0+2 .. z0+NW+1     (needs level s rows z0 .. z0+NW+3),
+//   level s+2 on rows z0 .. z0+NW-1       (needs level s+1 rows z0-2 .. z0+NW+1
+//                                          and level s rows z0-1 .. z0+NW).
+// Hence a level-s window of NW+5 rows and a level-s+1 window of NW+4.  The
+// global loads of the next chunk are issued after the staging barrier and
+// land while the two updates run.  The per-point arithmetic, CPML memories
+// (two sets, as kernel 7), injections, receivers and snapshots/imaging are
+// kernel 7's.  Everything that is uniform over a launch (array bases with
+// 32-bit row offsets, the snapshot decision of each step, source amplitudes)
+// is computed once per workgroup; rows outside the CPML skip its code.
+// Time level t lives in f[t % 4].
+template <int NW>
+__global__ __launch_bounds__(BX * NW) void k_step2z(const ShotDev* __restrict__ shots,
+                                                    const TileRef* __restrict__ tiles,
+                                                    const int* __restrict__ base, int rel, int seg)
+{
+    constexpr int R0 = NW + 5, R1 = NW + 4, W0 = BX + 4;
+    static_assert(NW >= 4, "the prologue's four level-s+1 rows take one wavefront each");
+    __shared__ float L0[3 * R0 * W0];   // level s, columns x0-4 .. x0+63
+    __shared__ float L1[3 * R1 * BX];   // level s+1, columns x0-2 .. x0+61
+    __shared__ float LR[R1 * BX];       // rx of the level-s+1 rows
+    __shared__ int LK[R1 * BX];         // receiver slot of the level-s+1 rows
+
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    // The descriptor is re-read at the start of every phase: the empty asm
+    // makes the shot index opaque (readfirstlane keeps it uniform, and the
+    // base stays the read-only kernel argument, so the reads stay scalar
+    // loads), and its 36 stencil weights and field bases are then a few wide
+    // scalar loads per phase rather than loop invariants held in scalar
+    // registers across the march (which spill).
+    int shq = tr.shot;
+    auto refresh = [&]() {
+        asm volatile("" : "+s"(shq));
+        shq = __builtin_amdgcn_readfirstlane(shq);
+    };
+#define d (shots[shq])
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;   // uniform over the block
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int x0 = ti * X2W, zA = tj * seg;
+    const int nx = d.nx, nz = d.nz, np = d.npml;
+    const int zB = min(zA + seg, nz);
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const unsigned lane = threadIdx.x;
+    const unsigned P = (unsigned)d.pitch;
+    const bool second = s + 1 < d.nt;
+    const int pin = (s >> 1) & 1;
+    const int i = x0 - 2 + (int)lane;   // this lane's column (level s+1 and s+2)
+    const bool colin = i >= 0 && i < nx;
+    const bool xint = i >= np && i < nx - np;             // interior column
+    const bool xedge = x0 - 2 < np || x0 + 62 > nx - np;  // the strip meets an x CPML strip
+    const bool fwd = d.mode == MODE_FWD;
+    // array bases at (row 0, column x0-4) and (row 0, column x0-2); a row adds j * pitch
+    const unsigned o4 = (unsigned)MNAD_ZO * P + (unsigned)(x0 - 4 + MNAD_XO), o2 = o4 + 2;
+    auto F0 = [&](int f) { return (const gfloat*)gp(d.f[s & 3][f]) + o4; };        // level s
+    auto FM = [&](int f) { return (const gfloat*)gp(d.f[(s + 3) & 3][f]) + o2; };  // level s-1
+    auto F1 = [&](int f) { return gp(d.f[(s + 1) & 3][f]) + o2; };                 // level s+1
+    auto F2 = [&](int f) { return gp(d.f[(s + 2) & 3][f]) + o2; };                 // level s+2
+    const float amp1 = fwd && d.src_j >= 0 ? d.src_amp[s + 1] : 0.0f;
+    const float amp2 = fwd && d.src_j >= 0 && second ? d.src_amp[s + 2] : 0.0f;
+    gfloat* tr1 = fwd ? gp(d.gather) + (long)s * d.nrec : nullptr;          // gather row of step s
+    const gfloat* ra1 = fwd ? nullptr : gp(d.radj) + (long)s * d.nrec;     // adjoint injection
+    const SnapStep sn1 = snap_step(d, s), sn2 = snap_step(d, s + 1);
+    const long qw = nx - 2 * np;
+    const unsigned qi = (unsigned)(i - np);
+    auto slot0 = [&](int j) { return (j - zA + 5) % R0; };   // j >= zA - 5
+    auto slot1 = [&](int j) { return (j - zA + 2) % R1; };   // j >= zA - 2
+    // level-s value of field f, row j, at this lane's column (+ offsets)
+    auto l0 = [&](int f, int j) { return L0 + (f * R0 + slot0(j)) * W0 + lane + 2; };
+    auto l1 = [&](int f, int j) { return L1 + (f * R1 + slot1(j)) * BX + lane; };
+
+    // one level-s row of the strip: loaded into registers, then stored to LDS
+    struct Row0 {
+        float a[3], e[3];
+    };
+    auto fetch0 = [&](int j) {
+        Row0 o{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        if (j >= 0 && j < nz) {
+            const unsigned r = (unsigned)j * P;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const gfloat* src = F0(f) + r;
+                o.a[f] = src[lane];
+                if (lane < 4) o.e[f] = src[BX + lane];
+            }
+        }
+        return o;
+    };
+    auto put0 = [&](int j, const Row0& o) {
+        const int sl = slot0(j);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            L0[(f * R0 + sl) * W0 + lane] = o.a[f];
+            if (lane < 4) L0[(f * R0 + sl) * W0 + BX + lane] = o.e[f];
+        }
+    };
+
+    // per-point global inputs of a level-s+1 row (rx, level s-1, receiver slot)
+    struct In1 {
+        float rx, pu, pv, pw;
+        int r;
+    };
+    auto load1 = [&](int j) {
+        In1 a{0.0f, 0.0f, 0.0f, 0.0f, -1};
+        if (j >= 0 && j < nz) {
+            const unsigned r = (unsigned)j * P;
+            if (colin) {
+                a.rx = (gp(d.rx) + o2)[r + lane];
+                a.pu = FM(0)[r + lane];
+                a.pv = FM(1)[r + lane];
+                a.pw = FM(2)[r + lane];
+            }
+            if (d.nrec > 0) {
+                const int off = __builtin_amdgcn_readfirstlane(d.rrow[j]);
+                if (off >= 0 && colin) a.r = uni(gp(d.rcol) + off + x0 - 2)[lane];
+            }
+        }
+        return a;
+    };
+
+    // level s+1 at row j (a row of the segment is written out; rows outside it
+    // are the halo the level-s+2 update needs)
+    auto phase1 = [&](int j, const In1& in) {
+        float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+        const float rx = in.rx;
+        const int r = in.r;
+        const bool rowin = j >= 0 && j < nz;
+        if (rowin && colin) {
+            Nbh u, v, w;
+            const float *un = l0(0, j - 1), *uc = l0(0, j), *us = l0(0, j + 1);
+            const float *vn = l0(1, j - 1), *vc = l0(1, j), *vs = l0(1, j + 1);
+            const float *wn = l0(2, j - 1), *wc = l0(2, j), *ws = l0(2, j + 1);
+            u.nw = un[-1]; u.n = un[0]; u.ne = un[1]; u.w = uc[-1]; u.c = uc[0]; u.e = uc[1];
+            u.sw = us[-1]; u.s = us[0]; u.se = us[1];
+            v.nw = vn[-1]; v.n = vn[0]; v.ne = vn[1]; v.w = vc[-1]; v.c = vc[0]; v.e = vc[1];
+            v.sw = vs[-1]; v.s = vs[0]; v.se = vs[1];
+            w.nw = wn[-1]; w.n = wn[0]; w.ne = wn[1]; w.w = wc[-1]; w.c = wc[0]; w.e = wc[1];
+            w.sw = ws[-1]; w.s = ws[0]; w.se = ws[1];
+            const float vww = vc[-2], vee = vc[2];
+            const float wnn = l0(2, j - 2)[0], wss = l0(2, j + 2)[0];
+            float du, dv, dw;
+            mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+            const bool rown = j >= zA && j < zB;
+            const bool own = rown && lane >= 2 && lane < BX - 2;
+            const bool zedge = j < np || j >= nz - np;
+            if (xedge || zedge) {
+                const PsiPt q = psi_point(d, j, i, x0 - 2, lane);
+                if (q.inx || q.inz) {
+                    const float rz = rx * d.kap;
+                    float a_c = u.c, a_n = u.n, a_s = u.s, a_e = u.e, a_w = u.w;
+                    pre_injection(d, s, j, i, a_c, a_n, a_s, a_e, a_w);
+                    const bool keep = own && !second;   // a one-step launch stores the s+1 set
+                    if (q.inx) {
+                        PsiW m;
+                        psi_load(m, gp(d.psx) + pin * 9 * q.nsx + q.rowx, gp(d.cfx) + q.rowx, q.nsx, q.bx);
+                        psi_step(m, rx, a_w, a_c, a_e, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+                        if (keep) psi_store(m, gp(d.psx) + (pin ^ 1) * 9 * q.nsx + q.rowx, q.nsx, q.bx);
+                    }
+                    if (q.inz) {
+                        PsiW m;
+                        psi_load(m, gp(d.psz) + pin * 9 * q.nsz + q.rowz, gp(d.cfz) + q.rowz, q.nsz, q.bz);
+                        psi_step(m, rz, a_n, a_c, a_s, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+                        if (keep) psi_store(m, gp(d.psz) + (pin ^ 1) * 9 * q.nsz + q.rowz, q.nsz, q.bz);
+                    }
+                }
+            }
+            n0 = (2.0f * u.c + du) - in.pu;
+            n1 = (2.0f * v.c + dv) - in.pv;
+            n2 = (2.0f * w.c + dw) - in.pw;
+            if (fwd) {
+                if (j == d.src_j && i == d.src_i) n0 += amp1;
+                if (own && r >= 0) tr1[(unsigned)r] = u.c;
+            } else if (r >= 0) {
+                n0 -= ra1[(unsigned)r];
+            }
+            if (own) {
+                const unsigned o = (unsigned)j * P + lane;
+                F1(0)[o] = n0;
+                F1(1)[o] = n1;
+                F1(2)[o] = n2;
+                if (sn1.kind && !zedge && xint) snap_apply(d, sn1, (long)(j - np) * qw, qi, du, n0);
+            }
+        }
+        const int sl = slot1(j);
+        l1(0, j)[0] = n0;
+        l1(1, j)[0] = n1;
+        l1(2, j)[0] = n2;
+        LR[sl * BX + lane] = rx;
+        LK[sl * BX + lane] = r;
+    };
+
+    // level s+2 at row j of the segment
+    auto phase2 = [&](int j) {
+        if (j >= zB || lane < 2 || lane >= BX - 2 || !colin) return;
+        Nbh u, v, w;
+        const float *un = l1(0, j - 1), *uc = l1(0, j), *us = l1(0, j + 1);
+        const float *vn = l1(1, j - 1), *vc = l1(1, j), *vs = l1(1, j + 1);
+        const float *wn = l1(2, j - 1), *wc = l1(2, j), *ws = l1(2, j + 1);
+        u.nw = un[-1]; u.n = un[0]; u.ne = un[1]; u.w = uc[-1]; u.c = uc[0]; u.e = uc[1];
+        u.sw = us[-1]; u.s = us[0]; u.se = us[1];
+        v.nw = vn[-1]; v.n = vn[0]; v.ne = vn[1]; v.w = vc[-1]; v.c = vc[0]; v.e = vc[1];
+        v.sw = vs[-1]; v.s = vs[0]; v.se = vs[1];
+        w.nw = wn[-1]; w.n = wn[0]; w.ne = wn[1]; w.w = wc[-1]; w.c = wc[0]; w.e = wc[1];
+        w.sw = ws[-1]; w.s = ws[0]; w.se = ws[1];
+        const float vww = vc[-2], vee = vc[2];
+        const float wnn = l1(2, j - 2)[0], wss = l1(2, j + 2)[0];
+        const int sl = slot1(j);
+        const float rx = LR[sl * BX + lane];
+        const int r = LK[sl * BX + lane];
+        float du, dv, dw;
+        mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+        const float *c0 = l0(0, j), *cv = l0(1, j), *cw = l0(2, j);   // level s at (j, i)
+        const bool zedge = j < np || j >= nz - np;
+        if (xedge || zedge) {
+            const PsiPt q = psi_point(d, j, i, x0 - 2, lane);
+            if (q.inx || q.inz) {
+                const float rz = rx * d.kap;
+                // level s neighbourhood along the axes (staged), pre-injection at step s
+                float a_c = c0[0], a_n = l0(0, j - 1)[0], a_s = l0(0, j + 1)[0], a_e = c0[1],
+                      a_w = c0[-1];
+                pre_injection(d, s, j, i, a_c, a_n, a_s, a_e, a_w);
+                float uc1 = u.c, un1 = u.n, us1 = u.s, ue1 = u.e, uw1 = u.w;
+                pre_injection(d, s + 1, j, i, uc1, un1, us1, ue1, uw1);
+                float t0, t1, t2;   // discarded increments of the repeated level s+1 update
+                if (q.inx) {
+                    PsiW m;
+                    psi_load(m, gp(d.psx) + pin * 9 * q.nsx + q.rowx, gp(d.cfx) + q.rowx, q.nsx, q.bx);
+                    t0 = t1 = t2 = 0.0f;
+                    psi_step(m, rx, a_w, a_c, a_e, cv[-1], cv[0], cv[1], cw[-1], cw[0], cw[1], t0, t1, t2);
+                    psi_step(m, rx, uw1, uc1, ue1, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+                    psi_store(m, gp(d.psx) + (pin ^ 1) * 9 * q.nsx + q.rowx, q.nsx, q.bx);
+                }
+                if (q.inz) {
+                    PsiW m;
+                    psi_load(m, gp(d.psz) + pin * 9 * q.nsz + q.rowz, gp(d.cfz) + q.rowz, q.nsz, q.bz);
+                    t0 = t1 = t2 = 0.0f;
+                    psi_step(m, rz, a_n, a_c, a_s, l0(1, j - 1)[0], cv[0], l0(1, j + 1)[0],
+                             l0(2, j - 1)[0], cw[0], l0(2, j + 1)[0], t0, t1, t2);
+                    psi_step(m, rz, un1, uc1, us1, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+                    psi_store(m, gp(d.psz) + (pin ^ 1) * 9 * q.nsz + q.rowz, q.nsz, q.bz);
+                }
+            }
+        }
+        float n0 = (2.0f * u.c + du) - c0[0];
+        const float n1 = (2.0f * v.c + dv) - cv[0];
+        const float n2 = (2.0f * w.c + dw) - cw[0];
+        if (fwd) {
+            if (j == d.src_j && i == d.src_i) n0 += amp2;
+            if (r >= 0) (tr1 + d.nrec)[(unsigned)r] = u.c;
+        } else if (r >= 0) {
+            n0 -= (ra1 + d.nrec)[(unsigned)r];
+        }
+        const unsigned o = (unsigned)j * P + lane;
+        F2(0)[o] = n0;
+        F2(1)[o] = n1;
+        F2(2)[o] = n2;
+        if (sn2.kind && !zedge && xint) snap_apply(d, sn2, (long)(j - np) * qw, qi, du, n0);
+    };
+
+    // prologue: level s rows zA-4 .. zA+3, level s+1 rows zA-2 .. zA+1
+    {
+        const In1 in = load1(zA - 2 + wv);
+        for (int q = wv; q < 8; q += NW) put0(zA - 4 + q, fetch0(zA - 4 + q));
+        __syncthreads();
+        if (wv < 4) phase1(zA - 2 + wv, in);
+        __syncthreads();
+    }
+    In1 in = load1(zA + 2 + wv);
+    Row0 st = fetch0(zA + 4 + wv);
+    for (int z0 = zA; z0 < zB; z0 += NW) {
+        put0(z0 + 4 + wv, st);
+        __syncthreads();
+        refresh();
+        const In1 cur = in;
+        if (z0 + NW < zB) {
+            in = load1(z0 + NW + 2 + wv);
+            st = fetch0(z0 + NW + 4 + wv);
+        }
+        phase1(z0 + 2 + wv, cur);
+        __syncthreads();
+        refresh();
+        if (second) phase2(z0 + wv);
+        __syncthreads();
+    }
+#undef d
+}
+
+// ------------------------------------ two steps per launch, register queues (kernel 13)
+// Each wavefront owns XQ = 56 output columns: lane l holds column x0 - 4 + l,
+// so level s is known on 64 columns, level s+1 on the 60 of lanes 2..61 and
+// level s+2 on the 56 of lanes 4..59.  x neighbours come from the adjacent
+// lanes (DPP wave shifts), z neighbours from per-lane register queues: the
+// wavefront marches down its segment of rows [zA, zB) and at march step t
+//   - has level s row r = zA - 4 + t (loaded QD steps ahead),
+//   - updates level s+1 on row a = r - 2 (level s rows a-2 .. a+2),
+//   - updates level s+2 on row b = r - 4 (level s+1 rows b-2 .. b+2, level s row b).
+// Every queue (level s, level s+1, rx, level s-1) is a ring of QP = 5 + QD
+// rows in which row j sits in slot (j - zA + 4) mod QP, and the march is
+// unrolled by QP, so that every slot is a fixed register (a ring holds only
+// the rows still live, so the registers follow the live rows, not QP).  No
+// LDS and no barriers: the waves of a workgroup are independent.  Tiles whose
+// box reaches within max(npml + 2, 4) cells of the grid edge take the checked
+// path with the CPML code (both CPML memory sets as kernel 7: phase 1 reads set
+// pin everywhere, phase 2 re-runs the level-s memory step of its own points
+// and writes set pin ^ 1); the others are interior tiles whose every load is in
+// the grid and whose level s+1 halo points lie outside the CPML, and which run
+// without bounds or CPML code (it runs one step body with the rings rotated
+// by one slot after each step).  Injections, receivers, snapshots/imaging and
+// the point arithmetic are kernel 7's.  Time level t lives in f[t % 4].
+constexpr int XQ = BX - 8;
+constexpr int QD = 2;        // prefetch distance, in march steps
+constexpr int QP = 5 + QD;   // ring length
+
+__host__ __device__ constexpr int qslot(int ph, int off) { return ((ph + off) % QP + QP) % QP; }
+
+// Buffer access to a field plane of `bytes` bytes: the plane's resource in
+// scalar registers, the row (and the wave's first column) as a uniform byte
+// offset `4 off`, the lane as its byte offset vo = 4 lane; no per-lane
+// address arithmetic.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t qres(const void* p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float qld(const float* p, unsigned bytes, unsigned off, unsigned vo)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(qres(p, bytes), (int)vo,
+                                                                           (int)(4u * off), 0));
+}
+
+__device__ __forceinline__ void qst(float* p, unsigned bytes, unsigned off, unsigned vo, float x)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), qres(p, bytes), (int)vo,
+                                          (int)(4u * off), 0);
+}
+
+struct March {
+    float q0[3][QP];   // level s (U, Vs, Ws)
+    float q1[3][QP];   // level s+1
+    float qr[QP];      // rx
+    float qp[3][QP];   // level s-1
+};
+
+// Level s+1 at row a (phase 1), step t with PH = t mod QP: level s row a + k
+// in slot qslot(PH, k - 2); rx, level s-1 and the new level s+1 of row a in
+// slot qslot(PH, -2).
+template <bool EDGE, int PH>
+__device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int a, int zA, int zB,
+                                         int i, unsigned lane, unsigned xo, unsigned vo, unsigned pb, bool colin, bool ownc,
+                                         bool xint, bool xedge, const SnapStep& sn1, float amp1,
+                                         bool second)
+{
+    constexpr int sM2 = qslot(PH, -4), sM1 = qslot(PH, -3), sC = qslot(PH, -2), sP1 = qslot(PH, -1),
+                  sP2 = qslot(PH, 0);
+    const int nz = d.nz, np = d.npml;
+    Nbh u, v, w;
+    u.n = m.q0[0][sM1]; u.c = m.q0[0][sC]; u.s = m.q0[0][sP1];
+    v.n = m.q0[1][sM1]; v.c = m.q0[1][sC]; v.s = m.q0[1][sP1];
+    w.n = m.q0[2][sM1]; w.c = m.q0[2][sC]; w.s = m.q0[2][sP1];
+    u.nw = lane_prev(u.n); u.ne = lane_next(u.n); u.w = lane_prev(u.c); u.e = lane_next(u.c);
+    u.sw = lane_prev(u.s); u.se = lane_next(u.s);
+    v.nw = lane_prev(v.n); v.ne = lane_next(v.n); v.w = lane_prev(v.c); v.e = lane_next(v.c);
+    v.sw = lane_prev(v.s); v.se = lane_next(v.s);
+    w.nw = lane_prev(w.n); w.ne = lane_next(w.n); w.w = lane_prev(w.c); w.e = lane_next(w.c);
+    w.sw = lane_prev(w.s); w.se = lane_next(w.s);
+    const float vww = lane_prev(v.w), vee = lane_next(v.e);
+    const float wnn = m.q0[2][sM2], wss = m.q0[2][sP2];
+    const float rx = m.qr[sC];
+    float du, dv, dw;
+    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    const bool rowin = !EDGE || (a >= 0 && a < nz);
+    const bool own = ownc && a >= zA && a < zB;
+    bool zedge = false;
+    if (EDGE) {
+        zedge = a < np || a >= nz - np;
+        if ((xedge || zedge) && rowin) {
+            const PsiPt q = psi_point(d, a, i, i - (int)lane, lane);
+            if (colin && (q.inx || q.inz)) {
+                const int pin = (s >> 1) & 1;
+                const float rz = rx * d.kap;
+                float a_c = u.c, a_n = u.n, a_s = u.s, a_e = u.e, a_w = u.w;
+                pre_injection(d, s, a, i, a_c, a_n, a_s, a_e, a_w);
+                const bool keep = own && !second;   // a one-step launch stores the s+1 set
+                if (q.inx) {
+                    PsiW mm;
+                    psi_load(mm, gp(d.psx) + pin * 9 * q.nsx + q.rowx, gp(d.cfx) + q.rowx, q.nsx, q.bx);
+                    psi_step(mm, rx, a_w, a_c, a_e, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+                    if (keep) psi_store(mm, gp(d.psx) + (pin ^ 1) * 9 * q.nsx + q.rowx, q.nsx, q.bx);
+                }
+                if (q.inz) {
+                    PsiW mm;
+                    psi_load(mm, gp(d.psz) + pin * 9 * q.nsz + q.rowz, gp(d.cfz) + q.rowz, q.nsz, q.bz);
+                    psi_step(mm, rz, a_n, a_c, a_s, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+                    if (keep) psi_store(mm, gp(d.psz) + (pin ^ 1) * 9 * q.nsz + q.rowz, q.nsz, q.bz);
+                }
+            }
+        }
+    }
+    float n0 = (2.0f * u.c + du) - m.qp[0][sC];
+    float n1 = (2.0f * v.c + dv) - m.qp[1][sC];
+    float n2 = (2.0f * w.c + dw) - m.qp[2][sC];
+    const bool fwd = d.mode == MODE_FWD;
+    if (fwd && a == d.src_j && i == d.src_i) n0 += amp1;
+    if (d.nrec > 0 && rowin) {
+        const int off = __builtin_amdgcn_readfirstlane(d.rrow[a]);
+        if (off >= 0 && colin) {
+            const int r = uni(gp(d.rcol) + off)[(unsigned)i];
+            if (r >= 0) {
+                if (fwd) {
+                    if (own) (gp(d.gather) + (long)s * d.nrec)[(unsigned)r] = u.c;
+                } else {
+                    n0 -= (gp(d.radj) + (long)s * d.nrec)[(unsigned)r];
+                }
+            }
+        }
+    }
+    if (EDGE && !(rowin && colin)) n0 = n1 = n2 = 0.0f;
+    if (own) {
+        const unsigned o = xo + (unsigned)a * (unsigned)d.pitch;
+        qst(d.f[(s + 1) & 3][0], pb, o, vo, n0);
+        qst(d.f[(s + 1) & 3][1], pb, o, vo, n1);
+        qst(d.f[(s + 1) & 3][2], pb, o, vo, n2);
+        if (sn1.kind && !zedge && xint)
+            snap_apply(d, sn1, (long)(a - np) * (d.nx - 2 * np), (unsigned)(i - np), du, n0);
+    }
+    m.q1[0][sC] = n0;
+    m.q1[1][sC] = n1;
+    m.q1[2][sC] = n2;
+}
+
+// Level s+2 at row b = a - 2 (phase 2), step t with PH = t mod QP: level s+1
+// row b + k in slot qslot(PH, k - 4), rx and level s of row b in slot
+// qslot(PH, -4).  bm0..bm2: level s row b - 1 (its slot already holds the
+// prefetched row; the edge path's repeated CPML step reads it).
+template <bool EDGE, int PH>
+__device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int b, int i,
+                                         unsigned lane, unsigned xo, unsigned vo, unsigned pb, bool ownc, bool xint,
+                                         bool xedge, const SnapStep& sn2, float amp2, float bm0,
+                                         float bm1, float bm2)
+{
+    constexpr int tM2 = qslot(PH, -6), tM1 = qslot(PH, -5), tC = qslot(PH, -4), tP1 = qslot(PH, -3),
+                  tP2 = qslot(PH, -2);
+    const int nz = d.nz, np = d.npml;
+    Nbh u, v, w;
+    u.n = m.q1[0][tM1]; u.c = m.q1[0][tC]; u.s = m.q1[0][tP1];
+    v.n = m.q1[1][tM1]; v.c = m.q1[1][tC]; v.s = m.q1[1][tP1];
+    w.n = m.q1[2][tM1]; w.c = m.q1[2][tC]; w.s = m.q1[2][tP1];
+    u.nw = lane_prev(u.n); u.ne = lane_next(u.n); u.w = lane_prev(u.c); u.e = lane_next(u.c);
+    u.sw = lane_prev(u.s); u.se = lane_next(u.s);
+    v.nw = lane_prev(v.n); v.ne = lane_next(v.n); v.w = lane_prev(v.c); v.e = lane_next(v.c);
+    v.sw = lane_prev(v.s); v.se = lane_next(v.s);
+    w.nw = lane_prev(w.n); w.ne = lane_next(w.n); w.w = lane_prev(w.c); w.e = lane_next(w.c);
+    w.sw = lane_prev(w.s); w.se = lane_next(w.s);
+    const float vww = lane_prev(v.w), vee = lane_next(v.e);
+    const float wnn = m.q1[2][tM2], wss = m.q1[2][tP2];
+    const float rx = m.qr[tC];
+    const float c0 = m.q0[0][tC], cv = m.q0[1][tC], cw = m.q0[2][tC];   // level s at (b, i)
+    float du, dv, dw;
+    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    bool zedge = false;
+    if (EDGE) {
+        zedge = b < np || b >= nz - np;
+        if (xedge || zedge) {
+            // level s along the axes at (b, i), x neighbours while every lane is active
+            const float c0e = lane_next(c0), c0w = lane_prev(c0), cve = lane_next(cv),
+                        cvw = lane_prev(cv), cwe = lane_next(cw), cww = lane_prev(cw);
+            const PsiPt q = psi_point(d, b, i, i - (int)lane, lane);
+            if (ownc && (q.inx || q.inz)) {
+                const int pin = (s >> 1) & 1;
+                const float rz = rx * d.kap;
+                float a_c = c0, a_n = bm0, a_s = m.q0[0][tP1], a_e = c0e, a_w = c0w;
+                pre_injection(d, s, b, i, a_c, a_n, a_s, a_e, a_w);
+                float uc1 = u.c, un1 = u.n, us1 = u.s, ue1 = u.e, uw1 = u.w;
+                pre_injection(d, s + 1, b, i, uc1, un1, us1, ue1, uw1);
+                float t0, t1, t2;   // discarded increments of the repeated level s+1 update
+                if (q.inx) {
+                    PsiW mm;
+                    psi_load(mm, gp(d.psx) + pin * 9 * q.nsx + q.rowx, gp(d.cfx) + q.rowx, q.nsx, q.bx);
+                    t0 = t1 = t2 = 0.0f;
+                    psi_step(mm, rx, a_w, a_c, a_e, cvw, cv, cve, cww, cw, cwe, t0, t1, t2);
+                    psi_step(mm, rx, uw1, uc1, ue1, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
+                    psi_store(mm, gp(d.psx) + (pin ^ 1) * 9 * q.nsx + q.rowx, q.nsx, q.bx);
+                }
+                if (q.inz) {
+                    PsiW mm;
+                    psi_load(mm, gp(d.psz) + pin * 9 * q.nsz + q.rowz, gp(d.cfz) + q.rowz, q.nsz, q.bz);
+                    t0 = t1 = t2 = 0.0f;
+                    psi_step(mm, rz, a_n, a_c, a_s, bm1, cv, m.q0[1][tP1], bm2, cw, m.q0[2][tP1],
+                             t0, t1, t2);
+                    psi_step(mm, rz, un1, uc1, us1, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
+                    psi_store(mm, gp(d.psz) + (pin ^ 1) * 9 * q.nsz + q.rowz, q.nsz, q.bz);
+                }
+            }
+        }
+    }
+    float n0 = (2.0f * u.c + du) - c0;
+    const float n1 = (2.0f * v.c + dv) - cv;
+    const float n2 = (2.0f * w.c + dw) - cw;
+    const bool fwd = d.mode == MODE_FWD;
+    if (fwd && b == d.src_j && i == d.src_i) n0 += amp2;
+    if (d.nrec > 0) {
+        const int off = __builtin_amdgcn_readfirstlane(d.rrow[b]);
+        if (off >= 0 && ownc) {
+            const int r = uni(gp(d.rcol) + off)[(unsigned)i];
+            if (r >= 0) {
+                if (fwd) (gp(d.gather) + (long)(s + 1) * d.nrec)[(unsigned)r] = u.c;
+                else n0 -= (gp(d.radj) + (long)(s + 1) * d.nrec)[(unsigned)r];
+            }
+        }
+    }
+    if (ownc) {
+        const unsigned o = xo + (unsigned)b * (unsigned)d.pitch;
+        qst(d.f[(s + 2) & 3][0], pb, o, vo, n0);
+        qst(d.f[(s + 2) & 3][1], pb, o, vo, n1);
+        qst(d.f[(s + 2) & 3][2], pb, o, vo, n2);
+        if (sn2.kind && !zedge && xint)
+            snap_apply(d, sn2, (long)(b - np) * (d.nx - 2 * np), (unsigned)(i - np), du, n0);
+    }
+}
+
+// Level s row j into slot sl (zeros off the grid on the edge path).
+template <bool EDGE>
+__device__ __forceinline__ void q_load0(const ShotDev& d, March& m, int s, int j, int sl, unsigned xo,
+                                        unsigned vo, unsigned pb)
+{
+    if (!EDGE || (j >= 0 && j < d.nz)) {
+        const unsigned o = xo + (unsigned)j * (unsigned)d.pitch;
+        m.q0[0][sl] = qld(d.f[s & 3][0], pb, o, vo);
+        m.q0[1][sl] = qld(d.f[s & 3][1], pb, o, vo);
+        m.q0[2][sl] = qld(d.f[s & 3][2], pb, o, vo);
+    } else {
+        m.q0[0][sl] = m.q0[1][sl] = m.q0[2][sl] = 0.0f;
+    }
+}
+
+// Loads of march step t (PH = t mod QP): level s row r + QD, and rx and level
+// s-1 of row r - 2 + QD (the level-s+1 row of step t + QD).
+template <bool EDGE, int PH>
+__device__ __forceinline__ void q_loads(const ShotDev& d, March& m, int s, int r, int t, int NR,
+                                        unsigned xo, unsigned vo, unsigned pb, bool colin)
+{
+    if (t + QD < NR) {
+        q_load0<EDGE>(d, m, s, r + QD, qslot(PH, QD), xo, vo, pb);
+        if (t + QD >= 4) {
+            const int a = r - 2 + QD;
+            constexpr int sl = qslot(PH, QD - 2);
+            if ((!EDGE || (a >= 0 && a < d.nz)) && colin) {
+                const unsigned o = xo + (unsigned)a * (unsigned)d.pitch;
+                m.qr[sl] = qld(d.rx, pb, o, vo);
+                m.qp[0][sl] = qld(d.f[(s + 3) & 3][0], pb, o, vo);
+                m.qp[1][sl] = qld(d.f[(s + 3) & 3][1], pb, o, vo);
+                m.qp[2][sl] = qld(d.f[(s + 3) & 3][2], pb, o, vo);
+            } else {
+                m.qr[sl] = m.qp[0][sl] = m.qp[1][sl] = m.qp[2][sl] = 0.0f;
+            }
+        }
+    }
+}
+
+// March step t with the slots of phase PH = t mod QP.
+template <bool EDGE, int PH>
+__device__ __forceinline__ void q_step(const ShotDev& d, March& m, int s, int t, int NR, int zA,
+                                       int zB, int i, unsigned lane, unsigned xo, unsigned vo, unsigned pb, bool colin,
+                                       bool ownc, bool xint, bool xedge, const SnapStep& sn1,
+                                       const SnapStep& sn2, float amp1, float amp2, bool second)
+{
+    const int r = zA - 4 + t;
+    // level s row r - 5 = b - 1, whose slot the load below refills (edge path only)
+    constexpr int sb = qslot(PH, QD);
+    const float bm0 = EDGE ? m.q0[0][sb] : 0.0f, bm1 = EDGE ? m.q0[1][sb] : 0.0f,
+                bm2 = EDGE ? m.q0[2][sb] : 0.0f;
+    q_loads<EDGE, PH>(d, m, s, r, t, NR, xo, vo, pb, colin);
+    if (t >= 4)
+        q_phase1<EDGE, PH>(d, m, s, r - 2, zA, zB, i, lane, xo, vo, pb, colin, ownc, xint, xedge, sn1,
+                           amp1, second);
+    if (t >= 8 && second)
+        q_phase2<EDGE, PH>(d, m, s, r - 4, i, lane, xo, vo, pb, ownc, xint, xedge, sn2, amp2, bm0, bm1,
+                           bm2);
+}
+
+template <bool EDGE>
+__device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA, int zB, unsigned lane)
+{
+    const int nx = d.nx, np = d.npml;
+    const int i = x0 - 4 + (int)lane;
+    const bool colin = !EDGE || (i >= 0 && i < nx);
+    const bool ownc = lane >= 4 && lane < BX - 4 && (!EDGE || i < nx);
+    const bool xint = !EDGE || (i >= np && i < nx - np);
+    const bool xedge = EDGE && (x0 - 4 < np || x0 + XQ + 4 > nx - np);
+    const bool second = s + 1 < d.nt;
+    const bool fwd = d.mode == MODE_FWD;
+    // (0, x0 - 4) as a uniform element offset, and the lane's byte offset
+    const unsigned xo = (unsigned)MNAD_ZO * (unsigned)d.pitch + (unsigned)(x0 - 4 + MNAD_XO);
+    const unsigned vo = 4u * lane;
+    const unsigned pb = 4u * (unsigned)d.pitch * (unsigned)(d.nz + 2 * MNAD_ZO + MNAD_ZPAD);   // plane bytes
+    const float amp1 = fwd && d.src_j >= 0 ? d.src_amp[s + 1] : 0.0f;
+    const float amp2 = fwd && d.src_j >= 0 && second ? d.src_amp[s + 2] : 0.0f;
+    const SnapStep sn1 = snap_step(d, s), sn2 = snap_step(d, s + 1);
+    const int NR = zB - zA + 8;   // march steps: level s rows zA-4 .. zB+3
+    March m;
+    // level s rows zA-4 .. zA-4+QD-1 (the first QD steps' rows)
+#pragma unroll
+    for (int k = 0; k < QD; ++k) q_load0<EDGE>(d, m, s, zA - 4 + k, k, xo, vo, pb);
+#define MNAD_QSTEP(PH) q_step<EDGE, PH>(d, m, s, t + PH, NR, zA, zB, i, lane, xo, vo, pb, colin, ownc, xint, \
+                                        xedge, sn1, sn2, amp1, amp2, second)
+    if (!EDGE) {
+        // unrolled by QP = 7: the slots are fixed registers
+        static_assert(QP == 7, "the unrolled march below is written for QP = 7");
+        for (int t = 0; t < NR; t += QP) {
+            MNAD_QSTEP(0);
+            if (t + 1 >= NR) break;
+            MNAD_QSTEP(1);
+            if (t + 2 >= NR) break;
+            MNAD_QSTEP(2);
+            if (t + 3 >= NR) break;
+            MNAD_QSTEP(3);
+            if (t + 4 >= NR) break;
+            MNAD_QSTEP(4);
+            if (t + 5 >= NR) break;
+            MNAD_QSTEP(5);
+            if (t + 6 >= NR) break;
+            MNAD_QSTEP(6);
+        }
+    } else {
+        // one step body (PH = 0) with the rings rotated by one slot after each step
+        for (int t = 0; t < NR; ++t) {
+            MNAD_QSTEP(0);
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const float a0 = m.q0[f][0], b0 = m.q1[f][0], p0 = m.qp[f][0];
+#pragma unroll
+                for (int k = 0; k < QP - 1; ++k) {
+                    m.q0[f][k] = m.q0[f][k + 1];
+                    m.q1[f][k] = m.q1[f][k + 1];
+                    m.qp[f][k] = m.qp[f][k + 1];
+                }
+                m.q0[f][QP - 1] = a0;
+                m.q1[f][QP - 1] = b0;
+                m.qp[f][QP - 1] = p0;
+            }
+            const float r0 = m.qr[0];
+#pragma unroll
+            for (int k = 0; k < QP - 1; ++k) m.qr[k] = m.qr[k + 1];
+            m.qr[QP - 1] = r0;
+        }
+    }
+#undef MNAD_QSTEP
+}
+
+// Interior tiles (EDGE false) and edge tiles (EDGE true) run as two launches
+// of one step: the CPML code of the edge path would otherwise set the
+// register allocation of both.  The host orders the tile table interior tiles
+// first (q_interior_tile decides); an interior launch skips a tile that does
+// not pass the same test.
+template <int NW, bool EDGE>
+__global__ __launch_bounds__(BX * NW) void k_step2q(const ShotDev* __restrict__ shots,
+                                                    const TileRef* __restrict__ tiles,
+                                                    const int* __restrict__ base, int rel, int seg)
+{
+    const TileRef tr = tiles[blockIdx.x];
+    if (tr.shot < 0) return;
+    const ShotDev& d = shots[tr.shot];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;   // uniform over the block
+    const int tj = tr.tile / d.ntx;
+    const int ti = tr.tile - tj * d.ntx;
+    const int nx = d.nx, nz = d.nz;
+    const int zA = tj * seg, zB = min(zA + seg, nz);
+    if (!EDGE && !q_interior_tile(nx, nz, d.npml, ti * NW * XQ, zA, NW * XQ, seg)) return;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int x0 = (ti * NW + wv) * XQ;
+    if (x0 >= nx) return;   // no barriers in this kernel: a wave may leave early
+    q_march<EDGE>(d, s, x0, zA, zB, threadIdx.x);
+}
+
 __global__ void k_advance(int* base, int n)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *base += n;
@@ -2369,6 +3135,10 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL((k_step2w<28, 16>), g, dim3(BX, 16), 0, st, shots, tiles, base, rel);
     else if (kernel == 3 && brows == 16 && rows == 1)
         hipLaunchKernelGGL(k_step2, g, dim3(X2, Z2), 0, st, shots, tiles, base, rel);
+    else if (kernel == 12 && brows == 4 && rows >= 1)
+        hipLaunchKernelGGL(k_step2z<4>, g, dim3(BX, 4), 0, st, shots, tiles, base, rel, 4 * rows);
+    else if (kernel == 12 && brows == 8 && rows >= 1)
+        hipLaunchKernelGGL(k_step2z<8>, g, dim3(BX, 8), 0, st, shots, tiles, base, rel, 8 * rows);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -2416,6 +3186,48 @@ hipError_t mnad_launch_resident(const ShotDev* shots, const ResBlock* blocks, in
     if (!k || nblocks <= 0 || spin < 0) return hipErrorInvalidValue;
     void* args[] = {&shots, &blocks, &ebuf, &err, &nsteps, &bh, &spin};
     return hipLaunchCooperativeKernel(k, dim3(nblocks), dim3(BX, RES_NW), args, res_lds(bh), st);
+}
+
+// Step kernel 13: tiles [0, ninterior) of the table are interior tiles, the
+// rest edge tiles (build_tiles orders them so); one launch for each part.
+// With a side stream the edge launch runs beside the interior one (fork on
+// event `fork`, join on `join`; both parts read only the levels before the
+// step and write disjoint points), so that its few long-running waves
+// overlap the interior instead of following it.  Under graph capture this
+// records the two launches as parallel nodes.
+hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ninterior,
+                              int ntiles, const int* base, int rel, int rows, int brows,
+                              hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join)
+{
+    if (ninterior < 0 || ninterior > ntiles || rows < 1) return hipErrorInvalidValue;
+    if (brows != 1 && brows != 2 && brows != 4) return hipErrorInvalidValue;
+    const int ne = ntiles - ninterior, seg = 8 * rows;
+    const bool par = side && fork && join && ne > 0 && ninterior > 0;
+    hipStream_t se = par ? side : st;
+    hipError_t e = hipSuccess;
+    if (par) {
+        if ((e = hipEventRecord(fork, st)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(side, fork, 0)) != hipSuccess) return e;
+    }
+#define MNAD_Q2(NW)                                                                          \
+    do {                                                                                     \
+        if (ne > 0)                                                                          \
+            hipLaunchKernelGGL((k_step2q<NW, true>), dim3(ne), dim3(BX, NW), 0, se, shots,   \
+                               tiles + ninterior, base, rel, seg);                           \
+        if (ninterior > 0)                                                                   \
+            hipLaunchKernelGGL((k_step2q<NW, false>), dim3(ninterior), dim3(BX, NW), 0, st,   \
+                               shots, tiles, base, rel, seg);                                \
+    } while (0)
+    if (brows == 1) MNAD_Q2(1);
+    else if (brows == 2) MNAD_Q2(2);
+    else MNAD_Q2(4);
+#undef MNAD_Q2
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (par) {
+        if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(st, join, 0)) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st)

@@ -35,6 +35,9 @@ hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, i
 hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
                             const int* base, int rel, int kernel, int rows, int brows,
                             hipStream_t st);
+hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ninterior,
+                              int ntiles, const int* base, int rel, int rows, int brows,
+                              hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st);
 hipError_t mnad_launch_init_source(ShotDev* shots, int nshots, hipStream_t st);
 hipError_t mnad_launch_make_radj(const float* dobs, const float* dsyn, const float* arec,
@@ -681,6 +684,14 @@ void tile_shape(const mnad_ctx* c, int* tw, int* th, int* spl)
         *tw = 60;
         *th = 4 * c->rz;
         *spl = 2;
+    } else if (c->kernel == 12) {  // two steps per launch, z-marching: 60 x (block_rows x rows) segments
+        *tw = 60;
+        *th = c->brows * c->rz;
+        *spl = 2;
+    } else if (c->kernel == 13) {  // two steps per launch, register queues: block_rows strips of 56
+        *tw = 56 * c->brows;       // columns x segments of 8 x rows rows
+        *th = 8 * c->rz;
+        *spl = 2;
     } else {                // 64 columns x (block_rows * rows_per_thread) rows
                             // (kernel 6: the one-point tiles; see build_tiles)
         *tw = 64;
@@ -708,7 +719,8 @@ static void x4_layout(ShotDev& d, int nc, int nt1)
 
 int interior_columns_per_lane(int kernel) { return kernel == 6 ? 4 : (kernel == 10 ? 2 : 0); }
 
-void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table)
+void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table,
+                 int* ninterior)
 {
     std::vector<TileRef> list;
     std::vector<char> strip;
@@ -736,7 +748,10 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
             // a reverse reconstruction touches the interior and its two-cell ring only
             if (rec && (r1 <= np - 2 || r0 >= nz - np + 2 || c1 <= np - 2 || c0 >= nx - np + 2))
                 continue;
-            const bool st = !rec && (c0 < np || c1 > nx - np || r0 < np || r1 > nz - np);
+            // kernel 13: its edge tiles (q_interior_tile) stand in for the strip tiles
+            const bool st = c->kernel == 13
+                                ? !q_interior_tile(nx, nz, np, c0, r0, tw, th)
+                                : !rec && (c0 < np || c1 > nx - np || r0 < np || r1 > nz - np);
             list.push_back(TileRef{k, t});
             strip.push_back(st);
             work.push_back(st ? sw : 1.0);
@@ -762,23 +777,43 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
     // 561 x 1061 grid: 8.7 us by count, 7.8 us by work).  Within each run
     // the strip tiles go first.
     const int nx8 = 8;
-    double total = 0.0;
-    for (double w : work) total += w;
-    std::vector<std::vector<size_t>> runs(nx8);
-    double acc = 0.0;
+    // kernel 13 launches its interior and edge tiles separately: each part is
+    // ordered on its own and the interior part, padded to a multiple of 8,
+    // comes first
+    auto order = [&](const std::vector<size_t>& sel) {
+        double total = 0.0;
+        for (size_t q : sel) total += work[q];
+        std::vector<std::vector<size_t>> runs(nx8);
+        double acc = 0.0;
+        for (size_t q : sel) {
+            const double mid = acc + 0.5 * work[q];
+            acc += work[q];
+            runs[std::min(nx8 - 1, (int)(mid * nx8 / total))].push_back(q);
+        }
+        size_t per = 0;
+        for (const auto& run : runs) per = std::max(per, run.size());
+        const size_t at = table.size();
+        table.resize(at + per * nx8, TileRef{-1, 0});
+        for (int x = 0; x < nx8; ++x) {
+            auto& run = runs[x];
+            if (c->strip_first)
+                std::stable_partition(run.begin(), run.end(), [&](size_t q) { return strip[q] != 0; });
+            for (size_t m = 0; m < run.size(); ++m) table[at + m * nx8 + x] = list[run[m]];
+        }
+    };
+    table.clear();
+    std::vector<size_t> all, inner, edge;
     for (size_t q = 0; q < list.size(); ++q) {
-        const double mid = acc + 0.5 * work[q];
-        acc += work[q];
-        runs[std::min(nx8 - 1, (int)(mid * nx8 / total))].push_back(q);
+        all.push_back(q);
+        (strip[q] ? edge : inner).push_back(q);
     }
-    size_t per = 0;
-    for (const auto& run : runs) per = std::max(per, run.size());
-    table.assign(per * nx8, TileRef{-1, 0});
-    for (int x = 0; x < nx8; ++x) {
-        auto& run = runs[x];
-        if (c->strip_first)
-            std::stable_partition(run.begin(), run.end(), [&](size_t q) { return strip[q] != 0; });
-        for (size_t m = 0; m < run.size(); ++m) table[m * nx8 + x] = list[run[m]];
+    if (c->kernel == 13) {
+        if (!inner.empty()) order(inner);
+        if (ninterior) *ninterior = (int)table.size();
+        if (!edge.empty()) order(edge);
+    } else {
+        if (!all.empty()) order(all);
+        if (ninterior) *ninterior = 0;
     }
 }
 
@@ -954,7 +989,8 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         return run_resident(c, h, nsteps, bytes);
     }
     std::vector<TileRef> table;
-    build_tiles(c, h, table);
+    int ninner = 0;
+    build_tiles(c, h, table, &ninner);
     // descriptors in the context's persistent buffers, so that a cached graph
     // chunk of the same shape replays on them
     CHK(ensure_batch_buf(c, c->bdesc, h.size() * sizeof(ShotDev)));
@@ -982,16 +1018,26 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     int tw, th, spl;
     tile_shape(c, &tw, &th, &spl);
     const int nlaunch = (nsteps + spl - 1) / spl;   // a launch covers spl steps
+    if (kern == 13 && !c->side) {   // side stream of kernel 13's edge launches (default priority)
+        HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
+    }
+    auto step = [&](int rel) {
+        return kern == 13 ? mnad_launch_step2q(ds, dt, ninner, ntiles, base, rel, rz, br, c->stream,
+                                               c->side, c->fork, c->join)
+                          : mnad_launch_step(ds, dt, ntiles, base, rel, kern, rz, br, c->stream);
+    };
     if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nlaunch < 2 * chunk) {
         HIPCHK(hipEventRecord(ev0, c->stream));
-        for (int s = 0; s < nlaunch; ++s)
-            HIPCHK(mnad_launch_step(ds, dt, ntiles, base, s * spl, kern, rz, br, c->stream));
+        for (int s = 0; s < nlaunch; ++s) HIPCHK(step(s * spl));
     } else {
         hipGraphExec_t exec = nullptr;
         for (size_t k = 0; k < c->graphs.size(); ++k) {
             const GraphEntry& e = c->graphs[k];
             if (e.kern == kern && e.rz == rz && e.brows == br && e.ntiles == ntiles &&
-                e.chunk == chunk && e.spl == spl && e.ds == ds && e.dt == dt && e.base == base) {
+                e.ninner == ninner && e.chunk == chunk && e.spl == spl && e.ds == ds &&
+                e.dt == dt && e.base == base) {
                 exec = e.exec;
                 break;
             }
@@ -1001,7 +1047,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             hipError_t ce = hipSuccess;
             for (int r = 0; r < chunk && ce == hipSuccess; ++r)
-                ce = mnad_launch_step(ds, dt, ntiles, base, r * spl, kern, rz, br, c->stream);
+                ce = step(r * spl);
             if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk * spl, c->stream);
             hipError_t ee = hipStreamEndCapture(c->stream, &graph);
             HIPCHK(ce);
@@ -1016,13 +1062,13 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
                 (void)hipGraphDestroy(c->graphs.front().graph);
                 c->graphs.erase(c->graphs.begin());
             }
-            c->graphs.push_back(GraphEntry{kern, rz, br, ntiles, chunk, spl, ds, dt, base, graph, exec});
+            c->graphs.push_back(
+                GraphEntry{kern, rz, br, ntiles, ninner, chunk, spl, ds, dt, base, graph, exec});
         }
         const int full = nlaunch / chunk, rem = nlaunch - full * chunk;
         HIPCHK(hipEventRecord(ev0, c->stream));   // after instantiation
         for (int k = 0; k < full; ++k) HIPCHK(hipGraphLaunch(exec, c->stream));
-        for (int r = 0; r < rem; ++r)
-            HIPCHK(mnad_launch_step(ds, dt, ntiles, base, r * spl, kern, rz, br, c->stream));
+        for (int r = 0; r < rem; ++r) HIPCHK(step(r * spl));
     }
     HIPCHK(hipEventRecord(ev1, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

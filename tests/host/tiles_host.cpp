@@ -11,6 +11,11 @@
 //  - the estimated work per XCD differs by at most two tile weights, half a
 //    tile at either end of each XCD's range (two-step kernels: counts);
 //  - with strip_first, each XCD dispatches its CPML-strip tiles first;
+//  - kernel 13: the interior tiles come first, in a part whose length is a
+//    multiple of 8, and the edge tiles after it; each interior tile passes
+//    q_interior_tile, every load of its march (four cells around it) lies in
+//    the grid and every level-s+1 halo point (two cells) outside the CPML;
+//    each part on its own meets the XCD checks above;
 // and the block heights the resident kernel (9) picks for a few batches;
 // and the host pass over a velocity model (scan_model, model_digest):
 //  - the first value that is not positive and finite (0, -0, negative, NaN,
@@ -32,6 +37,11 @@ hipError_t mnad_launch_setup_model(const double*, int, int, int, int, double, do
                                    float*, float*, float*, hipStream_t) { return hipErrorNotSupported; }
 hipError_t mnad_launch_step(const mnad::ShotDev*, const mnad::TileRef*, int, const int*, int, int,
                             int, int, hipStream_t) { return hipErrorNotSupported; }
+hipError_t mnad_launch_step2q(const mnad::ShotDev*, const mnad::TileRef*, int, int, const int*, int,
+                              int, int, hipStream_t, hipStream_t, hipEvent_t, hipEvent_t)
+{
+    return hipErrorNotSupported;
+}
 hipError_t mnad_launch_advance(int*, int, hipStream_t) { return hipErrorNotSupported; }
 hipError_t mnad_launch_init_source(mnad::ShotDev*, int, hipStream_t) { return hipErrorNotSupported; }
 hipError_t mnad_launch_make_radj(const float*, const float*, const float*, int, int, float*,
@@ -83,13 +93,34 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
         h[k].npml = shapes[k].np;
     }
     std::vector<TileRef> table;
-    build_tiles(&c, h, table);
+    int ninner = -1;
+    build_tiles(&c, h, table, &ninner);
     int tw, th, spl;
     tile_shape(&c, &tw, &th, &spl);
     const int nc = interior_columns_per_lane(kernel);
-    const bool x4 = nc > 0, two = two_step_kernel(kernel);
+    const bool x4 = nc > 0, two = two_step_kernel(kernel), q13 = kernel == 13;
     char buf[256];
     if (table.size() % 8 != 0) return "table length not a multiple of 8";
+    if (q13 ? (ninner < 0 || ninner % 8 != 0 || ninner > (int)table.size()) : ninner != 0)
+        return "interior part length wrong";
+    if (q13) {
+        for (size_t b = 0; b < table.size(); ++b) {
+            const TileRef& t = table[b];
+            if (t.shot < 0) continue;
+            const Shape& s = shapes[t.shot];
+            const ShotDev& d = h[t.shot];
+            const int tj = t.tile / d.ntx, ti = t.tile - tj * d.ntx;
+            const int x0 = ti * tw, zA = tj * th;
+            const bool inner = q_interior_tile(s.nx, s.nz, s.np, x0, zA, tw, th);
+            if (inner != ((int)b < ninner)) return "a tile in the wrong part of the table";
+            if (!inner) continue;
+            if (x0 - 4 < 0 || x0 + tw + 4 > s.nx || zA - 4 < 0 || zA + th + 4 > s.nz)
+                return "an interior tile's march loads outside the grid";
+            if (x0 - 2 < s.np || x0 + tw + 2 > s.nx - s.np || zA - 2 < s.np ||
+                zA + th + 2 > s.nz - s.np)
+                return "an interior tile's level-s+1 halo meets the CPML";
+        }
+    }
 
     // coverage: every point exactly once
     std::vector<std::vector<int>> cover(shapes.size());
@@ -135,16 +166,26 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
             return c.x4_weight * nc / 4.0;
         }
         const int tj = t.tile / d.ntx, ti = t.tile - tj * d.ntx;
-        *strip = is_strip_tile(tj * th, std::min(s.nz, (tj + 1) * th), ti * tw,
-                               std::min(s.nx, (ti + 1) * tw), s);
+        *strip = q13 ? !q_interior_tile(s.nx, s.nz, s.np, ti * tw, tj * th, tw, th)
+                     : is_strip_tile(tj * th, std::min(s.nz, (tj + 1) * th), ti * tw,
+                                     std::min(s.nx, (ti + 1) * tw), s);
         return two ? 1.0 : (*strip ? c.strip_weight : 1.0);
     };
+    // kernel 13: the two parts are ordered separately
+    const std::vector<std::pair<size_t, size_t>> parts =
+        q13 ? std::vector<std::pair<size_t, size_t>>{{0, (size_t)ninner}, {(size_t)ninner, table.size()}}
+            : std::vector<std::pair<size_t, size_t>>{{0, table.size()}};
+    for (const auto& part : parts) {
+    std::vector<std::pair<int, int>> psorted;
+    for (size_t b = part.first; b < part.second; ++b)
+        if (table[b].shot >= 0) psorted.emplace_back(table[b].shot, table[b].tile);
+    std::sort(psorted.begin(), psorted.end());
     double wmax = 0.0, wlo = 1e300, whi = 0.0;
     for (int x = 0; x < 8; ++x) {
         std::vector<std::pair<int, int>> mine;
         double w = 0.0;
         bool seen_interior = false;
-        for (size_t b = x; b < table.size(); b += 8) {
+        for (size_t b = part.first + x; b < part.second; b += 8) {
             const TileRef& t = table[b];
             if (t.shot < 0) continue;
             bool strip = false;
@@ -161,8 +202,8 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
         whi = std::max(whi, w);
         if (mine.empty()) continue;
         std::sort(mine.begin(), mine.end());
-        const auto first = std::lower_bound(sorted.begin(), sorted.end(), mine.front());
-        if (first + (long)mine.size() > sorted.end() ||
+        const auto first = std::lower_bound(psorted.begin(), psorted.end(), mine.front());
+        if (first + (long)mine.size() > psorted.end() ||
             !std::equal(mine.begin(), mine.end(), first))
             return "an XCD's tiles are not a contiguous range";
     }
@@ -171,6 +212,7 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
         snprintf(buf, sizeof buf, "work per XCD from %.1f to %.1f (tile weight at most %.1f)", wlo,
                  whi, wmax);
         return buf;
+    }
     }
     return "";
 }
@@ -183,6 +225,7 @@ int main()
         std::vector<Shape> shapes;
     };
     const Shape c2{561, 1061, 30}, c1{241, 241, 20}, tiny{13, 13, 6}, nocpml{70, 300, 0};
+    const Shape c5w{1061, 561, 30};
     const std::vector<Case> cases = {
         {"k4_b1_c2", 4, 1, 1, 1, {c2}},
         {"k4_b4_c2x2", 4, 1, 4, 1, {c2, c2}},
@@ -197,6 +240,10 @@ int main()
         {"k7_r5_b8_c2x3", 7, 5, 8, 1, {c2, c2, c2}},
         {"k7_r3_b16_mixed", 7, 3, 16, 1, {c1, tiny, c2}},
         {"k3_c1", 3, 1, 16, 1, {c1}},
+        {"k12_r16_b4_mixed", 12, 16, 4, 1, {c2, c1, tiny}},
+        {"k13_r8_b1_c2", 13, 8, 1, 1, {c2}},
+        {"k13_r4_b4_mixed", 13, 4, 4, 1, {c1, c2, tiny, nocpml}},
+        {"k13_r2_b2_c5w", 13, 2, 2, 0, {c5w, c2}},
     };
     int bad = 0;
     // Block heights of the resident kernel (9): the smallest height >= 4 whose

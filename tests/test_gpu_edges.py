@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
 KERNELS = [(4, 1, 1), (4, 1, 4), (5, 1, 4), (6, 1, 2), (7, 3, 16), (9, 1, 8), (9, 4, 8),
-           (10, 1, 1), (10, 1, 4), (11, 2, 1)]
+           (10, 1, 1), (10, 1, 4), (11, 2, 1), (12, 2, 4), (12, 8, 8), (13, 1, 1), (13, 4, 4)]
+IDS = ["k%d-%d-%d" % k for k in KERNELS]
 
 
 def _say(msg):
@@ -58,7 +59,7 @@ def _compare(mnad, oracle, kernel, v, npml, w, zs, xs, zr, xr, dx=10.0, dz=10.0,
     return r
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
 def test_no_cpml(mnad, oracle, kernel):
     _say(f"edge: npml 0, kernel {kernel}")
     v = _model(70, 300, 1)
@@ -67,7 +68,7 @@ def test_no_cpml(mnad, oracle, kernel):
     _compare(mnad, oracle, kernel, v, 0, w, 35, 150, np.full(xr.size, 2), xr)
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
 @pytest.mark.parametrize("nt", [1, 2, 3])
 def test_shortest_runs(mnad, oracle, kernel, nt):
     _say(f"edge: nt {nt}, kernel {kernel}")
@@ -79,7 +80,7 @@ def test_shortest_runs(mnad, oracle, kernel, nt):
     _compare(mnad, oracle, kernel, v, 5, w, 12, 130, zr, xr)
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
 def test_no_receivers(mnad, oracle, kernel):
     _say(f"edge: no receivers, kernel {kernel}")
     v = _model(40, 270, 3)
@@ -88,7 +89,7 @@ def test_no_receivers(mnad, oracle, kernel):
     assert r["gather"].shape == (90, 0)
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
 @pytest.mark.parametrize("corner", ["nw", "se"])
 def test_source_on_grid_corner_and_boundary_receivers(mnad, oracle, kernel, corner):
     """Source at (0, 0) or (nz-1, nx-1), inside both CPML strips; receivers
@@ -104,7 +105,7 @@ def test_source_on_grid_corner_and_boundary_receivers(mnad, oracle, kernel, corn
     _compare(mnad, oracle, kernel, v, 6, w, zs, xs, zr, xr)
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
 def test_minimal_grid_for_cpml(mnad, oracle, kernel):
     """nz = nx = 2 npml + 1: a single interior point."""
     _say(f"edge: minimal grid, kernel {kernel}")
@@ -113,7 +114,8 @@ def test_minimal_grid_for_cpml(mnad, oracle, kernel):
     _compare(mnad, oracle, kernel, v, 6, w, 6, 6, [6, 0, 12], [6, 12, 0])
 
 
-@pytest.mark.parametrize("kernel,rz,br", [(7, 3, 16), (7, 5, 8), (3, 1, 16)])
+@pytest.mark.parametrize("kernel,rz,br", [(7, 3, 16), (7, 5, 8), (3, 1, 16), (12, 2, 4),
+                                          (12, 8, 8), (13, 1, 1), (13, 4, 4)])
 def test_two_step_kernels_odd_step_count_rtm(mnad, kernel, rz, br):
     """rtm_rect has nt = 187: the last launch of a two-step kernel runs one step
     (and stores the level s+1 CPML memories itself)."""
