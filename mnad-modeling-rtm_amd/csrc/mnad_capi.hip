@@ -271,7 +271,8 @@ int mnad_fwd(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, const d
             s.rrow.reset();
             s.rcol.reset();
             s.illum.reset();
-            for (int q = 6; q < 12; ++q) s.state[q].reset();
+            // (the field planes are one allocation: the spare levels of a
+            // two-step kernel's shot stay with it)
             k.fwd = sp_shot;
         }
         ctx->kept[ss] = std::move(k);
@@ -435,11 +436,14 @@ int ring_save(mnad_ctx* ctx, const Kept& k, const char* path)
     h.rord = s.d.rord; h.nring = s.d.nring; h.rfmt = s.d.rfmt;
     h.src_j = s.d.src_j; h.src_i = s.d.src_i; h.pitch = s.d.pitch;
     h.dx = k.dx; h.dz = k.dz; h.dt = k.dt; h.rsc = s.d.rsc; h.rcs = s.d.rcs;
-    h.plane_bytes = s.state[0].n; h.amp_bytes = s.amp.n; h.rq_bytes = s.rq.n; h.rst_bytes = s.rst.n;
+    h.plane_bytes = plane_stride_bytes(k.nz, s.d.pitch);
+    h.amp_bytes = s.amp.n; h.rq_bytes = s.rq.n; h.rst_bytes = s.rst.n;
     FILE* f = fopen(path, "wb");
     if (!f) return fail(MNAD_ERR_ARG, "cannot open %s for writing", path);
     int rc = fwrite(&h, sizeof(h), 1, f) == 1 ? MNAD_OK : fail(MNAD_ERR_ARG, "write to %s failed", path);
-    for (int q = 0; q < 6 && rc == MNAD_OK; ++q) rc = write_dev(ctx, f, s.state[q].p, s.state[q].n);
+    // levels 0 and 1 (U, Vs, Ws each), one plane after the other
+    for (int q = 0; q < 6 && rc == MNAD_OK; ++q)
+        rc = write_dev(ctx, f, s.d.f[q / 3][q % 3], h.plane_bytes);
     if (rc == MNAD_OK) rc = write_dev(ctx, f, s.amp.p, s.amp.n);
     if (rc == MNAD_OK) rc = write_dev(ctx, f, s.rq.p, s.rq.n);
     if (rc == MNAD_OK) rc = write_dev(ctx, f, s.rst.p, s.rst.n);
@@ -455,7 +459,7 @@ int ring_load(mnad_ctx* ctx, FILE* f, const char* path, Kept& k)
         return fail(MNAD_ERR_ARG, "%s is not a snapshot file", path);
     mnad_grid g{h.nz, h.nx, h.npml, h.dx, h.dz, h.dt};
     const int pitch = ((MNAD_XO + h.nx + 2) + 31) / 32 * 32;
-    const size_t plane = ((size_t)(h.nz + 2 * MNAD_ZO + MNAD_ZPAD) * pitch * 4 + 255) & ~(size_t)255;
+    const size_t plane = plane_stride_bytes(h.nz, pitch);
     const size_t nr = (h.npml >= 2 && h.nz > 2 * h.npml && h.nx > 2 * h.npml)
                           ? (size_t)ring_cells(h.nz, h.nx, h.npml) : 0;
     auto r256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -468,10 +472,11 @@ int ring_load(mnad_ctx* ctx, FILE* f, const char* path, Kept& k)
     auto s = std::make_shared<Shot>();
     base_desc(g, pitch, h.nt, MODE_FWD, 2, s->d);
     ShotDev& d = s->d;
+    CHK(dalloc(ctx, s->state[0], 6 * plane, false));   // one allocation, as make_shot's
+    d.pbytes = (unsigned)plane;
     for (int q = 0; q < 6; ++q) {
-        CHK(dalloc(ctx, s->state[q], plane, false));
-        CHK(read_dev(ctx, f, s->state[q].p, plane, path));
-        d.f[q / 3][q % 3] = s->state[q].as<float>();
+        d.f[q / 3][q % 3] = s->state[0].as<float>() + q * (plane / sizeof(float));
+        CHK(read_dev(ctx, f, d.f[q / 3][q % 3], plane, path));
     }
     CHK(dalloc(ctx, s->amp, h.amp_bytes, false));
     CHK(read_dev(ctx, f, s->amp.p, h.amp_bytes, path));

@@ -86,7 +86,18 @@ struct ShotDev {
     long long* rst;                 // [P][3][nring] decoded values, level L in slot L % P
     double rsc, rsi;                // fixed-point unit 2^-E: rsc = 2^E, rsi = 2^-E
     double rcs, rci;                // code scale (a power of two) and its inverse
+    int kap1;                       // dx == dz (kap exactly 1): ic equals incr_consts_of(1.0)
+    int rlo, rhi;                   // first and last row holding a receiver (rlo > rhi: none)
+    unsigned pbytes;                // field plane stride in bytes: plane 3 level + field at
+                                    // f[0][0] + that * pbytes (one allocation); 0 if not so
 };
+
+// Bytes from one field plane to the next in a shot's state allocation: the
+// plane rounded up to 256 bytes.
+MNAD_HD inline size_t plane_stride_bytes(int nz, int pitch)
+{
+    return ((size_t)(nz + 2 * MNAD_ZO + MNAD_ZPAD) * (size_t)pitch * 4 + 255) & ~(size_t)255;
+}
 
 static_assert(offsetof(ShotDev, rcol) + sizeof(void*) <= 192, "hot descriptor fields");
 

@@ -2630,25 +2630,50 @@ constexpr int QP = 5 + QD;   // ring length
 
 __host__ __device__ constexpr int qslot(int ph, int off) { return ((ph + off) % QP + QP) % QP; }
 
-// Buffer access to a field plane of `bytes` bytes: the plane's resource in
-// scalar registers, the row (and the wave's first column) as a uniform byte
-// offset `4 off`, the lane as its byte offset vo = 4 lane; no per-lane
-// address arithmetic.
+// Buffer access to the field planes: a shot's 3 nlev planes are one
+// allocation (plane 3 level + field at byte (3 level + field) * plane bytes),
+// so one resource in scalar registers covers all of them and rx has its own.
+// An access gives the level, row and wave as a uniform byte offset (soffset)
+// and the field and lane as a per-lane constant (voffset vo[f] = 4 lane + f
+// plane bytes), with no per-lane address arithmetic.  The values are read
+// from the descriptor once, before the march stores anything: the compiler
+// cannot tell that the buffer stores leave the descriptor alone, and would
+// otherwise read it again after each of them.
+struct QCtx {
+    __amdgpu_buffer_rsrc_t rs, rr;   // field planes, rx
+    unsigned pitch4;                 // row bytes
+    unsigned b0, bm, b1, b2, bx;     // (0, x0 - 4) of levels s, s-1, s+1, s+2 and of rx, in bytes
+    unsigned vo[3];                  // per lane: 4 lane + field * plane bytes
+};
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t qres(const void* p, unsigned bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
 }
 
-__device__ __forceinline__ float qld(const float* p, unsigned bytes, unsigned off, unsigned vo)
+__device__ __forceinline__ float qld(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned vo)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(qres(p, bytes), (int)vo,
-                                                                           (int)(4u * off), 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)off, 0));
 }
 
-__device__ __forceinline__ void qst(float* p, unsigned bytes, unsigned off, unsigned vo, float x)
+__device__ __forceinline__ void qst(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned vo, float x)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), qres(p, bytes), (int)vo,
-                                          (int)(4u * off), 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)vo, (int)off, 0);
+}
+
+// A lane offset that the buffer range check rejects: a store through it is
+// dropped.  The interior path masks its stores this way instead of branching
+// around them (every state allocation is under 2 GiB, the run_batch check).
+constexpr unsigned QOFF = 0x80000000u;
+
+// Receiver-row table entry j, read through the constant address space: the
+// table is never written during a launch, so the read is a scalar load,
+// which counts on lgkmcnt and does not make the wave wait for its in-flight
+// prefetches (vector loads complete in order, so waiting for a vector load
+// of the table would wait for all of them).
+__device__ __forceinline__ int q_rrow(const ShotDev& d, int j)
+{
+    return ((const __attribute__((address_space(4))) int*)d.rrow)[j];
 }
 
 struct March {
@@ -2661,9 +2686,9 @@ struct March {
 // Level s+1 at row a (phase 1), step t with PH = t mod QP: level s row a + k
 // in slot qslot(PH, k - 2); rx, level s-1 and the new level s+1 of row a in
 // slot qslot(PH, -2).
-template <bool EDGE, int PH>
+template <bool EDGE, bool K1, int PH, bool PURE = false>
 __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int a, int zA, int zB,
-                                         int i, unsigned lane, unsigned xo, unsigned vo, unsigned pb, bool colin, bool ownc,
+                                         int i, unsigned lane, const QCtx& qc, bool colin, bool ownc,
                                          bool xint, bool xedge, const SnapStep& sn1, float amp1,
                                          bool second)
 {
@@ -2684,7 +2709,12 @@ __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int 
     const float wnn = m.q0[2][sM2], wss = m.q0[2][sP2];
     const float rx = m.qr[sC];
     float du, dv, dw;
-    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    if constexpr (K1) {   // dx == dz: the weights as literals
+        constexpr IncrC c1 = incr_consts_of(1.0);
+        mnad_incr(c1, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    } else {
+        mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    }
     const bool rowin = !EDGE || (a >= 0 && a < nz);
     const bool own = ownc && a >= zA && a < zB;
     bool zedge = false;
@@ -2718,8 +2748,8 @@ __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int 
     float n2 = (2.0f * w.c + dw) - m.qp[2][sC];
     const bool fwd = d.mode == MODE_FWD;
     if (fwd && a == d.src_j && i == d.src_i) n0 += amp1;
-    if (d.nrec > 0 && rowin) {
-        const int off = __builtin_amdgcn_readfirstlane(d.rrow[a]);
+    if (!PURE && d.nrec > 0 && rowin) {
+        const int off = q_rrow(d, a);
         if (off >= 0 && colin) {
             const int r = uni(gp(d.rcol) + off)[(unsigned)i];
             if (r >= 0) {
@@ -2732,13 +2762,21 @@ __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int 
         }
     }
     if (EDGE && !(rowin && colin)) n0 = n1 = n2 = 0.0f;
-    if (own) {
-        const unsigned o = xo + (unsigned)a * (unsigned)d.pitch;
-        qst(d.f[(s + 1) & 3][0], pb, o, vo, n0);
-        qst(d.f[(s + 1) & 3][1], pb, o, vo, n1);
-        qst(d.f[(s + 1) & 3][2], pb, o, vo, n2);
-        if (sn1.kind && !zedge && xint)
-            snap_apply(d, sn1, (long)(a - np) * (d.nx - 2 * np), (unsigned)(i - np), du, n0);
+    if (EDGE) {
+        if (own) {
+            const unsigned o = qc.b1 + (unsigned)a * qc.pitch4;
+            qst(qc.rs, o, qc.vo[0], n0);
+            qst(qc.rs, o, qc.vo[1], n1);
+            qst(qc.rs, o, qc.vo[2], n2);
+        }
+    } else {   // no branch: the stores of lanes and rows that are not this wave's are dropped
+        const unsigned o = qc.b1 + (unsigned)a * qc.pitch4, msk = own ? 0u : QOFF;
+        qst(qc.rs, o, qc.vo[0] + msk, n0);
+        qst(qc.rs, o, qc.vo[1] + msk, n1);
+        qst(qc.rs, o, qc.vo[2] + msk, n2);
+    }
+    if (!PURE && sn1.kind && !zedge && xint && a >= zA && a < zB) {
+        if (ownc) snap_apply(d, sn1, (long)(a - np) * (d.nx - 2 * np), (unsigned)(i - np), du, n0);
     }
     m.q1[0][sC] = n0;
     m.q1[1][sC] = n1;
@@ -2749,9 +2787,9 @@ __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int 
 // row b + k in slot qslot(PH, k - 4), rx and level s of row b in slot
 // qslot(PH, -4).  bm0..bm2: level s row b - 1 (its slot already holds the
 // prefetched row; the edge path's repeated CPML step reads it).
-template <bool EDGE, int PH>
+template <bool EDGE, bool K1, int PH, bool PURE = false>
 __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int b, int i,
-                                         unsigned lane, unsigned xo, unsigned vo, unsigned pb, bool ownc, bool xint,
+                                         unsigned lane, const QCtx& qc, bool ownc, bool xint,
                                          bool xedge, const SnapStep& sn2, float amp2, float bm0,
                                          float bm1, float bm2)
 {
@@ -2773,7 +2811,12 @@ __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int 
     const float rx = m.qr[tC];
     const float c0 = m.q0[0][tC], cv = m.q0[1][tC], cw = m.q0[2][tC];   // level s at (b, i)
     float du, dv, dw;
-    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    if constexpr (K1) {   // dx == dz: the weights as literals
+        constexpr IncrC c1 = incr_consts_of(1.0);
+        mnad_incr(c1, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    } else {
+        mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
+    }
     bool zedge = false;
     if (EDGE) {
         zedge = b < np || b >= nz - np;
@@ -2815,8 +2858,8 @@ __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int 
     const float n2 = (2.0f * w.c + dw) - cw;
     const bool fwd = d.mode == MODE_FWD;
     if (fwd && b == d.src_j && i == d.src_i) n0 += amp2;
-    if (d.nrec > 0) {
-        const int off = __builtin_amdgcn_readfirstlane(d.rrow[b]);
+    if (!PURE && d.nrec > 0) {
+        const int off = q_rrow(d, b);
         if (off >= 0 && ownc) {
             const int r = uni(gp(d.rcol) + off)[(unsigned)i];
             if (r >= 0) {
@@ -2825,26 +2868,33 @@ __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int 
             }
         }
     }
-    if (ownc) {
-        const unsigned o = xo + (unsigned)b * (unsigned)d.pitch;
-        qst(d.f[(s + 2) & 3][0], pb, o, vo, n0);
-        qst(d.f[(s + 2) & 3][1], pb, o, vo, n1);
-        qst(d.f[(s + 2) & 3][2], pb, o, vo, n2);
-        if (sn2.kind && !zedge && xint)
-            snap_apply(d, sn2, (long)(b - np) * (d.nx - 2 * np), (unsigned)(i - np), du, n0);
+    if (EDGE) {
+        if (ownc) {
+            const unsigned o = qc.b2 + (unsigned)b * qc.pitch4;
+            qst(qc.rs, o, qc.vo[0], n0);
+            qst(qc.rs, o, qc.vo[1], n1);
+            qst(qc.rs, o, qc.vo[2], n2);
+        }
+    } else {   // no branch: the stores of lanes that are not this wave's are dropped
+        const unsigned o = qc.b2 + (unsigned)b * qc.pitch4, msk = ownc ? 0u : QOFF;
+        qst(qc.rs, o, qc.vo[0] + msk, n0);
+        qst(qc.rs, o, qc.vo[1] + msk, n1);
+        qst(qc.rs, o, qc.vo[2] + msk, n2);
+    }
+    if (!PURE && sn2.kind && !zedge && xint) {
+        if (ownc) snap_apply(d, sn2, (long)(b - np) * (d.nx - 2 * np), (unsigned)(i - np), du, n0);
     }
 }
 
 // Level s row j into slot sl (zeros off the grid on the edge path).
 template <bool EDGE>
-__device__ __forceinline__ void q_load0(const ShotDev& d, March& m, int s, int j, int sl, unsigned xo,
-                                        unsigned vo, unsigned pb)
+__device__ __forceinline__ void q_load0(const ShotDev& d, March& m, int s, int j, int sl, const QCtx& qc)
 {
     if (!EDGE || (j >= 0 && j < d.nz)) {
-        const unsigned o = xo + (unsigned)j * (unsigned)d.pitch;
-        m.q0[0][sl] = qld(d.f[s & 3][0], pb, o, vo);
-        m.q0[1][sl] = qld(d.f[s & 3][1], pb, o, vo);
-        m.q0[2][sl] = qld(d.f[s & 3][2], pb, o, vo);
+        const unsigned o = qc.b0 + (unsigned)j * qc.pitch4;
+        m.q0[0][sl] = qld(qc.rs, o, qc.vo[0]);
+        m.q0[1][sl] = qld(qc.rs, o, qc.vo[1]);
+        m.q0[2][sl] = qld(qc.rs, o, qc.vo[2]);
     } else {
         m.q0[0][sl] = m.q0[1][sl] = m.q0[2][sl] = 0.0f;
     }
@@ -2854,19 +2904,23 @@ __device__ __forceinline__ void q_load0(const ShotDev& d, March& m, int s, int j
 // s-1 of row r - 2 + QD (the level-s+1 row of step t + QD).
 template <bool EDGE, int PH>
 __device__ __forceinline__ void q_loads(const ShotDev& d, March& m, int s, int r, int t, int NR,
-                                        unsigned xo, unsigned vo, unsigned pb, bool colin)
+                                        const QCtx& qc, bool colin)
 {
-    if (t + QD < NR) {
-        q_load0<EDGE>(d, m, s, r + QD, qslot(PH, QD), xo, vo, pb);
-        if (t + QD >= 4) {
+    // The interior path loads without conditions (a branch around a load
+    // leaves the wait counts unknown after it): the rows past the segment's
+    // last are inside the plane's padding rows, and those before its first
+    // inside the grid; their values are not used.
+    if (!EDGE || t + QD < NR) {
+        q_load0<EDGE>(d, m, s, r + QD, qslot(PH, QD), qc);
+        if (!EDGE || t + QD >= 4) {
             const int a = r - 2 + QD;
             constexpr int sl = qslot(PH, QD - 2);
             if ((!EDGE || (a >= 0 && a < d.nz)) && colin) {
-                const unsigned o = xo + (unsigned)a * (unsigned)d.pitch;
-                m.qr[sl] = qld(d.rx, pb, o, vo);
-                m.qp[0][sl] = qld(d.f[(s + 3) & 3][0], pb, o, vo);
-                m.qp[1][sl] = qld(d.f[(s + 3) & 3][1], pb, o, vo);
-                m.qp[2][sl] = qld(d.f[(s + 3) & 3][2], pb, o, vo);
+                const unsigned o = (unsigned)a * qc.pitch4;
+                m.qr[sl] = qld(qc.rr, qc.bx + o, qc.vo[0]);
+                m.qp[0][sl] = qld(qc.rs, qc.bm + o, qc.vo[0]);
+                m.qp[1][sl] = qld(qc.rs, qc.bm + o, qc.vo[1]);
+                m.qp[2][sl] = qld(qc.rs, qc.bm + o, qc.vo[2]);
             } else {
                 m.qr[sl] = m.qp[0][sl] = m.qp[1][sl] = m.qp[2][sl] = 0.0f;
             }
@@ -2875,9 +2929,9 @@ __device__ __forceinline__ void q_loads(const ShotDev& d, March& m, int s, int r
 }
 
 // March step t with the slots of phase PH = t mod QP.
-template <bool EDGE, int PH>
+template <bool EDGE, bool K1, int PH>
 __device__ __forceinline__ void q_step(const ShotDev& d, March& m, int s, int t, int NR, int zA,
-                                       int zB, int i, unsigned lane, unsigned xo, unsigned vo, unsigned pb, bool colin,
+                                       int zB, int i, unsigned lane, const QCtx& qc, bool colin,
                                        bool ownc, bool xint, bool xedge, const SnapStep& sn1,
                                        const SnapStep& sn2, float amp1, float amp2, bool second)
 {
@@ -2886,16 +2940,33 @@ __device__ __forceinline__ void q_step(const ShotDev& d, March& m, int s, int t,
     constexpr int sb = qslot(PH, QD);
     const float bm0 = EDGE ? m.q0[0][sb] : 0.0f, bm1 = EDGE ? m.q0[1][sb] : 0.0f,
                 bm2 = EDGE ? m.q0[2][sb] : 0.0f;
-    q_loads<EDGE, PH>(d, m, s, r, t, NR, xo, vo, pb, colin);
+    q_loads<EDGE, PH>(d, m, s, r, t, NR, qc, colin);
     if (t >= 4)
-        q_phase1<EDGE, PH>(d, m, s, r - 2, zA, zB, i, lane, xo, vo, pb, colin, ownc, xint, xedge, sn1,
+        q_phase1<EDGE, K1, PH>(d, m, s, r - 2, zA, zB, i, lane, qc, colin, ownc, xint, xedge, sn1,
                            amp1, second);
     if (t >= 8 && second)
-        q_phase2<EDGE, PH>(d, m, s, r - 4, i, lane, xo, vo, pb, ownc, xint, xedge, sn2, amp2, bm0, bm1,
+        q_phase2<EDGE, K1, PH>(d, m, s, r - 4, i, lane, qc, ownc, xint, xedge, sn2, amp2, bm0, bm1,
                            bm2);
 }
 
-template <bool EDGE>
+// A main-loop step of the interior path (t >= 8, both phases, no receiver
+// rows near the segment, no snapshot or imaging work in the launch): loads,
+// phase 1 and phase 2 with no branch, so that the wait counts stay exact and
+// a step waits only for the loads issued QD steps earlier.
+template <bool K1, int PH>
+__device__ __forceinline__ void q_step_main(const ShotDev& d, March& m, int s, int t, int zA, int zB,
+                                            int i, unsigned lane, const QCtx& qc, bool ownc,
+                                            const SnapStep& sn, float amp1, float amp2)
+{
+    const int r = zA - 4 + t;
+    q_loads<false, PH>(d, m, s, r, t, 0, qc, true);
+    q_phase1<false, K1, PH, true>(d, m, s, r - 2, zA, zB, i, lane, qc, true, ownc, true, false, sn,
+                                  amp1, true);
+    q_phase2<false, K1, PH, true>(d, m, s, r - 4, i, lane, qc, ownc, true, false, sn, amp2, 0.0f,
+                                  0.0f, 0.0f);
+}
+
+template <bool EDGE, bool K1>
 __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA, int zB, unsigned lane)
 {
     const int nx = d.nx, np = d.npml;
@@ -2906,10 +2977,22 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
     const bool xedge = EDGE && (x0 - 4 < np || x0 + XQ + 4 > nx - np);
     const bool second = s + 1 < d.nt;
     const bool fwd = d.mode == MODE_FWD;
-    // (0, x0 - 4) as a uniform element offset, and the lane's byte offset
-    const unsigned xo = (unsigned)MNAD_ZO * (unsigned)d.pitch + (unsigned)(x0 - 4 + MNAD_XO);
-    const unsigned vo = 4u * lane;
-    const unsigned pb = 4u * (unsigned)d.pitch * (unsigned)(d.nz + 2 * MNAD_ZO + MNAD_ZPAD);   // plane bytes
+    QCtx qc;
+    {
+        const unsigned pb = d.pbytes;   // the host checks the planes are one allocation
+        const unsigned xo4 = 4u * ((unsigned)MNAD_ZO * (unsigned)d.pitch + (unsigned)(x0 - 4 + MNAD_XO));
+        qc.pitch4 = 4u * (unsigned)d.pitch;
+        qc.rs = qres(d.f[0][0], 3u * (unsigned)d.nlev * pb);
+        qc.rr = qres(d.rx, pb);
+        qc.b0 = 3u * (unsigned)(s & 3) * pb + xo4;
+        qc.bm = 3u * (unsigned)((s + 3) & 3) * pb + xo4;
+        qc.b1 = 3u * (unsigned)((s + 1) & 3) * pb + xo4;
+        qc.b2 = 3u * (unsigned)((s + 2) & 3) * pb + xo4;
+        qc.bx = xo4;
+        qc.vo[0] = 4u * lane;
+        qc.vo[1] = 4u * lane + pb;
+        qc.vo[2] = 4u * lane + 2u * pb;
+    }
     const float amp1 = fwd && d.src_j >= 0 ? d.src_amp[s + 1] : 0.0f;
     const float amp2 = fwd && d.src_j >= 0 && second ? d.src_amp[s + 2] : 0.0f;
     const SnapStep sn1 = snap_step(d, s), sn2 = snap_step(d, s + 1);
@@ -2917,29 +3000,50 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
     March m;
     // level s rows zA-4 .. zA-4+QD-1 (the first QD steps' rows)
 #pragma unroll
-    for (int k = 0; k < QD; ++k) q_load0<EDGE>(d, m, s, zA - 4 + k, k, xo, vo, pb);
-#define MNAD_QSTEP(PH) q_step<EDGE, PH>(d, m, s, t + PH, NR, zA, zB, i, lane, xo, vo, pb, colin, ownc, xint, \
+    for (int k = 0; k < QD; ++k) q_load0<EDGE>(d, m, s, zA - 4 + k, k, qc);
+#define MNAD_QSTEP(PH) q_step<EDGE, K1, PH>(d, m, s, t + PH, NR, zA, zB, i, lane, qc, colin, ownc, xint, \
                                         xedge, sn1, sn2, amp1, amp2, second)
-    if (!EDGE) {
-        // unrolled by QP = 7: the slots are fixed registers
+    // receiver rows among the rows the march updates (zA-2 .. zB+1)
+    const bool rcv = d.nrec > 0 && d.rlo <= zB + 1 && d.rhi >= zA - 2;
+    if (!EDGE && second && !sn1.kind && !sn2.kind && !rcv) {
+        // unrolled by QP = 7: the slots are fixed registers.  Steps 0 .. 7
+        // (loads, then phase 1 from step 4), then the branch-free main loop.
         static_assert(QP == 7, "the unrolled march below is written for QP = 7");
-        for (int t = 0; t < NR; t += QP) {
+        {
+            const int t = 0;
             MNAD_QSTEP(0);
-            if (t + 1 >= NR) break;
             MNAD_QSTEP(1);
-            if (t + 2 >= NR) break;
             MNAD_QSTEP(2);
-            if (t + 3 >= NR) break;
             MNAD_QSTEP(3);
-            if (t + 4 >= NR) break;
             MNAD_QSTEP(4);
-            if (t + 5 >= NR) break;
             MNAD_QSTEP(5);
-            if (t + 6 >= NR) break;
             MNAD_QSTEP(6);
         }
+        {
+            const int t = 7;
+            MNAD_QSTEP(0);
+        }
+#define MNAD_QMAIN(PH, T) q_step_main<K1, PH>(d, m, s, T, zA, zB, i, lane, qc, ownc, sn1, amp1, amp2)
+        for (int t = 8; t < NR; t += QP) {   // NR = seg + 8 >= 16
+            MNAD_QMAIN(1, t);
+            if (t + 1 >= NR) break;
+            MNAD_QMAIN(2, t + 1);
+            if (t + 2 >= NR) break;
+            MNAD_QMAIN(3, t + 2);
+            if (t + 3 >= NR) break;
+            MNAD_QMAIN(4, t + 3);
+            if (t + 4 >= NR) break;
+            MNAD_QMAIN(5, t + 4);
+            if (t + 5 >= NR) break;
+            MNAD_QMAIN(6, t + 5);
+            if (t + 6 >= NR) break;
+            MNAD_QMAIN(0, t + 6);
+        }
+#undef MNAD_QMAIN
     } else {
-        // one step body (PH = 0) with the rings rotated by one slot after each step
+        // One step body (PH = 0) with the rings rotated by one slot after each
+        // step: the edge path, and interior blocks with receiver rows,
+        // snapshot or imaging work, or a last launch of one step.
         for (int t = 0; t < NR; ++t) {
             MNAD_QSTEP(0);
 #pragma unroll
@@ -2966,10 +3070,12 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
 
 // Interior tiles (EDGE false) and edge tiles (EDGE true) run as two launches
 // of one step: the CPML code of the edge path would otherwise set the
-// register allocation of both.  The host orders the tile table interior tiles
+// register allocation of both.  K1 (all shots of the batch have dx == dz):
+// the interior launch takes the stencil weights as literals, which frees the
+// scalar registers the 36 weights otherwise hold.  The host orders the tile table interior tiles
 // first (q_interior_tile decides); an interior launch skips a tile that does
 // not pass the same test.
-template <int NW, bool EDGE>
+template <int NW, bool EDGE, bool K1>
 __global__ __launch_bounds__(BX * NW) void k_step2q(const ShotDev* __restrict__ shots,
                                                     const TileRef* __restrict__ tiles,
                                                     const int* __restrict__ base, int rel, int seg)
@@ -2987,7 +3093,7 @@ __global__ __launch_bounds__(BX * NW) void k_step2q(const ShotDev* __restrict__ 
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int x0 = (ti * NW + wv) * XQ;
     if (x0 >= nx) return;   // no barriers in this kernel: a wave may leave early
-    q_march<EDGE>(d, s, x0, zA, zB, threadIdx.x);
+    q_march<EDGE, K1>(d, s, x0, zA, zB, threadIdx.x);
 }
 
 __global__ void k_advance(int* base, int n)
@@ -3190,13 +3296,14 @@ hipError_t mnad_launch_resident(const ShotDev* shots, const ResBlock* blocks, in
 
 // Step kernel 13: tiles [0, ninterior) of the table are interior tiles, the
 // rest edge tiles (build_tiles orders them so); one launch for each part.
+// k1: every shot of the batch has dx == dz (interior launch with literal weights).
 // With a side stream the edge launch runs beside the interior one (fork on
 // event `fork`, join on `join`; both parts read only the levels before the
 // step and write disjoint points), so that its few long-running waves
 // overlap the interior instead of following it.  Under graph capture this
 // records the two launches as parallel nodes.
 hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ninterior,
-                              int ntiles, const int* base, int rel, int rows, int brows,
+                              int ntiles, const int* base, int rel, int rows, int brows, int k1,
                               hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join)
 {
     if (ninterior < 0 || ninterior > ntiles || rows < 1) return hipErrorInvalidValue;
@@ -3212,11 +3319,14 @@ hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ni
 #define MNAD_Q2(NW)                                                                          \
     do {                                                                                     \
         if (ne > 0)                                                                          \
-            hipLaunchKernelGGL((k_step2q<NW, true>), dim3(ne), dim3(BX, NW), 0, se, shots,   \
-                               tiles + ninterior, base, rel, seg);                           \
-        if (ninterior > 0)                                                                   \
-            hipLaunchKernelGGL((k_step2q<NW, false>), dim3(ninterior), dim3(BX, NW), 0, st,   \
-                               shots, tiles, base, rel, seg);                                \
+            hipLaunchKernelGGL((k_step2q<NW, true, false>), dim3(ne), dim3(BX, NW), 0, se,    \
+                               shots, tiles + ninterior, base, rel, seg);                    \
+        if (ninterior > 0 && k1)                                                             \
+            hipLaunchKernelGGL((k_step2q<NW, false, true>), dim3(ninterior), dim3(BX, NW), 0, \
+                               st, shots, tiles, base, rel, seg);                            \
+        else if (ninterior > 0)                                                              \
+            hipLaunchKernelGGL((k_step2q<NW, false, false>), dim3(ninterior), dim3(BX, NW),   \
+                               0, st, shots, tiles, base, rel, seg);                         \
     } while (0)
     if (brows == 1) MNAD_Q2(1);
     else if (brows == 2) MNAD_Q2(2);

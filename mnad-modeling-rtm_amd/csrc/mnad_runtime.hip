@@ -36,7 +36,7 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
                             const int* base, int rel, int kernel, int rows, int brows,
                             hipStream_t st);
 hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ninterior,
-                              int ntiles, const int* base, int rel, int rows, int brows,
+                              int ntiles, const int* base, int rel, int rows, int brows, int k1,
                               hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st);
 hipError_t mnad_launch_init_source(ShotDev* shots, int nshots, hipStream_t st);
@@ -428,6 +428,8 @@ int receiver_tables(const mnad_grid& g, int nr, const int* zr, const int* xr, Sh
     s.d.nrec = s.nu;
     rrow.assign(g.nz, -1);
     rcol.clear();
+    s.d.rlo = 1;
+    s.d.rhi = 0;
     if (s.nu == 0) return MNAD_OK;
     std::vector<int> rows;
     for (auto& pt : s.upts)
@@ -435,6 +437,8 @@ int receiver_tables(const mnad_grid& g, int nr, const int* zr, const int* xr, Sh
             rrow[pt.first] = (int)rows.size() * g.nx;
             rows.push_back(pt.first);
         }
+    s.d.rlo = *std::min_element(rows.begin(), rows.end());
+    s.d.rhi = *std::max_element(rows.begin(), rows.end());
     rcol.assign(rows.size() * (size_t)g.nx, -1);
     for (int u = 0; u < s.nu; ++u) rcol[rrow[s.upts[u].first] + s.upts[u].second] = u;
     return MNAD_OK;
@@ -472,10 +476,14 @@ void base_desc(const mnad_grid& g, int pitch, int nt, int mode, int nlev, ShotDe
     d.mode = mode;
     d.kap = (float)((g.dx * g.dx) / (g.dz * g.dz));
     incr_consts((g.dx * g.dx) / (g.dz * g.dz), d.ic);
+    d.kap1 = (g.dx * g.dx) / (g.dz * g.dz) == 1.0;
     d.inv_dt2 = (float)(1.0 / (g.dt * g.dt));
     d.ntx = (g.nx + 63) / 64;
     d.src_j = d.src_i = -1;
     d.nlev = nlev;
+    d.rlo = 1;   // no receivers
+    d.rhi = 0;
+    d.pbytes = 0;
 }
 
 int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
@@ -484,10 +492,13 @@ int make_shot(mnad_ctx* c, const Model& m, const ShotSpec& sp, Shot& s)
     if (sp.nt < 1) return fail(MNAD_ERR_ARG, "nt must be >= 1");
     base_desc(g, m.pitch, sp.nt, sp.mode, two_step_kernel(c->kernel) ? 4 : 2, s.d);
     s.d.rx = m.rx();
-    for (int k = 0; k < 3 * s.d.nlev; ++k) {
-        CHK(dalloc(c, s.state[k], m.elems * sizeof(float), true));
-        s.d.f[k / 3][k % 3] = s.state[k].as<float>();
-    }
+    // all 3 nlev field planes in one allocation, plane k = 3 level + field at
+    // k * pbytes (step kernel 13 addresses them through one buffer resource)
+    const size_t pbytes = plane_stride_bytes(g.nz, m.pitch);
+    CHK(dalloc(c, s.state[0], 3 * s.d.nlev * pbytes, true));
+    s.d.pbytes = (unsigned)pbytes;
+    for (int k = 0; k < 3 * s.d.nlev; ++k)
+        s.d.f[k / 3][k % 3] = s.state[0].as<float>() + k * (pbytes / sizeof(float));
     const int np = g.npml;
     if (np > 0) {
         // two-steps-per-launch kernels keep two sets of memories (read one, write the other)
@@ -606,6 +617,8 @@ int make_rec_shot(mnad_ctx* c, Shot& fwd, Shot& adj, Shot& rec)
     const int first = (1 + f.shift) * f.T;
     d.nt = std::max(1, f.nt - first + 1);
     d.nrec = 0;
+    d.rlo = 1;
+    d.rhi = 0;
     d.rrow = nullptr;
     d.rcol = nullptr;
     d.gather = nullptr;
@@ -979,6 +992,16 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         if (h[k].nlev != (two_step_kernel(kern) ? 4 : 2))
             return fail(MNAD_ERR_STATE, "shot built for %d time levels cannot run on step kernel %d",
                         h[k].nlev, kern);
+        if (kern == 13) {   // one buffer resource spans a shot's field planes
+            const size_t pe = h[k].pbytes / sizeof(float);
+            bool contiguous = h[k].pbytes == plane_stride_bytes(h[k].nz, h[k].pitch) &&
+                              3 * h[k].nlev * (size_t)h[k].pbytes < ((size_t)1 << 31);
+            for (int q = 0; q < 3 * h[k].nlev; ++q)
+                contiguous = contiguous && h[k].f[q / 3][q % 3] == h[k].f[0][0] + q * pe;
+            if (!contiguous)
+                return fail(MNAD_ERR_STATE, "step kernel 13 needs a shot's field planes in one "
+                            "allocation of under 2 GiB; use another kernel for this grid");
+        }
         if ((h[k].rord > 0 || h[k].mode == MODE_REC) && !ring_kernel(kern))
             return fail(MNAD_ERR_STATE, "step kernel %d does not save or replay the boundary ring "
                         "(kernels 4, 5, 6, 10, 11 do)", kern);
@@ -1023,9 +1046,11 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
     }
+    int k1 = 1;   // kernel 13: every shot has dx == dz
+    for (const ShotDev& x : h) k1 &= x.kap1 ? 1 : 0;
     auto step = [&](int rel) {
-        return kern == 13 ? mnad_launch_step2q(ds, dt, ninner, ntiles, base, rel, rz, br, c->stream,
-                                               c->side, c->fork, c->join)
+        return kern == 13 ? mnad_launch_step2q(ds, dt, ninner, ntiles, base, rel, rz, br, k1,
+                                               c->stream, c->side, c->fork, c->join)
                           : mnad_launch_step(ds, dt, ntiles, base, rel, kern, rz, br, c->stream);
     };
     if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nlaunch < 2 * chunk) {

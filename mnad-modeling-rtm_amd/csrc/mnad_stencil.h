@@ -50,7 +50,11 @@ struct IncrC {
     float aw[5], bw[6];   // Ws: Y, dxD1uz, dxW, dzW, CxV     | Y, dxD1uz, dxW, dzW, dxzW, CxV
 };
 
-inline void incr_consts(double k, IncrC& c)
+// The weights for kap = k (dx^2 / dz^2), each formed in double and rounded
+// once to float.  constexpr: incr_consts_k1() below is the same evaluation at
+// compile time, for step kernels that take the weights as literals when
+// dx == dz.
+MNAD_HD constexpr IncrC incr_consts_of(double k)
 {
     const double k2 = k * k;
     const double au[6] = {24.0 / 11, 24.0 / 11 * k, 20.0 / 33, 20.0 / 33 * k, -1.0 / 132, -k / 132};
@@ -60,13 +64,17 @@ inline void incr_consts(double k, IncrC& c)
     const double bv[6] = {15.0, 0.5 * k2 - 1.25 * k, -2.5 * k, 2.5, -0.25 * k, 0.25 * k2};
     const double aw[5] = {-15.0 * k, -0.5, 1.0, -1.5 * k, -0.25};
     const double bw[6] = {15.0 * k2, 0.5 - 1.25 * k, -2.5 * k, 2.5 * k2, -0.25 * k, 0.25};
+    IncrC c{};
     for (int i = 0; i < 6; ++i) c.au[i] = (float)au[i];
     for (int i = 0; i < 8; ++i) c.bu[i] = (float)bu[i];
     for (int i = 0; i < 5; ++i) c.av[i] = (float)av[i];
     for (int i = 0; i < 6; ++i) c.bv[i] = (float)bv[i];
     for (int i = 0; i < 5; ++i) c.aw[i] = (float)aw[i];
     for (int i = 0; i < 6; ++i) c.bw[i] = (float)bw[i];
+    return c;
 }
+
+inline void incr_consts(double k, IncrC& c) { c = incr_consts_of(k); }
 
 // Increments of (U, Vs, Ws) at one point (without 2F - F_prev and without
 // the CPML term).  u, v, w: neighbourhoods of U, Vs, Ws; vww/vee: Vs at
