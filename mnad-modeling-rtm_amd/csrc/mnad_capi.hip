@@ -310,7 +310,11 @@ int mnad_born_adj(mnad_ctx* ctx, const mnad_grid* g, const double* v, int nt, in
                     ss, k.nz, k.nx, k.npml, k.nt, k.T, k.K, g->nz, g->nx, g->npml, nt, sp.T, sp.K);
     if (k.fwd) {   // ring storage: reverse reconstruction beside the adjoint
         TunePick ring_pick{kDefaultKernel, kDefaultRows, kDefaultBrows, 1, 0.0, {}};
-        TuningGuard tg(ctx, ring_kernel(ctx->kernel) ? nullptr : &ring_pick);
+        // the kept forward's levels (two, or four for a two-step kernel) fix
+        // the kernel family the reconstruction runs on
+        const bool fits = ring_kernel(ctx->kernel) &&
+                          k.fwd->d.nlev == (two_step_kernel(ctx->kernel) ? 4 : 2);
+        TuningGuard tg(ctx, fits ? nullptr : &ring_pick);
         Shot s;
         CHK(make_shot(ctx, m, sp, s));
         Shot& f = *k.fwd;
@@ -441,9 +445,12 @@ int ring_save(mnad_ctx* ctx, const Kept& k, const char* path)
     FILE* f = fopen(path, "wb");
     if (!f) return fail(MNAD_ERR_ARG, "cannot open %s for writing", path);
     int rc = fwrite(&h, sizeof(h), 1, f) == 1 ? MNAD_OK : fail(MNAD_ERR_ARG, "write to %s failed", path);
-    // levels 0 and 1 (U, Vs, Ws each), one plane after the other
-    for (int q = 0; q < 6 && rc == MNAD_OK; ++q)
-        rc = write_dev(ctx, f, s.d.f[q / 3][q % 3], h.plane_bytes);
+    // the forward's last two levels, nt - 1 and nt, as a two-level shot keeps
+    // them (level L in slot L mod 2), U, Vs, Ws each, one plane after the other
+    for (int q = 0; q < 6 && rc == MNAD_OK; ++q) {
+        const int L = ((k.nt - 1) % 2 == q / 3) ? k.nt - 1 : k.nt;
+        rc = write_dev(ctx, f, s.d.f[L % s.d.nlev][q % 3], h.plane_bytes);
+    }
     if (rc == MNAD_OK) rc = write_dev(ctx, f, s.amp.p, s.amp.n);
     if (rc == MNAD_OK) rc = write_dev(ctx, f, s.rq.p, s.rq.n);
     if (rc == MNAD_OK) rc = write_dev(ctx, f, s.rst.p, s.rst.n);

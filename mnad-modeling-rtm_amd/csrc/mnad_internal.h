@@ -280,7 +280,7 @@ int make_rec_shot(mnad_ctx* c, Shot& fwd, Shot& adj, Shot& rec);
 // Kernels that implement the ring code and the reverse reconstruction.
 inline bool ring_kernel(int kernel)
 {
-    return kernel == 4 || kernel == 5 || kernel == 6 || kernel == 10 || kernel == 11;
+    return kernel == 4 || kernel == 5 || kernel == 6 || kernel == 10 || kernel == 11 || kernel == 13;
 }
 void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
 int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);

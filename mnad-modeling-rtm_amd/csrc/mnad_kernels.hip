@@ -2685,16 +2685,16 @@ struct March {
 
 // Level s+1 at row a (phase 1), step t with PH = t mod QP: level s row a + k
 // in slot qslot(PH, k - 2); rx, level s-1 and the new level s+1 of row a in
-// slot qslot(PH, -2).
+// slot qslot(PH, -2).  PURE: no receiver rows in reach (the interior main loop).
 template <bool EDGE, bool K1, int PH, bool PURE = false>
 __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int a, int zA, int zB,
                                          int i, unsigned lane, const QCtx& qc, bool colin, bool ownc,
-                                         bool xint, bool xedge, const SnapStep& sn1, float amp1,
-                                         bool second)
+                                         bool xedge, float amp1, bool second)
 {
     constexpr int sM2 = qslot(PH, -4), sM1 = qslot(PH, -3), sC = qslot(PH, -2), sP1 = qslot(PH, -1),
                   sP2 = qslot(PH, 0);
-    const int nz = d.nz, np = d.npml;
+    const int nz = d.nz, nx = d.nx, np = d.npml;
+    const bool rec = d.mode == MODE_REC, adj = d.mode == MODE_ADJ;
     Nbh u, v, w;
     u.n = m.q0[0][sM1]; u.c = m.q0[0][sC]; u.s = m.q0[0][sP1];
     v.n = m.q0[1][sM1]; v.c = m.q0[1][sC]; v.s = m.q0[1][sP1];
@@ -2717,9 +2717,8 @@ __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int 
     }
     const bool rowin = !EDGE || (a >= 0 && a < nz);
     const bool own = ownc && a >= zA && a < zB;
-    bool zedge = false;
-    if (EDGE) {
-        zedge = a < np || a >= nz - np;
+    if (EDGE && !rec) {   // a reconstruction runs inside R, which holds no CPML memories
+        const bool zedge = a < np || a >= nz - np;
         if ((xedge || zedge) && rowin) {
             const PsiPt q = psi_point(d, a, i, i - (int)lane, lane);
             if (colin && (q.inx || q.inz)) {
@@ -2746,37 +2745,41 @@ __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int 
     float n0 = (2.0f * u.c + du) - m.qp[0][sC];
     float n1 = (2.0f * v.c + dv) - m.qp[1][sC];
     float n2 = (2.0f * w.c + dw) - m.qp[2][sC];
-    const bool fwd = d.mode == MODE_FWD;
-    if (fwd && a == d.src_j && i == d.src_i) n0 += amp1;
-    if (!PURE && d.nrec > 0 && rowin) {
+    if (!adj && a == d.src_j && i == d.src_i) n0 += amp1;
+    if (!PURE && adj && d.nrec > 0 && rowin) {   // the adjoint's receiver injection
         const int off = q_rrow(d, a);
         if (off >= 0 && colin) {
             const int r = uni(gp(d.rcol) + off)[(unsigned)i];
-            if (r >= 0) {
-                if (fwd) {
-                    if (own) (gp(d.gather) + (long)s * d.nrec)[(unsigned)r] = u.c;
-                } else {
-                    n0 -= (gp(d.radj) + (long)s * d.nrec)[(unsigned)r];
-                }
-            }
+            if (r >= 0) n0 -= (gp(d.radj) + (long)s * d.nrec)[(unsigned)r];
         }
     }
-    if (EDGE && !(rowin && colin)) n0 = n1 = n2 = 0.0f;
+    bool put = own;
     if (EDGE) {
-        if (own) {
+        if (!(rowin && colin)) {
+            n0 = n1 = n2 = 0.0f;
+        } else if (rec) {
+            const bool inr = in_rec_region(a, i, nz, nx, np);
+            put = put && inr;
+            if (ring_index(a, i, nz, nx, np) >= 0) {   // decoded before the launch
+                const unsigned o = qc.b1 + (unsigned)a * qc.pitch4;
+                n0 = qld(qc.rs, o, qc.vo[0]);
+                n1 = qld(qc.rs, o, qc.vo[1]);
+                n2 = qld(qc.rs, o, qc.vo[2]);
+            } else if (!inr) {
+                n0 = n1 = n2 = 0.0f;   // R never reads it
+            }
+        }
+        if (put) {
             const unsigned o = qc.b1 + (unsigned)a * qc.pitch4;
             qst(qc.rs, o, qc.vo[0], n0);
             qst(qc.rs, o, qc.vo[1], n1);
             qst(qc.rs, o, qc.vo[2], n2);
         }
     } else {   // no branch: the stores of lanes and rows that are not this wave's are dropped
-        const unsigned o = qc.b1 + (unsigned)a * qc.pitch4, msk = own ? 0u : QOFF;
+        const unsigned o = qc.b1 + (unsigned)a * qc.pitch4, msk = put ? 0u : QOFF;
         qst(qc.rs, o, qc.vo[0] + msk, n0);
         qst(qc.rs, o, qc.vo[1] + msk, n1);
         qst(qc.rs, o, qc.vo[2] + msk, n2);
-    }
-    if (!PURE && sn1.kind && !zedge && xint && a >= zA && a < zB) {
-        if (ownc) snap_apply(d, sn1, (long)(a - np) * (d.nx - 2 * np), (unsigned)(i - np), du, n0);
     }
     m.q1[0][sC] = n0;
     m.q1[1][sC] = n1;
@@ -2789,13 +2792,13 @@ __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int 
 // prefetched row; the edge path's repeated CPML step reads it).
 template <bool EDGE, bool K1, int PH, bool PURE = false>
 __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int b, int i,
-                                         unsigned lane, const QCtx& qc, bool ownc, bool xint,
-                                         bool xedge, const SnapStep& sn2, float amp2, float bm0,
-                                         float bm1, float bm2)
+                                         unsigned lane, const QCtx& qc, bool ownc, bool xedge,
+                                         float amp2, float bm0, float bm1, float bm2)
 {
     constexpr int tM2 = qslot(PH, -6), tM1 = qslot(PH, -5), tC = qslot(PH, -4), tP1 = qslot(PH, -3),
                   tP2 = qslot(PH, -2);
-    const int nz = d.nz, np = d.npml;
+    const int nz = d.nz, nx = d.nx, np = d.npml;
+    const bool rec = d.mode == MODE_REC, adj = d.mode == MODE_ADJ;
     Nbh u, v, w;
     u.n = m.q1[0][tM1]; u.c = m.q1[0][tC]; u.s = m.q1[0][tP1];
     v.n = m.q1[1][tM1]; v.c = m.q1[1][tC]; v.s = m.q1[1][tP1];
@@ -2817,9 +2820,8 @@ __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int 
     } else {
         mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
     }
-    bool zedge = false;
-    if (EDGE) {
-        zedge = b < np || b >= nz - np;
+    if (EDGE && !rec) {
+        const bool zedge = b < np || b >= nz - np;
         if (xedge || zedge) {
             // level s along the axes at (b, i), x neighbours while every lane is active
             const float c0e = lane_next(c0), c0w = lane_prev(c0), cve = lane_next(cv),
@@ -2856,20 +2858,17 @@ __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int 
     float n0 = (2.0f * u.c + du) - c0;
     const float n1 = (2.0f * v.c + dv) - cv;
     const float n2 = (2.0f * w.c + dw) - cw;
-    const bool fwd = d.mode == MODE_FWD;
-    if (fwd && b == d.src_j && i == d.src_i) n0 += amp2;
-    if (!PURE && d.nrec > 0) {
+    if (!adj && b == d.src_j && i == d.src_i) n0 += amp2;
+    if (!PURE && adj && d.nrec > 0) {   // the adjoint's receiver injection
         const int off = q_rrow(d, b);
         if (off >= 0 && ownc) {
             const int r = uni(gp(d.rcol) + off)[(unsigned)i];
-            if (r >= 0) {
-                if (fwd) (gp(d.gather) + (long)(s + 1) * d.nrec)[(unsigned)r] = u.c;
-                else n0 -= (gp(d.radj) + (long)(s + 1) * d.nrec)[(unsigned)r];
-            }
+            if (r >= 0) n0 -= (gp(d.radj) + (long)(s + 1) * d.nrec)[(unsigned)r];
         }
     }
     if (EDGE) {
-        if (ownc) {
+        // a reconstruction writes R only (its ring cells were decoded before the launch)
+        if (ownc && (!rec || in_rec_region(b, i, nz, nx, np))) {
             const unsigned o = qc.b2 + (unsigned)b * qc.pitch4;
             qst(qc.rs, o, qc.vo[0], n0);
             qst(qc.rs, o, qc.vo[1], n1);
@@ -2880,9 +2879,6 @@ __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int 
         qst(qc.rs, o, qc.vo[0] + msk, n0);
         qst(qc.rs, o, qc.vo[1] + msk, n1);
         qst(qc.rs, o, qc.vo[2] + msk, n2);
-    }
-    if (!PURE && sn2.kind && !zedge && xint) {
-        if (ownc) snap_apply(d, sn2, (long)(b - np) * (d.nx - 2 * np), (unsigned)(i - np), du, n0);
     }
 }
 
@@ -2932,8 +2928,7 @@ __device__ __forceinline__ void q_loads(const ShotDev& d, March& m, int s, int r
 template <bool EDGE, bool K1, int PH>
 __device__ __forceinline__ void q_step(const ShotDev& d, March& m, int s, int t, int NR, int zA,
                                        int zB, int i, unsigned lane, const QCtx& qc, bool colin,
-                                       bool ownc, bool xint, bool xedge, const SnapStep& sn1,
-                                       const SnapStep& sn2, float amp1, float amp2, bool second)
+                                       bool ownc, bool xedge, float amp1, float amp2, bool second)
 {
     const int r = zA - 4 + t;
     // level s row r - 5 = b - 1, whose slot the load below refills (edge path only)
@@ -2942,28 +2937,24 @@ __device__ __forceinline__ void q_step(const ShotDev& d, March& m, int s, int t,
                 bm2 = EDGE ? m.q0[2][sb] : 0.0f;
     q_loads<EDGE, PH>(d, m, s, r, t, NR, qc, colin);
     if (t >= 4)
-        q_phase1<EDGE, K1, PH>(d, m, s, r - 2, zA, zB, i, lane, qc, colin, ownc, xint, xedge, sn1,
-                           amp1, second);
+        q_phase1<EDGE, K1, PH>(d, m, s, r - 2, zA, zB, i, lane, qc, colin, ownc, xedge, amp1, second);
     if (t >= 8 && second)
-        q_phase2<EDGE, K1, PH>(d, m, s, r - 4, i, lane, qc, ownc, xint, xedge, sn2, amp2, bm0, bm1,
-                           bm2);
+        q_phase2<EDGE, K1, PH>(d, m, s, r - 4, i, lane, qc, ownc, xedge, amp2, bm0, bm1, bm2);
 }
 
-// A main-loop step of the interior path (t >= 8, both phases, no receiver
-// rows near the segment, no snapshot or imaging work in the launch): loads,
-// phase 1 and phase 2 with no branch, so that the wait counts stay exact and
-// a step waits only for the loads issued QD steps earlier.
+// A main-loop step of the interior path (t >= 8, both phases, no adjoint
+// receiver rows near the segment): loads, phase 1 and phase 2 with no branch,
+// so that the wait counts stay exact and a step waits only for the loads
+// issued QD steps earlier.
 template <bool K1, int PH>
 __device__ __forceinline__ void q_step_main(const ShotDev& d, March& m, int s, int t, int zA, int zB,
                                             int i, unsigned lane, const QCtx& qc, bool ownc,
-                                            const SnapStep& sn, float amp1, float amp2)
+                                            float amp1, float amp2)
 {
     const int r = zA - 4 + t;
     q_loads<false, PH>(d, m, s, r, t, 0, qc, true);
-    q_phase1<false, K1, PH, true>(d, m, s, r - 2, zA, zB, i, lane, qc, true, ownc, true, false, sn,
-                                  amp1, true);
-    q_phase2<false, K1, PH, true>(d, m, s, r - 4, i, lane, qc, ownc, true, false, sn, amp2, 0.0f,
-                                  0.0f, 0.0f);
+    q_phase1<false, K1, PH, true>(d, m, s, r - 2, zA, zB, i, lane, qc, true, ownc, false, amp1, true);
+    q_phase2<false, K1, PH, true>(d, m, s, r - 4, i, lane, qc, ownc, false, amp2, 0.0f, 0.0f, 0.0f);
 }
 
 template <bool EDGE, bool K1>
@@ -2973,10 +2964,23 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
     const int i = x0 - 4 + (int)lane;
     const bool colin = !EDGE || (i >= 0 && i < nx);
     const bool ownc = lane >= 4 && lane < BX - 4 && (!EDGE || i < nx);
-    const bool xint = !EDGE || (i >= np && i < nx - np);
     const bool xedge = EDGE && (x0 - 4 < np || x0 + XQ + 4 > nx - np);
     const bool second = s + 1 < d.nt;
-    const bool fwd = d.mode == MODE_FWD;
+    const bool rec = d.mode == MODE_REC;
+    // injections: the forward's a_src w into level s+1 (src_amp[s + 1]); a
+    // reconstruction takes the forward's injection into level c+1 back out
+    // of level c-1 (c = ntf - 1 - s: src_amp[c + 1])
+    float amp1 = 0.0f, amp2 = 0.0f;
+    if (d.src_j >= 0 && d.mode == MODE_FWD) {
+        amp1 = d.src_amp[s + 1];
+        amp2 = second ? d.src_amp[s + 2] : 0.0f;
+    } else if (d.src_j >= 0 && rec) {
+        amp1 = d.src_amp[d.ntf - s];
+        amp2 = second ? d.src_amp[d.ntf - 1 - s] : 0.0f;
+    }
+    // plane of level s + k: forward and adjoint (s + k) mod 4, reconstruction
+    // (ntf - 1 - s - k) mod 4 (the forward's levels backwards)
+    auto plane = [&](int k) { return rec ? (d.ntf - 1 - s - k) & 3 : (s + k) & 3; };
     QCtx qc;
     {
         const unsigned pb = d.pbytes;   // the host checks the planes are one allocation
@@ -2984,28 +2988,25 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
         qc.pitch4 = 4u * (unsigned)d.pitch;
         qc.rs = qres(d.f[0][0], 3u * (unsigned)d.nlev * pb);
         qc.rr = qres(d.rx, pb);
-        qc.b0 = 3u * (unsigned)(s & 3) * pb + xo4;
-        qc.bm = 3u * (unsigned)((s + 3) & 3) * pb + xo4;
-        qc.b1 = 3u * (unsigned)((s + 1) & 3) * pb + xo4;
-        qc.b2 = 3u * (unsigned)((s + 2) & 3) * pb + xo4;
+        qc.b0 = 3u * (unsigned)plane(0) * pb + xo4;
+        qc.bm = 3u * (unsigned)plane(-1) * pb + xo4;
+        qc.b1 = 3u * (unsigned)plane(1) * pb + xo4;
+        qc.b2 = 3u * (unsigned)plane(2) * pb + xo4;
         qc.bx = xo4;
         qc.vo[0] = 4u * lane;
         qc.vo[1] = 4u * lane + pb;
         qc.vo[2] = 4u * lane + 2u * pb;
     }
-    const float amp1 = fwd && d.src_j >= 0 ? d.src_amp[s + 1] : 0.0f;
-    const float amp2 = fwd && d.src_j >= 0 && second ? d.src_amp[s + 2] : 0.0f;
-    const SnapStep sn1 = snap_step(d, s), sn2 = snap_step(d, s + 1);
     const int NR = zB - zA + 8;   // march steps: level s rows zA-4 .. zB+3
     March m;
     // level s rows zA-4 .. zA-4+QD-1 (the first QD steps' rows)
 #pragma unroll
     for (int k = 0; k < QD; ++k) q_load0<EDGE>(d, m, s, zA - 4 + k, k, qc);
-#define MNAD_QSTEP(PH) q_step<EDGE, K1, PH>(d, m, s, t + PH, NR, zA, zB, i, lane, qc, colin, ownc, xint, \
-                                        xedge, sn1, sn2, amp1, amp2, second)
-    // receiver rows among the rows the march updates (zA-2 .. zB+1)
-    const bool rcv = d.nrec > 0 && d.rlo <= zB + 1 && d.rhi >= zA - 2;
-    if (!EDGE && second && !sn1.kind && !sn2.kind && !rcv) {
+#define MNAD_QSTEP(PH) q_step<EDGE, K1, PH>(d, m, s, t + PH, NR, zA, zB, i, lane, qc, colin, ownc, \
+                                            xedge, amp1, amp2, second)
+    // adjoint receiver rows among the rows the march updates (zA-2 .. zB+1)
+    const bool rcv = d.mode == MODE_ADJ && d.nrec > 0 && d.rlo <= zB + 1 && d.rhi >= zA - 2;
+    if (!EDGE && second && !rcv) {
         // unrolled by QP = 7: the slots are fixed registers.  Steps 0 .. 7
         // (loads, then phase 1 from step 4), then the branch-free main loop.
         static_assert(QP == 7, "the unrolled march below is written for QP = 7");
@@ -3023,7 +3024,7 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
             const int t = 7;
             MNAD_QSTEP(0);
         }
-#define MNAD_QMAIN(PH, T) q_step_main<K1, PH>(d, m, s, T, zA, zB, i, lane, qc, ownc, sn1, amp1, amp2)
+#define MNAD_QMAIN(PH, T) q_step_main<K1, PH>(d, m, s, T, zA, zB, i, lane, qc, ownc, amp1, amp2)
         for (int t = 8; t < NR; t += QP) {   // NR = seg + 8 >= 16
             MNAD_QMAIN(1, t);
             if (t + 1 >= NR) break;
@@ -3042,8 +3043,8 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
 #undef MNAD_QMAIN
     } else {
         // One step body (PH = 0) with the rings rotated by one slot after each
-        // step: the edge path, and interior blocks with receiver rows,
-        // snapshot or imaging work, or a last launch of one step.
+        // step: the edge path, and interior blocks with adjoint receiver rows
+        // or a last launch of one step.
         for (int t = 0; t < NR; ++t) {
             MNAD_QSTEP(0);
 #pragma unroll
@@ -3094,6 +3095,133 @@ __global__ __launch_bounds__(BX * NW) void k_step2q(const ShotDev* __restrict__ 
     const int x0 = (ti * NW + wv) * XQ;
     if (x0 >= nx) return;   // no barriers in this kernel: a wave may leave early
     q_march<EDGE, K1>(d, s, x0, zA, zB, threadIdx.x);
+}
+
+// ------------------------------------ passes around a kernel-13 launch
+// Level s + k of a shot after a kernel-13 launch of step s, field f (the
+// level mapping of q_march).
+__device__ __forceinline__ const float* q_level(const ShotDev& d, int s, int k, int f)
+{
+    const int l = d.mode == MODE_REC ? (d.ntf - 1 - s - k) & 3 : (s + k) & 3;
+    return d.f[l][f];
+}
+
+__device__ __forceinline__ long q_at(const ShotDev& d, int j, int i)
+{
+    return (long)(j + MNAD_ZO) * d.pitch + (i + MNAD_XO);
+}
+
+// After a launch of steps s, s+1 of every shot, off the stored levels.
+// PASS 0: snapshots and illumination (forward), imaging (adjoint), gathers
+// and ring codes (forward).  PASS 1 (launched after PASS 0): the
+// reconstruction's hand-off slot, so that an adjoint reads a slot before the
+// reconstruction overwrites it.  The increment of step q at a point is the
+// stored second difference of the levels, (U(q+1) - 2 U(q)) + U(q-1), less
+// the source term at the source point: the update's own increment up to
+// float rounding of the levels (the reverse pass and the snapshots need it to
+// about 1e-5, the image tolerance being 1e-3).  Every workgroup loops over
+// the shots and takes a slice of each shot's points.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_q_post(const ShotDev* __restrict__ shots, int nshots,
+                                                const int* __restrict__ base, int rel)
+{
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nth = (long)gridDim.x * blockDim.x;
+    for (int k = 0; k < nshots; ++k) {
+        const ShotDev& d = shots[k];
+        const int s = *base + rel - d.t0;
+        if (s < 0 || s >= d.nt) continue;   // uniform
+        const bool second = s + 1 < d.nt;
+        const int np = d.npml, nxi = d.nx - 2 * np, nzi = d.nz - 2 * np;
+        const long ni = nxi > 0 && nzi > 0 ? (long)nzi * nxi : 0;
+        for (int h = 0; h < (second ? 2 : 1); ++h) {
+            const int q = s + h;   // step q turns level q into level q+1
+            const SnapStep st = snap_step(d, q);
+            if (PASS == 0 ? !(st.kind == 1 || st.kind == 2) : st.kind != 3) continue;
+            const float* un = q_level(d, s, h + 1, 0);
+            const float* uc = q_level(d, s, h, 0);
+            const float* up = q_level(d, s, h - 1, 0);
+            int sj = -1, si = -1;
+            float amp = 0.0f;
+            if (d.src_j >= 0 && d.mode != MODE_ADJ) {
+                sj = d.src_j;
+                si = d.src_i;
+                amp = d.mode == MODE_REC ? d.src_amp[d.ntf - q] : d.src_amp[q + 1];
+            }
+            for (long p = tid; p < ni; p += nth) {
+                const int jj = (int)(p / nxi), ii = (int)(p - (long)jj * nxi);
+                const int j = np + jj, i = np + ii;
+                const long o = q_at(d, j, i);
+                const float nu = un[o];
+                float du = 0.0f;
+                if (st.kind != 2) {
+                    du = (nu - 2.0f * uc[o]) + up[o];
+                    if (j == sj && i == si) du -= amp;
+                }
+                snap_apply(d, st, (long)jj * nxi, (unsigned)ii, du, nu);
+            }
+        }
+        if (PASS != 0 || d.mode != MODE_FWD) continue;
+        if (d.nrec > 0) {   // gathers: step q records level q at each receiver
+            const float* l0 = q_level(d, s, 0, 0);
+            const float* l1 = q_level(d, s, 1, 0);
+            for (int j = d.rlo; j <= d.rhi; ++j) {
+                const int off = d.rrow[j];
+                if (off < 0) continue;
+                for (long i = tid; i < d.nx; i += nth) {
+                    const int r = d.rcol[off + i];
+                    if (r < 0) continue;
+                    const long o = q_at(d, j, (int)i);
+                    d.gather[(long)s * d.nrec + r] = l0[o];
+                    if (second) d.gather[(long)(s + 1) * d.nrec + r] = l1[o];
+                }
+            }
+        }
+        if (d.rord > 0) {   // ring codes of levels s+1 and s+2, in order per cell
+            for (long r = tid; r < d.nring; r += nth) {
+                int j, i;
+                ring_point((int)r, d.nz, d.nx, np, &j, &i);
+                const long o = q_at(d, j, i);
+                ring_put(d, s + 1, (unsigned)r, q_level(d, s, 1, 0)[o], q_level(d, s, 1, 1)[o],
+                         q_level(d, s, 1, 2)[o]);
+                if (second)
+                    ring_put(d, s + 2, (unsigned)r, q_level(d, s, 2, 0)[o], q_level(d, s, 2, 1)[o],
+                             q_level(d, s, 2, 2)[o]);
+            }
+        }
+    }
+}
+
+// Before a launch of steps s, s+1: a reconstruction's ring cells get levels
+// c-1 and c-2 (c = ntf - 1 - s) decoded from the saved ring, in order per
+// cell (rec_one's decode, two levels at a time).
+__global__ __launch_bounds__(256) void k_q_pre(const ShotDev* __restrict__ shots, int nshots,
+                                               const int* __restrict__ base, int rel)
+{
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nth = (long)gridDim.x * blockDim.x;
+    for (int k = 0; k < nshots; ++k) {
+        const ShotDev& d = shots[k];
+        if (d.mode != MODE_REC || d.rord <= 0) continue;
+        const int s = *base + rel - d.t0;
+        if (s < 0 || s >= d.nt) continue;
+        const bool second = s + 1 < d.nt;
+        const int c = d.ntf - 1 - s;
+        for (long r = tid; r < d.nring; r += nth) {
+            int j, i;
+            ring_point((int)r, d.nz, d.nx, d.npml, &j, &i);
+            const long o = q_at(d, j, i);
+            float v[3];
+            ring_get(d, c + 1, (unsigned)r, v);
+#pragma unroll
+            for (int f = 0; f < 3; ++f) ((float*)q_level(d, s, 1, f))[o] = v[f];
+            if (second) {
+                ring_get(d, c, (unsigned)r, v);
+#pragma unroll
+                for (int f = 0; f < 3; ++f) ((float*)q_level(d, s, 2, f))[o] = v[f];
+            }
+        }
+    }
 }
 
 __global__ void k_advance(int* base, int n)
@@ -3297,14 +3425,17 @@ hipError_t mnad_launch_resident(const ShotDev* shots, const ResBlock* blocks, in
 // Step kernel 13: tiles [0, ninterior) of the table are interior tiles, the
 // rest edge tiles (build_tiles orders them so); one launch for each part.
 // k1: every shot of the batch has dx == dz (interior launch with literal weights).
+// passes: the batch's work around the step (k_q_pre before it; k_q_post<0>
+// and k_q_post<1> after it).
 // With a side stream the edge launch runs beside the interior one (fork on
 // event `fork`, join on `join`; both parts read only the levels before the
 // step and write disjoint points), so that its few long-running waves
 // overlap the interior instead of following it.  Under graph capture this
 // records the two launches as parallel nodes.
-hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ninterior,
-                              int ntiles, const int* base, int rel, int rows, int brows, int k1,
-                              hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join)
+hipError_t mnad_launch_step2q(const ShotDev* shots, int nshots, const TileRef* tiles,
+                              int ninterior, int ntiles, const int* base, int rel, int rows,
+                              int brows, int k1, int passes, hipStream_t st, hipStream_t side,
+                              hipEvent_t fork, hipEvent_t join)
 {
     if (ninterior < 0 || ninterior > ntiles || rows < 1) return hipErrorInvalidValue;
     if (brows != 1 && brows != 2 && brows != 4) return hipErrorInvalidValue;
@@ -3312,6 +3443,9 @@ hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ni
     const bool par = side && fork && join && ne > 0 && ninterior > 0;
     hipStream_t se = par ? side : st;
     hipError_t e = hipSuccess;
+    // passes: bit 0 ring decode before the step, bit 1 and bit 2 the passes after it
+    const dim3 pg(1024), pb(256);
+    if (passes & 1) hipLaunchKernelGGL(k_q_pre, pg, pb, 0, st, shots, nshots, base, rel);
     if (par) {
         if ((e = hipEventRecord(fork, st)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(side, fork, 0)) != hipSuccess) return e;
@@ -3337,7 +3471,9 @@ hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ni
         if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(st, join, 0)) != hipSuccess) return e;
     }
-    return hipSuccess;
+    if (passes & 2) hipLaunchKernelGGL(k_q_post<0>, pg, pb, 0, st, shots, nshots, base, rel);
+    if (passes & 4) hipLaunchKernelGGL(k_q_post<1>, pg, pb, 0, st, shots, nshots, base, rel);
+    return hipGetLastError();
 }
 
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st)

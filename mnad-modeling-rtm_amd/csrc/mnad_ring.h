@@ -66,6 +66,24 @@ MNAD_RHD int ring_index(int j, int i, int nz, int nx, int np)
 
 MNAD_RHD int ring_cells(int nz, int nx, int np) { return 4 * ((nx - 2 * np + 4) + (nz - 2 * np)); }
 
+// The grid point (j, i) of ring cell r (the inverse of ring_index).
+MNAD_RHD void ring_point(int r, int nz, int nx, int np, int* j, int* i)
+{
+    const int wb = nx - 2 * np + 4;
+    if (r < 2 * wb) {
+        *j = np - 2 + r / wb;
+        *i = np - 2 + r % wb;
+    } else if (r < 4 * wb) {
+        const int q = r - 2 * wb;
+        *j = nz - np + q / wb;
+        *i = np - 2 + q % wb;
+    } else {
+        const int q = r - 4 * wb, c = q % 4;
+        *j = np + q / 4;
+        *i = c < 2 ? np - 2 + c : nx - np + (c - 2);
+    }
+}
+
 // Point (j, i) inside the reconstruction region.
 MNAD_RHD bool in_rec_region(int j, int i, int nz, int nx, int np)
 {

@@ -35,9 +35,10 @@ hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, i
 hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
                             const int* base, int rel, int kernel, int rows, int brows,
                             hipStream_t st);
-hipError_t mnad_launch_step2q(const ShotDev* shots, const TileRef* tiles, int ninterior,
-                              int ntiles, const int* base, int rel, int rows, int brows, int k1,
-                              hipStream_t st, hipStream_t side, hipEvent_t fork, hipEvent_t join);
+hipError_t mnad_launch_step2q(const ShotDev* shots, int nshots, const TileRef* tiles,
+                              int ninterior, int ntiles, const int* base, int rel, int rows,
+                              int brows, int k1, int passes, hipStream_t st, hipStream_t side,
+                              hipEvent_t fork, hipEvent_t join);
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st);
 hipError_t mnad_launch_init_source(ShotDev* shots, int nshots, hipStream_t st);
 hipError_t mnad_launch_make_radj(const float* dobs, const float* dsyn, const float* arec,
@@ -606,8 +607,8 @@ int make_rec_shot(mnad_ctx* c, Shot& fwd, Shot& adj, Shot& rec)
     const ShotDev& f = fwd.d;
     if (f.mode != MODE_FWD || f.rord <= 0 || f.T <= 0)
         return fail(MNAD_ERR_STATE, "reverse reconstruction needs a forward that saved its ring");
-    if (f.nlev != 2)
-        return fail(MNAD_ERR_STATE, "reverse reconstruction runs on one-step kernels only");
+    if (f.nlev != 2 && f.nlev != 4)
+        return fail(MNAD_ERR_STATE, "reverse reconstruction needs a forward of 2 or 4 levels");
     rec.d = f;
     ShotDev& d = rec.d;
     d.mode = MODE_REC;
@@ -637,12 +638,24 @@ int make_rec_shot(mnad_ctx* c, Shot& fwd, Shot& adj, Shot& rec)
     const size_t ni = (size_t)(f.nz - 2 * f.npml) * (f.nx - 2 * f.npml);
     CHK(dalloc(c, rec.snaps, ni * sizeof(float), true));
     d.snaps = rec.snaps.p;
-    // the adjoint reads the slot one launch after REC writes it:
-    // t0(rec) - t0(adj) = shift T - 1
+    // the adjoint reads the slot one launch after REC writes it: one-step
+    // kernels t0(rec) - t0(adj) = shift T - 1.  Kernel 13 (two steps per
+    // launch, the slot written and read by the passes after a launch, the
+    // reads first) needs the read L = 2 or 3 steps after the write and both
+    // launch origins even (a launch covers steps 2n, 2n + 1 of every shot):
+    // shift T - L with L making it even, and L <= T so that the next write
+    // comes no earlier than the read's launch
     adj.d.snaps = rec.snaps.p;
     adj.d.sfmt = SNAP_SLOT;
     adj.d.sscale = adj.d.sinv = 1.0f;
-    const long long lag = (long long)f.shift * f.T - 1;
+    long long lag = (long long)f.shift * f.T - 1;
+    if (f.nlev == 4) {
+        const int L = (((long long)f.shift * f.T) % 2 == 0) ? 2 : 3;
+        if (L > f.T)
+            return fail(MNAD_ERR_STATE, "reverse reconstruction on a two-step kernel needs a "
+                        "snapshot period of at least 3 (this schedule has %d)", f.T);
+        lag = (long long)f.shift * f.T - L;
+    }
     if (lag >= 0) {
         adj.t0 = 0;
         rec.t0 = lag;
@@ -988,6 +1001,9 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         if (shots[k]->t0 < 0 || shots[k]->t0 > (1 << 20))
             return fail(MNAD_ERR_STATE, "shot launch origin %lld out of range", shots[k]->t0);
         h[k].t0 = (int)shots[k]->t0;
+        if (two_step_kernel(kern) && (h[k].t0 & 1))   // its step 0 would fall in a skipped half
+            return fail(MNAD_ERR_STATE, "shot launch origin %d is odd: step kernel %d runs two "
+                        "steps per launch from even origins", h[k].t0, kern);
         nsteps = std::max(nsteps, h[k].t0 + h[k].nt);
         if (h[k].nlev != (two_step_kernel(kern) ? 4 : 2))
             return fail(MNAD_ERR_STATE, "shot built for %d time levels cannot run on step kernel %d",
@@ -1004,7 +1020,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         }
         if ((h[k].rord > 0 || h[k].mode == MODE_REC) && !ring_kernel(kern))
             return fail(MNAD_ERR_STATE, "step kernel %d does not save or replay the boundary ring "
-                        "(kernels 4, 5, 6, 10, 11 do)", kern);
+                        "(kernels 4, 5, 6, 10, 11, 13 do)", kern);
     }
     if (resident_kernel(kern)) {
         double bytes = 0.0;
@@ -1046,11 +1062,18 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
     }
-    int k1 = 1;   // kernel 13: every shot has dx == dz
-    for (const ShotDev& x : h) k1 &= x.kap1 ? 1 : 0;
+    int k1 = 1;       // kernel 13: every shot has dx == dz
+    int passes = 0;   // kernel 13: the work of the passes around each launch
+    for (const ShotDev& x : h) {
+        k1 &= x.kap1 ? 1 : 0;
+        if (x.mode == MODE_REC && x.rord > 0) passes |= 1;
+        if ((x.T > 0 && x.mode != MODE_REC) || (x.mode == MODE_FWD && (x.nrec > 0 || x.rord > 0)))
+            passes |= 2;
+        if (x.T > 0 && x.mode == MODE_REC) passes |= 4;
+    }
     auto step = [&](int rel) {
-        return kern == 13 ? mnad_launch_step2q(ds, dt, ninner, ntiles, base, rel, rz, br, k1,
-                                               c->stream, c->side, c->fork, c->join)
+        return kern == 13 ? mnad_launch_step2q(ds, n, dt, ninner, ntiles, base, rel, rz, br, k1,
+                                               passes, c->stream, c->side, c->fork, c->join)
                           : mnad_launch_step(ds, dt, ntiles, base, rel, kern, rz, br, c->stream);
     };
     if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nlaunch < 2 * chunk) {

@@ -1,6 +1,7 @@
 // Host check of the boundary-ring layout and code (csrc/mnad_ring.h), built
 // with g++ alone (no HIP).  One line per case, "ok <name>" or
 // "FAIL <name>: <reason>":
+//  - ring_point(r) is the point ring_index maps to r, for every cell;
 //  - ring_index maps the two-cell band around the reconstruction region onto
 //    [0, nring) one to one, and gives -1 inside it and beyond the band;
 //  - every 16-bit code decodes to the value an independent formula gives
@@ -67,6 +68,15 @@ static void check_index(int nz, int nx, int np)
             ok = false;
             why = "cell " + std::to_string(r) + " hit " + std::to_string(hits[r]) + " times";
         }
+    for (int r = 0; r < n && ok; ++r) {   // ring_point inverts ring_index
+        int j = -1, i = -1;
+        ring_point(r, nz, nx, np, &j, &i);
+        if (ring_index(j, i, nz, nx, np) != r) {
+            ok = false;
+            why = "ring_point of cell " + std::to_string(r) + " maps back to " +
+                  std::to_string(ring_index(j, i, nz, nx, np));
+        }
+    }
     report("ring_index_" + std::to_string(nz) + "x" + std::to_string(nx) + "_np" + std::to_string(np),
            ok, why);
 }

@@ -37,8 +37,9 @@ hipError_t mnad_launch_setup_model(const double*, int, int, int, int, double, do
                                    float*, float*, float*, hipStream_t) { return hipErrorNotSupported; }
 hipError_t mnad_launch_step(const mnad::ShotDev*, const mnad::TileRef*, int, const int*, int, int,
                             int, int, hipStream_t) { return hipErrorNotSupported; }
-hipError_t mnad_launch_step2q(const mnad::ShotDev*, const mnad::TileRef*, int, int, const int*, int,
-                              int, int, int, hipStream_t, hipStream_t, hipEvent_t, hipEvent_t)
+hipError_t mnad_launch_step2q(const mnad::ShotDev*, int, const mnad::TileRef*, int, int, const int*,
+                              int, int, int, int, int, hipStream_t, hipStream_t, hipEvent_t,
+                              hipEvent_t)
 {
     return hipErrorNotSupported;
 }

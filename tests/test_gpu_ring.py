@@ -134,13 +134,15 @@ def test_ring_schedules_agree(mnad):
 
 
 @pytest.mark.parametrize("kernel,rz,br", [(4, 1, 4), (5, 1, 1), (6, 1, 1), (6, 1, 4), (10, 1, 2),
-                                          (11, 2, 1)])
+                                          (11, 2, 1), (13, 1, 1), (13, 2, 4)])
 def test_ring_kernel_families(mnad, kernel, rz, br):
     """Every kernel family that codes the ring and runs the reconstruction
     (one point per lane, one set of CPML memories prefetched, four and two
-    columns per lane in the interior, two rows per wavefront) reproduces the
-    fixture image and agrees with the default kernel; on C1 (241 x 241, so the
-    multi-column interior spans several tiles) as well."""
+    columns per lane in the interior, two rows per wavefront, and the
+    two-step register-queue kernel, whose ring codes, ring decodes and d2u
+    come from the passes around its launches) reproduces the fixture image and
+    agrees with the default kernel; on C1 (241 x 241, so the multi-column
+    interior spans several tiles) as well."""
     _say(f"ring kernel {kernel},{rz},{br}")
     f = golden("rtm_two_shot")
     g = _grid(mnad, f, "vs")
@@ -228,6 +230,38 @@ def test_ring_file_handoff_and_failed_adjoint(mnad, ctx, tmp_path, fmt):
         other.close()
     assert np.array_equal(im_file, im_dev)
     assert rel_l2(im_dev, f["Im0"]) < IMAGE_TOL
+
+
+def test_ring_file_handoff_two_step_kernel(mnad, tmp_path):
+    """A ring forward kept on the two-step kernel (13, four time levels)
+    images on that kernel, and spilled to a file and loaded in a context on
+    the default kernel it images there too (the file holds the forward's last
+    two levels as a two-level shot): the two images agree to 1e-4 and meet
+    the fixture."""
+    _say("ring file hand-off from kernel 13")
+    f = golden("rtm_rect")
+    g = _grid(mnad, f, "vs")
+    w, zs, xs, zr, xr = f["w"], int(f["zs0"]), int(f["xs0"]), f["zr0"], f["xr0"]
+    path = str(tmp_path / "UU_7.mnad")
+    ctx = mnad.Context(0)
+    other = mnad.Context(0)
+    try:
+        ctx.set_tuning(kernel=13, rows_per_thread=1, block_rows=1)
+        obs = ctx.fwd(g, f["v"], w, zs, xs, zr, xr, want_state=False)["gather"]
+        syn = ctx.fwd(g, f["vs"], w, zs, xs, zr, xr, ss=7, indx=1, snap_fmt=mnad.SNAP_RING_BF16,
+                      want_state=False)
+        data = obs - syn["gather"]
+        ctx.snap_save(7, path)
+        im13 = ctx.born_adj(g, f["vs"], data, zr, xr, ss=7)
+        other.snap_load(7, path)
+        im4 = other.born_adj(g, f["vs"], data, zr, xr, ss=7)
+    finally:
+        other.close()
+        ctx.close()
+    _say(f"  kernel 13 vs file on kernel 4: {rel_l2(im13, im4):.2e}")
+    assert rel_l2(im13, im4) < 1e-4
+    assert rel_l2(im13, f["Im0"]) < IMAGE_TOL
+    assert rel_l2(im4, f["Im0"]) < IMAGE_TOL
 
 
 def test_ring_needs_two_cpml_cells(mnad, ctx):
