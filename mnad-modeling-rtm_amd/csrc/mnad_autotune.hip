@@ -31,8 +31,8 @@ struct Variant {
 };
 
 const Variant kVariants[] = {{4, 1, 4}, {4, 1, 1}, {11, 2, 1}, {0, 1, 4}, {6, 1, 2}, {6, 1, 4},
-                             {10, 1, 2}, {7, 3, 16}, {7, 5, 8}, {7, 7, 16}, {12, 8, 8},
-                             {12, 16, 4}};
+                             {10, 1, 2}, {7, 3, 16}, {7, 5, 8}, {7, 7, 16}, {13, 2, 4},
+                             {13, 4, 4}};
 
 // Snapshot period of an RTM forward on this model: T = floor(nt / nf / 2) with
 // nf = floor(nt vmin / (6 vmax)), about 3 vmax / vmin.

@@ -134,12 +134,6 @@ void mnad_ctx_destroy(mnad_ctx* ctx)
         ctx->models.clear();
         release_batch_cache(ctx);
         ctx->pool.trim();
-        if (ctx->side) {
-            (void)hipStreamSynchronize(ctx->side);
-            (void)hipStreamDestroy(ctx->side);
-            (void)hipEventDestroy(ctx->fork);
-            (void)hipEventDestroy(ctx->join);
-        }
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;

@@ -97,7 +97,7 @@ struct TunePick {
 // that reads the batch descriptors from the context's persistent buffers, so
 // it can be replayed for any batch with the same kernel and tile count.
 struct GraphEntry {
-    int kern, rz, brows, ntiles, ninner, chunk, spl;
+    int kern, rz, brows, ntiles, chunk, spl;
     const void* ds;
     const void* dt;
     const int* base;
@@ -154,9 +154,6 @@ struct mnad_ctx {
     std::shared_ptr<mnad::Server> server;
     std::mutex srv_mu;
     hipStream_t stream = nullptr;
-    // step kernel 13: a second stream and fork/join events for its edge launches
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
     std::recursive_mutex mu;
     mnad::Pool pool;
     std::map<int, mnad::Kept> kept;
@@ -285,9 +282,12 @@ inline bool ring_kernel(int kernel)
 void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
 int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);
 int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags);
-// ninterior (kernel 13): table entries before it are interior tiles, the rest edge tiles
-void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table,
-                 int* ninterior = nullptr);
+// Step kernel 13: estimated cost of an edge wave (bounds checks, CPML
+// memories of both sets) against an interior wave of the same size, from
+// kernel traces of separate interior and edge launches at the 2061 x 8061
+// grid (2.35x per point); build_tiles balances the XCDs' work with it.
+constexpr double kQEdgeWeight = 2.4;
+void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table);
 // Step kernels that advance two time levels per launch (shots keep four).
 inline bool two_step_kernel(int kernel)
 {

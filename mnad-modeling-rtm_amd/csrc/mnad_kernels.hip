@@ -28,6 +28,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <utility>
+
 #include "mnad_device.h"
 #include "mnad_stencil.h"
 
@@ -2625,10 +2627,15 @@ __global__ __launch_bounds__(BX * NW) void k_step2z(const ShotDev* __restrict__ 
 // by one slot after each step).  Injections, receivers, snapshots/imaging and
 // the point arithmetic are kernel 7's.  Time level t lives in f[t % 4].
 constexpr int XQ = BX - 8;
-constexpr int QD = 2;        // prefetch distance, in march steps
-constexpr int QP = 5 + QD;   // ring length
-
-__host__ __device__ constexpr int qslot(int ph, int off) { return ((ph + off) % QP + QP) % QP; }
+// QD: prefetch distance, in march steps; QP = 5 + QD: ring length.  The
+// launch runs QD = 2 on both paths: at the C5 grid a distance of 3 or 4 (110
+// and 120 VGPRs, the same 4 waves per SIMD) measured 0-1 % slower.
+template <int QD>
+__host__ __device__ constexpr int qslot(int ph, int off)
+{
+    constexpr int QP = 5 + QD;
+    return ((ph + off) % QP + QP) % QP;
+}
 
 // Buffer access to the field planes: a shot's 3 nlev planes are one
 // allocation (plane 3 level + field at byte (3 level + field) * plane bytes),
@@ -2676,7 +2683,9 @@ __device__ __forceinline__ int q_rrow(const ShotDev& d, int j)
     return ((const __attribute__((address_space(4))) int*)d.rrow)[j];
 }
 
+template <int QD>
 struct March {
+    static constexpr int QP = 5 + QD;
     float q0[3][QP];   // level s (U, Vs, Ws)
     float q1[3][QP];   // level s+1
     float qr[QP];      // rx
@@ -2684,15 +2693,15 @@ struct March {
 };
 
 // Level s+1 at row a (phase 1), step t with PH = t mod QP: level s row a + k
-// in slot qslot(PH, k - 2); rx, level s-1 and the new level s+1 of row a in
-// slot qslot(PH, -2).  PURE: no receiver rows in reach (the interior main loop).
-template <bool EDGE, bool K1, int PH, bool PURE = false>
-__device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int a, int zA, int zB,
+// in slot qslot<QD>(PH, k - 2); rx, level s-1 and the new level s+1 of row a in
+// slot qslot<QD>(PH, -2).  PURE: no receiver rows in reach (the interior main loop).
+template <int QD, bool EDGE, bool K1, int PH, bool PURE = false>
+__device__ __forceinline__ void q_phase1(const ShotDev& d, March<QD>& m, int s, int a, int zA, int zB,
                                          int i, unsigned lane, const QCtx& qc, bool colin, bool ownc,
                                          bool xedge, float amp1, bool second)
 {
-    constexpr int sM2 = qslot(PH, -4), sM1 = qslot(PH, -3), sC = qslot(PH, -2), sP1 = qslot(PH, -1),
-                  sP2 = qslot(PH, 0);
+    constexpr int sM2 = qslot<QD>(PH, -4), sM1 = qslot<QD>(PH, -3), sC = qslot<QD>(PH, -2), sP1 = qslot<QD>(PH, -1),
+                  sP2 = qslot<QD>(PH, 0);
     const int nz = d.nz, nx = d.nx, np = d.npml;
     const bool rec = d.mode == MODE_REC, adj = d.mode == MODE_ADJ;
     Nbh u, v, w;
@@ -2787,16 +2796,16 @@ __device__ __forceinline__ void q_phase1(const ShotDev& d, March& m, int s, int 
 }
 
 // Level s+2 at row b = a - 2 (phase 2), step t with PH = t mod QP: level s+1
-// row b + k in slot qslot(PH, k - 4), rx and level s of row b in slot
-// qslot(PH, -4).  bm0..bm2: level s row b - 1 (its slot already holds the
+// row b + k in slot qslot<QD>(PH, k - 4), rx and level s of row b in slot
+// qslot<QD>(PH, -4).  bm0..bm2: level s row b - 1 (its slot already holds the
 // prefetched row; the edge path's repeated CPML step reads it).
-template <bool EDGE, bool K1, int PH, bool PURE = false>
-__device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int b, int i,
+template <int QD, bool EDGE, bool K1, int PH, bool PURE = false>
+__device__ __forceinline__ void q_phase2(const ShotDev& d, March<QD>& m, int s, int b, int i,
                                          unsigned lane, const QCtx& qc, bool ownc, bool xedge,
                                          float amp2, float bm0, float bm1, float bm2)
 {
-    constexpr int tM2 = qslot(PH, -6), tM1 = qslot(PH, -5), tC = qslot(PH, -4), tP1 = qslot(PH, -3),
-                  tP2 = qslot(PH, -2);
+    constexpr int tM2 = qslot<QD>(PH, -6), tM1 = qslot<QD>(PH, -5), tC = qslot<QD>(PH, -4), tP1 = qslot<QD>(PH, -3),
+                  tP2 = qslot<QD>(PH, -2);
     const int nz = d.nz, nx = d.nx, np = d.npml;
     const bool rec = d.mode == MODE_REC, adj = d.mode == MODE_ADJ;
     Nbh u, v, w;
@@ -2883,8 +2892,8 @@ __device__ __forceinline__ void q_phase2(const ShotDev& d, March& m, int s, int 
 }
 
 // Level s row j into slot sl (zeros off the grid on the edge path).
-template <bool EDGE>
-__device__ __forceinline__ void q_load0(const ShotDev& d, March& m, int s, int j, int sl, const QCtx& qc)
+template <int QD, bool EDGE>
+__device__ __forceinline__ void q_load0(const ShotDev& d, March<QD>& m, int s, int j, int sl, const QCtx& qc)
 {
     if (!EDGE || (j >= 0 && j < d.nz)) {
         const unsigned o = qc.b0 + (unsigned)j * qc.pitch4;
@@ -2898,8 +2907,8 @@ __device__ __forceinline__ void q_load0(const ShotDev& d, March& m, int s, int j
 
 // Loads of march step t (PH = t mod QP): level s row r + QD, and rx and level
 // s-1 of row r - 2 + QD (the level-s+1 row of step t + QD).
-template <bool EDGE, int PH>
-__device__ __forceinline__ void q_loads(const ShotDev& d, March& m, int s, int r, int t, int NR,
+template <int QD, bool EDGE, int PH>
+__device__ __forceinline__ void q_loads(const ShotDev& d, March<QD>& m, int s, int r, int t, int NR,
                                         const QCtx& qc, bool colin)
 {
     // The interior path loads without conditions (a branch around a load
@@ -2907,10 +2916,10 @@ __device__ __forceinline__ void q_loads(const ShotDev& d, March& m, int s, int r
     // last are inside the plane's padding rows, and those before its first
     // inside the grid; their values are not used.
     if (!EDGE || t + QD < NR) {
-        q_load0<EDGE>(d, m, s, r + QD, qslot(PH, QD), qc);
+        q_load0<QD, EDGE>(d, m, s, r + QD, qslot<QD>(PH, QD), qc);
         if (!EDGE || t + QD >= 4) {
             const int a = r - 2 + QD;
-            constexpr int sl = qslot(PH, QD - 2);
+            constexpr int sl = qslot<QD>(PH, QD - 2);
             if ((!EDGE || (a >= 0 && a < d.nz)) && colin) {
                 const unsigned o = (unsigned)a * qc.pitch4;
                 m.qr[sl] = qld(qc.rr, qc.bx + o, qc.vo[0]);
@@ -2925,41 +2934,71 @@ __device__ __forceinline__ void q_loads(const ShotDev& d, March& m, int s, int r
 }
 
 // March step t with the slots of phase PH = t mod QP.
-template <bool EDGE, bool K1, int PH>
-__device__ __forceinline__ void q_step(const ShotDev& d, March& m, int s, int t, int NR, int zA,
+template <int QD, bool EDGE, bool K1, int PH>
+__device__ __forceinline__ void q_step(const ShotDev& d, March<QD>& m, int s, int t, int NR, int zA,
                                        int zB, int i, unsigned lane, const QCtx& qc, bool colin,
                                        bool ownc, bool xedge, float amp1, float amp2, bool second)
 {
     const int r = zA - 4 + t;
     // level s row r - 5 = b - 1, whose slot the load below refills (edge path only)
-    constexpr int sb = qslot(PH, QD);
+    constexpr int sb = qslot<QD>(PH, QD);
     const float bm0 = EDGE ? m.q0[0][sb] : 0.0f, bm1 = EDGE ? m.q0[1][sb] : 0.0f,
                 bm2 = EDGE ? m.q0[2][sb] : 0.0f;
-    q_loads<EDGE, PH>(d, m, s, r, t, NR, qc, colin);
+    q_loads<QD, EDGE, PH>(d, m, s, r, t, NR, qc, colin);
     if (t >= 4)
-        q_phase1<EDGE, K1, PH>(d, m, s, r - 2, zA, zB, i, lane, qc, colin, ownc, xedge, amp1, second);
+        q_phase1<QD, EDGE, K1, PH>(d, m, s, r - 2, zA, zB, i, lane, qc, colin, ownc, xedge, amp1, second);
     if (t >= 8 && second)
-        q_phase2<EDGE, K1, PH>(d, m, s, r - 4, i, lane, qc, ownc, xedge, amp2, bm0, bm1, bm2);
+        q_phase2<QD, EDGE, K1, PH>(d, m, s, r - 4, i, lane, qc, ownc, xedge, amp2, bm0, bm1, bm2);
 }
 
 // A main-loop step of the interior path (t >= 8, both phases, no adjoint
 // receiver rows near the segment): loads, phase 1 and phase 2 with no branch,
 // so that the wait counts stay exact and a step waits only for the loads
 // issued QD steps earlier.
-template <bool K1, int PH>
-__device__ __forceinline__ void q_step_main(const ShotDev& d, March& m, int s, int t, int zA, int zB,
+template <int QD, bool K1, int PH>
+__device__ __forceinline__ void q_step_main(const ShotDev& d, March<QD>& m, int s, int t, int zA, int zB,
                                             int i, unsigned lane, const QCtx& qc, bool ownc,
                                             float amp1, float amp2)
 {
     const int r = zA - 4 + t;
-    q_loads<false, PH>(d, m, s, r, t, 0, qc, true);
-    q_phase1<false, K1, PH, true>(d, m, s, r - 2, zA, zB, i, lane, qc, true, ownc, false, amp1, true);
-    q_phase2<false, K1, PH, true>(d, m, s, r - 4, i, lane, qc, ownc, false, amp2, 0.0f, 0.0f, 0.0f);
+    q_loads<QD, false, PH>(d, m, s, r, t, 0, qc, true);
+    q_phase1<QD, false, K1, PH, true>(d, m, s, r - 2, zA, zB, i, lane, qc, true, ownc, false, amp1, true);
+    q_phase2<QD, false, K1, PH, true>(d, m, s, r - 4, i, lane, qc, ownc, false, amp2, 0.0f, 0.0f, 0.0f);
 }
 
-template <bool EDGE, bool K1>
+// Steps T = 0 .. 7 of the march, each with the slots of phase T mod QP.
+template <int QD, bool EDGE, bool K1, int... T>
+__device__ __forceinline__ void q_prologue(const ShotDev& d, March<QD>& m, int s, int NR, int zA,
+                                           int zB, int i, unsigned lane, const QCtx& qc, bool colin,
+                                           bool ownc, bool xedge, float amp1, float amp2,
+                                           bool second, std::integer_sequence<int, T...>)
+{
+    (q_step<QD, EDGE, K1, T % (5 + QD)>(d, m, s, T, NR, zA, zB, i, lane, qc, colin, ownc, xedge,
+                                        amp1, amp2, second),
+     ...);
+}
+
+// One round of the main loop: steps t + K (K = 0 .. QP-1, t = 8 + n QP), each
+// with the slots of phase (8 + K) mod QP, up to step NR - 1.  Returns false
+// once it reached NR.
+template <int QD, bool K1, int... K>
+__device__ __forceinline__ bool q_main_round(const ShotDev& d, March<QD>& m, int s, int t, int NR,
+                                             int zA, int zB, int i, unsigned lane, const QCtx& qc,
+                                             bool ownc, float amp1, float amp2,
+                                             std::integer_sequence<int, K...>)
+{
+    return ((t + K < NR ? (q_step_main<QD, K1, (8 + K) % (5 + QD)>(d, m, s, t + K, zA, zB, i, lane,
+                                                                   qc, ownc, amp1, amp2),
+                           true)
+                        : false) &&
+            ...);
+}
+
+template <int QD, bool EDGE, bool K1>
 __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA, int zB, unsigned lane)
 {
+    static_assert(QD >= 2 && QD <= 4, "the march's first loads assume a prefetch distance of 2 to 4");
+    constexpr int QP = 5 + QD;
     const int nx = d.nx, np = d.npml;
     const int i = x0 - 4 + (int)lane;
     const bool colin = !EDGE || (i >= 0 && i < nx);
@@ -2998,55 +3037,29 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
         qc.vo[2] = 4u * lane + 2u * pb;
     }
     const int NR = zB - zA + 8;   // march steps: level s rows zA-4 .. zB+3
-    March m;
+    March<QD> m;
     // level s rows zA-4 .. zA-4+QD-1 (the first QD steps' rows)
 #pragma unroll
-    for (int k = 0; k < QD; ++k) q_load0<EDGE>(d, m, s, zA - 4 + k, k, qc);
-#define MNAD_QSTEP(PH) q_step<EDGE, K1, PH>(d, m, s, t + PH, NR, zA, zB, i, lane, qc, colin, ownc, \
-                                            xedge, amp1, amp2, second)
+    for (int k = 0; k < QD; ++k) q_load0<QD, EDGE>(d, m, s, zA - 4 + k, k, qc);
     // adjoint receiver rows among the rows the march updates (zA-2 .. zB+1)
     const bool rcv = d.mode == MODE_ADJ && d.nrec > 0 && d.rlo <= zB + 1 && d.rhi >= zA - 2;
     if (!EDGE && second && !rcv) {
-        // unrolled by QP = 7: the slots are fixed registers.  Steps 0 .. 7
-        // (loads, then phase 1 from step 4), then the branch-free main loop.
-        static_assert(QP == 7, "the unrolled march below is written for QP = 7");
-        {
-            const int t = 0;
-            MNAD_QSTEP(0);
-            MNAD_QSTEP(1);
-            MNAD_QSTEP(2);
-            MNAD_QSTEP(3);
-            MNAD_QSTEP(4);
-            MNAD_QSTEP(5);
-            MNAD_QSTEP(6);
-        }
-        {
-            const int t = 7;
-            MNAD_QSTEP(0);
-        }
-#define MNAD_QMAIN(PH, T) q_step_main<K1, PH>(d, m, s, T, zA, zB, i, lane, qc, ownc, amp1, amp2)
-        for (int t = 8; t < NR; t += QP) {   // NR = seg + 8 >= 16
-            MNAD_QMAIN(1, t);
-            if (t + 1 >= NR) break;
-            MNAD_QMAIN(2, t + 1);
-            if (t + 2 >= NR) break;
-            MNAD_QMAIN(3, t + 2);
-            if (t + 3 >= NR) break;
-            MNAD_QMAIN(4, t + 3);
-            if (t + 4 >= NR) break;
-            MNAD_QMAIN(5, t + 4);
-            if (t + 5 >= NR) break;
-            MNAD_QMAIN(6, t + 5);
-            if (t + 6 >= NR) break;
-            MNAD_QMAIN(0, t + 6);
-        }
-#undef MNAD_QMAIN
+        // unrolled by QP: the slots are fixed registers.  Steps 0 .. 7 (loads,
+        // then phase 1 from step 4), then the branch-free main loop in rounds
+        // of QP steps.
+        q_prologue<QD, EDGE, K1>(d, m, s, NR, zA, zB, i, lane, qc, colin, ownc, xedge, amp1, amp2,
+                                 second, std::make_integer_sequence<int, 8>{});
+        for (int t = 8; t < NR; t += QP)   // NR = seg + 8 >= 16
+            if (!q_main_round<QD, K1>(d, m, s, t, NR, zA, zB, i, lane, qc, ownc, amp1, amp2,
+                                      std::make_integer_sequence<int, QP>{}))
+                break;
     } else {
         // One step body (PH = 0) with the rings rotated by one slot after each
         // step: the edge path, and interior blocks with adjoint receiver rows
         // or a last launch of one step.
         for (int t = 0; t < NR; ++t) {
-            MNAD_QSTEP(0);
+            q_step<QD, EDGE, K1, 0>(d, m, s, t, NR, zA, zB, i, lane, qc, colin, ownc, xedge, amp1,
+                                    amp2, second);
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 const float a0 = m.q0[f][0], b0 = m.q1[f][0], p0 = m.qp[f][0];
@@ -3066,20 +3079,19 @@ __device__ __forceinline__ void q_march(const ShotDev& d, int s, int x0, int zA,
             m.qr[QP - 1] = r0;
         }
     }
-#undef MNAD_QSTEP
 }
 
-// Interior tiles (EDGE false) and edge tiles (EDGE true) run as two launches
-// of one step: the CPML code of the edge path would otherwise set the
-// register allocation of both.  K1 (all shots of the batch have dx == dz):
-// the interior launch takes the stencil weights as literals, which frees the
-// scalar registers the 36 weights otherwise hold.  The host orders the tile table interior tiles
-// first (q_interior_tile decides); an interior launch skips a tile that does
-// not pass the same test.
-template <int NW, bool EDGE, bool K1>
+// One launch of step kernel 13: a tile is NW waves side by side, each wave
+// marching its own 56 columns.  A wave whose columns pass q_interior_tile
+// takes the interior path (no bounds or CPML code; with K1, every shot of the
+// batch has dx == dz and the stencil weights are literals), the others the
+// edge path.  Both paths allocate 4 waves per SIMD (at most 128 VGPRs), so
+// one launch costs the interior no occupancy, and the edge waves, which are
+// few and long, run beside interior ones instead of in a launch of their own.
+template <int NW, bool K1>
 __global__ __launch_bounds__(BX * NW) void k_step2q(const ShotDev* __restrict__ shots,
-                                                    const TileRef* __restrict__ tiles,
-                                                    const int* __restrict__ base, int rel, int seg)
+                                                     const TileRef* __restrict__ tiles,
+                                                     const int* __restrict__ base, int rel, int seg)
 {
     const TileRef tr = tiles[blockIdx.x];
     if (tr.shot < 0) return;
@@ -3090,11 +3102,13 @@ __global__ __launch_bounds__(BX * NW) void k_step2q(const ShotDev* __restrict__ 
     const int ti = tr.tile - tj * d.ntx;
     const int nx = d.nx, nz = d.nz;
     const int zA = tj * seg, zB = min(zA + seg, nz);
-    if (!EDGE && !q_interior_tile(nx, nz, d.npml, ti * NW * XQ, zA, NW * XQ, seg)) return;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int x0 = (ti * NW + wv) * XQ;
     if (x0 >= nx) return;   // no barriers in this kernel: a wave may leave early
-    q_march<EDGE, K1>(d, s, x0, zA, zB, threadIdx.x);
+    if (q_interior_tile(nx, nz, d.npml, x0, zA, XQ, seg))
+        q_march<2, false, K1>(d, s, x0, zA, zB, threadIdx.x);
+    else
+        q_march<2, true, false>(d, s, x0, zA, zB, threadIdx.x);
 }
 
 // ------------------------------------ passes around a kernel-13 launch
@@ -3422,55 +3436,35 @@ hipError_t mnad_launch_resident(const ShotDev* shots, const ResBlock* blocks, in
     return hipLaunchCooperativeKernel(k, dim3(nblocks), dim3(BX, RES_NW), args, res_lds(bh), st);
 }
 
-// Step kernel 13: tiles [0, ninterior) of the table are interior tiles, the
-// rest edge tiles (build_tiles orders them so); one launch for each part.
-// k1: every shot of the batch has dx == dz (interior launch with literal weights).
-// passes: the batch's work around the step (k_q_pre before it; k_q_post<0>
-// and k_q_post<1> after it).
-// With a side stream the edge launch runs beside the interior one (fork on
-// event `fork`, join on `join`; both parts read only the levels before the
-// step and write disjoint points), so that its few long-running waves
-// overlap the interior instead of following it.  Under graph capture this
-// records the two launches as parallel nodes.
-hipError_t mnad_launch_step2q(const ShotDev* shots, int nshots, const TileRef* tiles,
-                              int ninterior, int ntiles, const int* base, int rel, int rows,
-                              int brows, int k1, int passes, hipStream_t st, hipStream_t side,
-                              hipEvent_t fork, hipEvent_t join)
+// Step kernel 13 over the tile table, two steps (s, s+1) per launch.
+// passes: the batch's work around the step (bit 0: k_q_pre before it; bits 1
+// and 2: k_q_post<0> and k_q_post<1> after it).
+hipError_t mnad_launch_step2q(const ShotDev* shots, int nshots, const TileRef* tiles, int ntiles,
+                              const int* base, int rel, int rows, int brows, int k1, int passes,
+                              hipStream_t st)
 {
-    if (ninterior < 0 || ninterior > ntiles || rows < 1) return hipErrorInvalidValue;
+    if (rows < 1 || ntiles < 0) return hipErrorInvalidValue;
     if (brows != 1 && brows != 2 && brows != 4) return hipErrorInvalidValue;
-    const int ne = ntiles - ninterior, seg = 8 * rows;
-    const bool par = side && fork && join && ne > 0 && ninterior > 0;
-    hipStream_t se = par ? side : st;
-    hipError_t e = hipSuccess;
-    // passes: bit 0 ring decode before the step, bit 1 and bit 2 the passes after it
+    const int seg = 8 * rows;
     const dim3 pg(1024), pb(256);
     if (passes & 1) hipLaunchKernelGGL(k_q_pre, pg, pb, 0, st, shots, nshots, base, rel);
-    if (par) {
-        if ((e = hipEventRecord(fork, st)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(side, fork, 0)) != hipSuccess) return e;
-    }
-#define MNAD_Q2(NW)                                                                          \
-    do {                                                                                     \
-        if (ne > 0)                                                                          \
-            hipLaunchKernelGGL((k_step2q<NW, true, false>), dim3(ne), dim3(BX, NW), 0, se,    \
-                               shots, tiles + ninterior, base, rel, seg);                    \
-        if (ninterior > 0 && k1)                                                             \
-            hipLaunchKernelGGL((k_step2q<NW, false, true>), dim3(ninterior), dim3(BX, NW), 0, \
-                               st, shots, tiles, base, rel, seg);                            \
-        else if (ninterior > 0)                                                              \
-            hipLaunchKernelGGL((k_step2q<NW, false, false>), dim3(ninterior), dim3(BX, NW),   \
-                               0, st, shots, tiles, base, rel, seg);                         \
+    if (ntiles > 0) {
+#define MNAD_Q2(NW)                                                                            \
+    do {                                                                                       \
+        if (k1)                                                                                \
+            hipLaunchKernelGGL((k_step2q<NW, true>), dim3(ntiles), dim3(BX, NW), 0, st, shots, \
+                               tiles, base, rel, seg);                                         \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_step2q<NW, false>), dim3(ntiles), dim3(BX, NW), 0, st,       \
+                               shots, tiles, base, rel, seg);                                  \
     } while (0)
-    if (brows == 1) MNAD_Q2(1);
-    else if (brows == 2) MNAD_Q2(2);
-    else MNAD_Q2(4);
+        if (brows == 1) MNAD_Q2(1);
+        else if (brows == 2) MNAD_Q2(2);
+        else MNAD_Q2(4);
 #undef MNAD_Q2
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (par) {
-        if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(st, join, 0)) != hipSuccess) return e;
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     if (passes & 2) hipLaunchKernelGGL(k_q_post<0>, pg, pb, 0, st, shots, nshots, base, rel);
     if (passes & 4) hipLaunchKernelGGL(k_q_post<1>, pg, pb, 0, st, shots, nshots, base, rel);
     return hipGetLastError();

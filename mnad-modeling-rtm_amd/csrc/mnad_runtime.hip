@@ -35,10 +35,9 @@ hipError_t mnad_launch_setup_model(const double* v, int nz, int nx, int pitch, i
 hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntiles,
                             const int* base, int rel, int kernel, int rows, int brows,
                             hipStream_t st);
-hipError_t mnad_launch_step2q(const ShotDev* shots, int nshots, const TileRef* tiles,
-                              int ninterior, int ntiles, const int* base, int rel, int rows,
-                              int brows, int k1, int passes, hipStream_t st, hipStream_t side,
-                              hipEvent_t fork, hipEvent_t join);
+hipError_t mnad_launch_step2q(const ShotDev* shots, int nshots, const TileRef* tiles, int ntiles,
+                              const int* base, int rel, int rows, int brows, int k1, int passes,
+                              hipStream_t st);
 hipError_t mnad_launch_advance(int* base, int n, hipStream_t st);
 hipError_t mnad_launch_init_source(ShotDev* shots, int nshots, hipStream_t st);
 hipError_t mnad_launch_make_radj(const float* dobs, const float* dsyn, const float* arec,
@@ -745,8 +744,7 @@ static void x4_layout(ShotDev& d, int nc, int nt1)
 
 int interior_columns_per_lane(int kernel) { return kernel == 6 ? 4 : (kernel == 10 ? 2 : 0); }
 
-void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table,
-                 int* ninterior)
+void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef>& table)
 {
     std::vector<TileRef> list;
     std::vector<char> strip;
@@ -774,10 +772,18 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
             // a reverse reconstruction touches the interior and its two-cell ring only
             if (rec && (r1 <= np - 2 || r0 >= nz - np + 2 || c1 <= np - 2 || c0 >= nx - np + 2))
                 continue;
-            // kernel 13: its edge tiles (q_interior_tile) stand in for the strip tiles
-            const bool st = c->kernel == 13
-                                ? !q_interior_tile(nx, nz, np, c0, r0, tw, th)
-                                : !rec && (c0 < np || c1 > nx - np || r0 < np || r1 > nz - np);
+            if (c->kernel == 13) {
+                // kernel 13: the tile's work is its interior waves (56 columns
+                // passing q_interior_tile) plus its edge waves at their weight
+                int nin = 0, ned = 0;
+                for (int x0 = c0; x0 < c1; x0 += 56)
+                    (q_interior_tile(nx, nz, np, x0, r0, 56, th) ? nin : ned)++;
+                list.push_back(TileRef{k, t});
+                strip.push_back(ned > 0);
+                work.push_back(nin + kQEdgeWeight * ned);
+                continue;
+            }
+            const bool st = !rec && (c0 < np || c1 > nx - np || r0 < np || r1 > nz - np);
             list.push_back(TileRef{k, t});
             strip.push_back(st);
             work.push_back(st ? sw : 1.0);
@@ -803,9 +809,6 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
     // 561 x 1061 grid: 8.7 us by count, 7.8 us by work).  Within each run
     // the strip tiles go first.
     const int nx8 = 8;
-    // kernel 13 launches its interior and edge tiles separately: each part is
-    // ordered on its own and the interior part, padded to a multiple of 8,
-    // comes first
     auto order = [&](const std::vector<size_t>& sel) {
         double total = 0.0;
         for (size_t q : sel) total += work[q];
@@ -828,19 +831,9 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
         }
     };
     table.clear();
-    std::vector<size_t> all, inner, edge;
-    for (size_t q = 0; q < list.size(); ++q) {
-        all.push_back(q);
-        (strip[q] ? edge : inner).push_back(q);
-    }
-    if (c->kernel == 13) {
-        if (!inner.empty()) order(inner);
-        if (ninterior) *ninterior = (int)table.size();
-        if (!edge.empty()) order(edge);
-    } else {
-        if (!all.empty()) order(all);
-        if (ninterior) *ninterior = 0;
-    }
+    std::vector<size_t> all(list.size());
+    for (size_t q = 0; q < list.size(); ++q) all[q] = q;
+    if (!all.empty()) order(all);
 }
 
 // Unique HBM bytes a shot's nt steps must move:
@@ -1028,8 +1021,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         return run_resident(c, h, nsteps, bytes);
     }
     std::vector<TileRef> table;
-    int ninner = 0;
-    build_tiles(c, h, table, &ninner);
+    build_tiles(c, h, table);
     // descriptors in the context's persistent buffers, so that a cached graph
     // chunk of the same shape replays on them
     CHK(ensure_batch_buf(c, c->bdesc, h.size() * sizeof(ShotDev)));
@@ -1057,11 +1049,6 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
     int tw, th, spl;
     tile_shape(c, &tw, &th, &spl);
     const int nlaunch = (nsteps + spl - 1) / spl;   // a launch covers spl steps
-    if (kern == 13 && !c->side) {   // side stream of kernel 13's edge launches (default priority)
-        HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
-    }
     int k1 = 1;       // kernel 13: every shot has dx == dz
     int passes = 0;   // kernel 13: the work of the passes around each launch
     for (const ShotDev& x : h) {
@@ -1072,8 +1059,8 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         if (x.T > 0 && x.mode == MODE_REC) passes |= 4;
     }
     auto step = [&](int rel) {
-        return kern == 13 ? mnad_launch_step2q(ds, n, dt, ninner, ntiles, base, rel, rz, br, k1,
-                                               passes, c->stream, c->side, c->fork, c->join)
+        return kern == 13 ? mnad_launch_step2q(ds, n, dt, ntiles, base, rel, rz, br, k1, passes,
+                                               c->stream)
                           : mnad_launch_step(ds, dt, ntiles, base, rel, kern, rz, br, c->stream);
     };
     if ((flags & MNAD_FLAG_NO_GRAPH) || chunk <= 1 || nlaunch < 2 * chunk) {
@@ -1084,7 +1071,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         for (size_t k = 0; k < c->graphs.size(); ++k) {
             const GraphEntry& e = c->graphs[k];
             if (e.kern == kern && e.rz == rz && e.brows == br && e.ntiles == ntiles &&
-                e.ninner == ninner && e.chunk == chunk && e.spl == spl && e.ds == ds &&
+                e.chunk == chunk && e.spl == spl && e.ds == ds &&
                 e.dt == dt && e.base == base) {
                 exec = e.exec;
                 break;
@@ -1111,7 +1098,7 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
                 c->graphs.erase(c->graphs.begin());
             }
             c->graphs.push_back(
-                GraphEntry{kern, rz, br, ntiles, ninner, chunk, spl, ds, dt, base, graph, exec});
+                GraphEntry{kern, rz, br, ntiles, chunk, spl, ds, dt, base, graph, exec});
         }
         const int full = nlaunch / chunk, rem = nlaunch - full * chunk;
         HIPCHK(hipEventRecord(ev0, c->stream));   // after instantiation

@@ -11,11 +11,12 @@
 //  - the estimated work per XCD differs by at most two tile weights, half a
 //    tile at either end of each XCD's range (two-step kernels: counts);
 //  - with strip_first, each XCD dispatches its CPML-strip tiles first;
-//  - kernel 13: the interior tiles come first, in a part whose length is a
-//    multiple of 8, and the edge tiles after it; each interior tile passes
-//    q_interior_tile, every load of its march (four cells around it) lies in
-//    the grid and every level-s+1 halo point (two cells) outside the CPML;
-//    each part on its own meets the XCD checks above;
+//  - kernel 13: a wave (56 columns of its tile) takes the interior path when
+//    it passes q_interior_tile, and then every load of its march (four
+//    cells around it) lies in the grid and every level-s+1 halo point (two
+//    cells) outside the CPML; a tile weighs its interior waves plus
+//    kQEdgeWeight per edge wave, and counts as a strip tile when it holds an
+//    edge wave;
 // and the block heights the resident kernel (9) picks for a few batches;
 // and the host pass over a velocity model (scan_model, model_digest):
 //  - the first value that is not positive and finite (0, -0, negative, NaN,
@@ -37,9 +38,8 @@ hipError_t mnad_launch_setup_model(const double*, int, int, int, int, double, do
                                    float*, float*, float*, hipStream_t) { return hipErrorNotSupported; }
 hipError_t mnad_launch_step(const mnad::ShotDev*, const mnad::TileRef*, int, const int*, int, int,
                             int, int, hipStream_t) { return hipErrorNotSupported; }
-hipError_t mnad_launch_step2q(const mnad::ShotDev*, int, const mnad::TileRef*, int, int, const int*,
-                              int, int, int, int, int, hipStream_t, hipStream_t, hipEvent_t,
-                              hipEvent_t)
+hipError_t mnad_launch_step2q(const mnad::ShotDev*, int, const mnad::TileRef*, int, const int*,
+                              int, int, int, int, int, hipStream_t)
 {
     return hipErrorNotSupported;
 }
@@ -94,34 +94,41 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
         h[k].npml = shapes[k].np;
     }
     std::vector<TileRef> table;
-    int ninner = -1;
-    build_tiles(&c, h, table, &ninner);
+    build_tiles(&c, h, table);
     int tw, th, spl;
     tile_shape(&c, &tw, &th, &spl);
     const int nc = interior_columns_per_lane(kernel);
     const bool x4 = nc > 0, two = two_step_kernel(kernel), q13 = kernel == 13;
     char buf[256];
     if (table.size() % 8 != 0) return "table length not a multiple of 8";
-    if (q13 ? (ninner < 0 || ninner % 8 != 0 || ninner > (int)table.size()) : ninner != 0)
-        return "interior part length wrong";
-    if (q13) {
-        for (size_t b = 0; b < table.size(); ++b) {
-            const TileRef& t = table[b];
-            if (t.shot < 0) continue;
-            const Shape& s = shapes[t.shot];
-            const ShotDev& d = h[t.shot];
-            const int tj = t.tile / d.ntx, ti = t.tile - tj * d.ntx;
-            const int x0 = ti * tw, zA = tj * th;
-            const bool inner = q_interior_tile(s.nx, s.nz, s.np, x0, zA, tw, th);
-            if (inner != ((int)b < ninner)) return "a tile in the wrong part of the table";
-            if (!inner) continue;
-            if (x0 - 4 < 0 || x0 + tw + 4 > s.nx || zA - 4 < 0 || zA + th + 4 > s.nz)
-                return "an interior tile's march loads outside the grid";
-            if (x0 - 2 < s.np || x0 + tw + 2 > s.nx - s.np || zA - 2 < s.np ||
+    // kernel 13: the tile's interior and edge waves
+    auto q_waves = [&](const TileRef& t, int* nin, int* ned) {
+        const Shape& s = shapes[t.shot];
+        const ShotDev& d = h[t.shot];
+        const int tj = t.tile / d.ntx, ti = t.tile - tj * d.ntx;
+        *nin = *ned = 0;
+        for (int x0 = ti * tw; x0 < std::min(s.nx, (ti + 1) * tw); x0 += 56) {
+            const int zA = tj * th;
+            if (!q_interior_tile(s.nx, s.nz, s.np, x0, zA, 56, th)) {
+                ++*ned;
+                continue;
+            }
+            ++*nin;
+            if (x0 - 4 < 0 || x0 + 56 + 4 > s.nx || zA - 4 < 0 || zA + th + 4 > s.nz)
+                return "an interior wave's march loads outside the grid";
+            if (x0 - 2 < s.np || x0 + 56 + 2 > s.nx - s.np || zA - 2 < s.np ||
                 zA + th + 2 > s.nz - s.np)
-                return "an interior tile's level-s+1 halo meets the CPML";
+                return "an interior wave's level-s+1 halo meets the CPML";
         }
-    }
+        return "";
+    };
+    if (q13)
+        for (const TileRef& t : table) {
+            if (t.shot < 0) continue;
+            int nin, ned;
+            const char* why = q_waves(t, &nin, &ned);
+            if (*why) return why;
+        }
 
     // coverage: every point exactly once
     std::vector<std::vector<int>> cover(shapes.size());
@@ -162,6 +169,12 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
     auto weight = [&](const TileRef& t, bool* strip) {
         const Shape& s = shapes[t.shot];
         const ShotDev& d = h[t.shot];
+        if (q13) {
+            int nin, ned;
+            q_waves(t, &nin, &ned);
+            *strip = ned > 0;
+            return nin + kQEdgeWeight * ned;
+        }
         if (x4 && t.tile >= d.nt1) {
             *strip = false;
             return c.x4_weight * nc / 4.0;
@@ -172,11 +185,8 @@ static std::string check(const char* name, int kernel, int rz, int brows, int st
                                      std::min(s.nx, (ti + 1) * tw), s);
         return two ? 1.0 : (*strip ? c.strip_weight : 1.0);
     };
-    // kernel 13: the two parts are ordered separately
-    const std::vector<std::pair<size_t, size_t>> parts =
-        q13 ? std::vector<std::pair<size_t, size_t>>{{0, (size_t)ninner}, {(size_t)ninner, table.size()}}
-            : std::vector<std::pair<size_t, size_t>>{{0, table.size()}};
-    for (const auto& part : parts) {
+    const std::pair<size_t, size_t> part{0, table.size()};
+    {
     std::vector<std::pair<int, int>> psorted;
     for (size_t b = part.first; b < part.second; ++b)
         if (table[b].shot >= 0) psorted.emplace_back(table[b].shot, table[b].tile);
@@ -245,6 +255,7 @@ int main()
         {"k13_r8_b1_c2", 13, 8, 1, 1, {c2}},
         {"k13_r4_b4_mixed", 13, 4, 4, 1, {c1, c2, tiny, nocpml}},
         {"k13_r2_b2_c5w", 13, 2, 2, 0, {c5w, c2}},
+        {"k13_r2_b4_mixed", 13, 2, 4, 1, {c2, c5w, c1, tiny, nocpml}},
     };
     int bad = 0;
     // Block heights of the resident kernel (9): the smallest height >= 4 whose
