@@ -48,17 +48,22 @@ MEASURED_TRAFFIC = {
     # C5, 20 shots, BF16 ring, groups of 10, pipelined: 2.0 x 6,480,359 +
     # 5,777,001 KiB per launch (profiles/k_stepx_c5_ring_bf16_batched.md)
     ("c5", "mnad::k_stepx<4>", "ring_bf16", 20, 10, True): 19.187e9,
+    # the same with step kernel 13 (16-row segments, 4 waves): 32,511,269 KiB per
+    # launch with its passes (profiles/k_step2q_c5_ring_bf16_batched.md)
+    ("c5", "mnad::k_step2q<4, true>", "ring_bf16", 20, 10, True): 33.291e9,
 }
 
 
-def kernel_name(t):
-    """Device symbol (as rocprofv3 prints it) of the step kernel a tuning runs."""
+def kernel_name(t, dx_eq_dz=True):
+    """Device symbol (as rocprofv3 prints it) of the step kernel a tuning runs
+    (kernel 13: the instance for grids with dx == dz when dx_eq_dz)."""
     k, r, b = t["kernel"], t["rows_per_thread"], t["block_rows"]
     return {0: f"mnad::k_step<{r}, {b}>", 1: "mnad::k_step_pf", 2: f"mnad::k_step_lds<{4 * r}>",
             3: "mnad::k_step2", 4: f"mnad::k_stepw<{b}, false>", 5: f"mnad::k_stepw<{b}, true>",
             6: f"mnad::k_stepx<{b}>", 7: f"mnad::k_step2w<{4 * r}, {b}>",
-            9: "mnad::k_res", 10: f"mnad::k_stepx2<{b}>",
-            11: "mnad::k_stepw2"}.get(k, f"kernel {k}")
+            9: "mnad::k_res", 10: f"mnad::k_stepx2<{b}>", 11: "mnad::k_stepw2",
+            12: f"mnad::k_step2z<{b}>",
+            13: f"mnad::k_step2q<{b}, {'true' if dx_eq_dz else 'false'}>"}.get(k, f"kernel {k}")
 
 
 def c2_workload(nt, npml=30):
