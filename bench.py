@@ -443,6 +443,9 @@ def run_c4(args, mnad):
     lock = threading.Lock()
     done_evt = threading.Event()
     with mnad.Context(0) as ctx:
+        if args.tuning:   # the server's one-step kernel (default: the library's)
+            k, rz, br = (int(x) for x in args.tuning.split(","))
+            ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
         ctx.serve_start(hbm_budget=int(args.budget_gb * 2**30), max_active=args.max_active,
                         chunk_steps=args.chunk, max_points=int(args.max_points))
 
@@ -497,6 +500,7 @@ def run_c4(args, mnad):
         "config": {"workload": f"C4 serving: {n} requests, Poisson {args.qps}/s, 70% modelling / 30% RTM, "
                                "grids 256x256..2048x1024 (+20 CPML), nt 1000..8000",
                    "qps": args.qps, "requests": n, "chunk_steps": args.chunk,
+                   "step_kernel": args.tuning or "4,1,1 (library default)",
                    "budget_gb": args.budget_gb, "max_active": args.max_active,
                    "max_points": args.max_points},
         "latency_ms": {"p50": round(float(np.percentile(L, 50)), 2),

@@ -182,11 +182,17 @@ struct mnad_ctx {
     mnad::DevBuf bdesc, btab, bbase;
     mnad::DevBuf bedge;   // resident step kernel: edge records of its blocks
     std::vector<mnad::GraphEntry> graphs;
-    // host -> device staging through pinned memory (upload / copy_in)
-    void* stage = nullptr;
-    size_t stage_n = 0;
-    hipEvent_t stage_ev = nullptr;
-    bool stage_pending = false;
+    // host -> device staging through pinned memory (upload / copy_in): a ring
+    // of slots, each reused once the copy that last read it has run
+    struct StageSlot {
+        void* p = nullptr;
+        size_t n = 0;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kStageSlots = 8;
+    StageSlot stage[kStageSlots];
+    int stage_next = 0;
     // step-kernel accounting (mnad_timing)
     double t_ms = 0.0;
     long long t_launches = 0;
@@ -301,11 +307,13 @@ inline bool resident_kernel(int kernel) { return kernel == 9; }
 // Output tile of the current step kernel (columns x rows) and steps per launch.
 void tile_shape(const mnad_ctx* c, int* tw, int* th, int* steps_per_launch);
 double algorithmic_bytes(const ShotDev& d);
-int download_gather(mnad_ctx* c, const Shot& s, int nt, int nr, double* out);
+// Downloads run on stream st (default: the context's stream) and wait for it.
+int download_gather(mnad_ctx* c, const Shot& s, int nt, int nr, double* out,
+                    hipStream_t st = nullptr);
 int download_field(mnad_ctx* c, const Model& m, const Shot& s, int field, double scale,
                    double* out);
 int download_interior(mnad_ctx* c, const mnad_grid& g, const float* a, const float* b,
-                      double scale, double* out);
+                      double scale, double* out, hipStream_t st = nullptr);
 int schedule(int nt, double vmin, double vmax, int* nf, int* T, int* K, int* ncol);
 int autotune(mnad_ctx* c, const Model& m, int max_shots, TunePick* out);
 void server_destroy(mnad_ctx* c);

@@ -161,31 +161,51 @@ int dalloc(mnad_ctx* c, DevBuf& b, size_t bytes, bool zero)
 int copy_in(mnad_ctx* c, void* dst, const void* src, size_t bytes)
 {
     if (bytes == 0) return MNAD_OK;
-    // one staging buffer: wait for its previous copy before overwriting it
-    if (c->stage_pending) {
-        HIPCHK(hipEventSynchronize(c->stage_ev));
-        c->stage_pending = false;
+    // A slot is free once the copy that last read it has run.  A caller that
+    // queues several uploads behind work already on the stream (the serving
+    // loop's set-up of new requests while a chunk runs) takes a free slot
+    // that is large enough, else the next slot in turn, and waits only when
+    // that one's copy has not run yet.
+    int pick = -1;
+    for (int k = 0; k < mnad_ctx::kStageSlots && pick < 0; ++k) {
+        const int q = (c->stage_next + k) % mnad_ctx::kStageSlots;
+        mnad_ctx::StageSlot& x = c->stage[q];
+        if (x.n < bytes) continue;
+        if (x.pending) {
+            const hipError_t e = hipEventQuery(x.ev);
+            if (e == hipErrorNotReady) continue;
+            HIPCHK(e);
+            x.pending = false;
+        }
+        pick = q;
     }
-    if (bytes > c->stage_n) {
-        if (c->stage) (void)hipHostFree(c->stage);
-        c->stage = nullptr;
-        c->stage_n = 0;
-        const size_t n = std::max(bytes, (size_t)4 << 20);
-        if (hipHostMalloc(&c->stage, n, hipHostMallocDefault) != hipSuccess) {
+    if (pick < 0) pick = c->stage_next;
+    c->stage_next = (pick + 1) % mnad_ctx::kStageSlots;
+    mnad_ctx::StageSlot& sl = c->stage[pick];
+    if (sl.pending) {
+        HIPCHK(hipEventSynchronize(sl.ev));
+        sl.pending = false;
+    }
+    if (bytes > sl.n) {
+        if (sl.p) (void)hipHostFree(sl.p);
+        sl.p = nullptr;
+        sl.n = 0;
+        const size_t n = std::max(bytes, (size_t)1 << 20);
+        if (hipHostMalloc(&sl.p, n, hipHostMallocDefault) != hipSuccess) {
             // no pinned memory: copy from the caller's buffer and wait for it
             (void)hipGetLastError();
-            c->stage = nullptr;
+            sl.p = nullptr;
             HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
             return MNAD_OK;
         }
-        c->stage_n = n;
+        sl.n = n;
     }
-    if (!c->stage_ev) HIPCHK(hipEventCreateWithFlags(&c->stage_ev, hipEventDisableTiming));
-    std::memcpy(c->stage, src, bytes);
-    HIPCHK(hipMemcpyAsync(dst, c->stage, bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipEventRecord(c->stage_ev, c->stream));
-    c->stage_pending = true;
+    if (!sl.ev) HIPCHK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+    std::memcpy(sl.p, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, sl.p, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(sl.ev, c->stream));
+    sl.pending = true;
     return MNAD_OK;
 }
 
@@ -206,13 +226,16 @@ void release_batch_cache(mnad_ctx* c)
     c->btab.reset();
     c->bbase.reset();
     c->bedge.reset();
-    if (c->stage_pending) (void)hipEventSynchronize(c->stage_ev);
-    c->stage_pending = false;
-    if (c->stage) (void)hipHostFree(c->stage);
-    c->stage = nullptr;
-    c->stage_n = 0;
-    if (c->stage_ev) (void)hipEventDestroy(c->stage_ev);
-    c->stage_ev = nullptr;
+    for (mnad_ctx::StageSlot& sl : c->stage) {
+        if (sl.pending) (void)hipEventSynchronize(sl.ev);
+        sl.pending = false;
+        if (sl.p) (void)hipHostFree(sl.p);
+        sl.p = nullptr;
+        sl.n = 0;
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        sl.ev = nullptr;
+    }
+    c->stage_next = 0;
 }
 
 // Grows a persistent batch buffer; graphs that read the old one are dropped.
@@ -1117,13 +1140,14 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
 
 // Gathers of a forward shot -> column-major [nt x nr] doubles (duplicated
 // receivers read the same unique point).
-int download_gather(mnad_ctx* c, const Shot& s, int nt, int nr, double* out)
+int download_gather(mnad_ctx* c, const Shot& s, int nt, int nr, double* out, hipStream_t st)
 {
     if (!out || nr == 0) return MNAD_OK;
+    if (!st) st = c->stream;
     std::vector<float> g((size_t)nt * s.nu);
     HIPCHK(hipMemcpyAsync(g.data(), s.gather.p, g.size() * sizeof(float), hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
     for (int r = 0; r < nr; ++r) {
         const int u = s.uid[r];
         for (int t = 0; t < nt; ++t) out[(size_t)t + (size_t)r * nt] = g[(size_t)t * s.nu + u];
@@ -1151,15 +1175,16 @@ int download_field(mnad_ctx* c, const Model& m, const Shot& s, int field, double
 
 // Interior row-major float array -> column-major doubles, out = scale*(a - b).
 int download_interior(mnad_ctx* c, const mnad_grid& g, const float* a, const float* b,
-                      double scale, double* out)
+                      double scale, double* out, hipStream_t st)
 {
     if (!out) return MNAD_OK;
+    if (!st) st = c->stream;
     const int nzi = g.nz - 2 * g.npml, nxi = g.nx - 2 * g.npml;
     const size_t ni = (size_t)nzi * nxi;
     std::vector<float> ha(ni), hb(b ? ni : 0);
-    HIPCHK(hipMemcpyAsync(ha.data(), a, ni * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    if (b) HIPCHK(hipMemcpyAsync(hb.data(), b, ni * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyAsync(ha.data(), a, ni * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (b) HIPCHK(hipMemcpyAsync(hb.data(), b, ni * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     for (int i = 0; i < nxi; ++i)
         for (int j = 0; j < nzi; ++j) {
             const size_t q = (size_t)j * nxi + i;

@@ -9,6 +9,14 @@
 // launch without waiting for a batch to drain, and leave it when their last
 // step has run.  A shot's arithmetic does not depend on its neighbours in the
 // launch, so each result equals the one computed alone.
+//
+// The loop keeps two chunks of steps on the stream: while the newer one runs,
+// it waits for the older one, downloads the results it finished on a second
+// stream, admits new requests (their set-up is queued behind the running
+// chunk) and queues the next chunk.  A chunk is a captured graph of its step
+// launches, replayed while the set of active shots keeps the same (padded)
+// tile count.  Device memory a finished or cancelled request gives back is
+// reused only by work queued after the chunks that may still touch it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,6 +86,7 @@ size_t field_bytes(const mnad_grid& g)
 
 struct Server {
     mnad_ctx* c = nullptr;   // private context: own stream and pool
+    hipStream_t copy = nullptr;   // downloads of finished requests, beside the next chunk
     std::thread th;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -90,18 +99,54 @@ struct Server {
     int max_active = 64, chunk = 32;
     bool stop = false;
     int64_t next_id = 1;
-    long long G = 0;   // global step counter (mirrors *dbase)
-    DevBuf dsh, dtab, dbase;
+    // Global step counter: G steps are enqueued (the device counter *dbase
+    // reaches G once they have run), G_done of them are known to have run.
+    long long G = 0, G_done = 0;
+    DevBuf dsh, dtab, dbase;   // descriptors and tile table of the active shots (grow-only)
     int ntiles = 0;
     bool changed = true;
+    // Chunks on the stream not yet waited for (at most kInflight).
+    struct Inflight {
+        long long g_end;
+        hipEvent_t ev;
+    };
+    std::deque<Inflight> inflight;
+    std::vector<hipEvent_t> spare;   // events of finished chunks, for reuse
+    // Captured chunks: `chunk` step launches and the counter advance.
+    struct ChunkGraph {
+        int kernel, rz, brows, ntiles;
+        const void* ds;
+        const void* dt;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<ChunkGraph> graphs;
 
     void run();
     int admit(Request* r);
     int start_adjoint(Request* r);
     void complete(Request* r, int rc, const char* msg);
-    int chunk_once();
+    int enqueue_chunk();
+    int wait_oldest();
+    int drain();
+    void drop_graphs(const void* ptr);
     void finish_ready();
 };
+
+constexpr int kInflight = 2;
+
+// Tile count a chunk graph is captured for: the table is padded (entries that
+// exit at once) to m 2^e with m in 4 .. 8, so that the graphs of a changing
+// mix of requests repeat.
+static size_t padded_tiles(size_t n)
+{
+    if (n <= 8) return 8;
+    size_t p = 1;
+    while (p * 8 < n) p *= 2;
+    size_t m = 4;
+    while (m * p < n) ++m;
+    return (m * p + 7) / 8 * 8;
+}
 
 // Conservative device bytes of a request at its peak.
 static size_t request_bytes(const mnad_request& q)
@@ -185,9 +230,11 @@ int Server::admit(Request* r)
     DevBuf tmp;
     rc = upload(c, tmp, h.data(), h.size() * sizeof(ShotDev));
     if (rc != MNAD_OK) return rc;
+    // enqueued behind the chunks in flight, which do not touch these shots;
+    // tmp's memory goes back to the pool, and any later use of it is ordered
+    // after this launch on the same stream
     if (mnad_launch_init_source(tmp.as<ShotDev>(), (int)h.size(), c->stream) != hipSuccess)
         return fail(MNAD_ERR_HIP, "init_source launch failed");
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(MNAD_ERR_HIP, "sync failed");
     return MNAD_OK;
 }
 
@@ -197,8 +244,8 @@ int Server::start_adjoint(Request* r)
     const mnad_request& q = r->spec;
     Shot& obs = *r->shots[0];
     Shot& syn = *r->shots[1];
-    if (q.out_gather) {
-        int rc = download_gather(c, obs, q.nt, q.nr, q.out_gather);
+    if (q.out_gather) {   // obs has run its last step: read it beside the next chunk
+        int rc = download_gather(c, obs, q.nt, q.nr, q.out_gather, copy);
         if (rc != MNAD_OK) return rc;
     }
     ShotSpec sp;
@@ -225,17 +272,56 @@ int Server::start_adjoint(Request* r)
     r->shots.clear();                       // observed forward released
     r->shots.push_back(std::move(adj));
     r->phase = 1;
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(MNAD_ERR_HIP, "sync failed");
     return MNAD_OK;
 }
 
-// One chunk of `chunk` steps over every active shot.
-int Server::chunk_once()
+void Server::drop_graphs(const void* ptr)
+{
+    for (size_t k = 0; k < graphs.size();) {
+        if (!ptr || graphs[k].ds == ptr || graphs[k].dt == ptr) {
+            (void)hipGraphExecDestroy(graphs[k].exec);
+            (void)hipGraphDestroy(graphs[k].graph);
+            graphs.erase(graphs.begin() + (long)k);
+        } else {
+            ++k;
+        }
+    }
+}
+
+// Waits for the oldest chunk in flight; G_done moves to its end.
+int Server::wait_oldest()
+{
+    if (inflight.empty()) return MNAD_OK;
+    const Inflight f = inflight.front();
+    inflight.pop_front();
+    spare.push_back(f.ev);
+    if (hipEventSynchronize(f.ev) != hipSuccess) return fail(MNAD_ERR_HIP, "step chunk failed");
+    G_done = f.g_end;
+    return MNAD_OK;
+}
+
+int Server::drain()
+{
+    while (!inflight.empty()) {
+        const int rc = wait_oldest();
+        if (rc != MNAD_OK) return rc;
+    }
+    return MNAD_OK;
+}
+
+// Enqueues one chunk of `chunk` steps over every active shot, without
+// waiting for the chunks before it.  A shot whose steps are all done exits
+// the kernel at its first check (its descriptor may still be listed until
+// the host has seen it finish).
+int Server::enqueue_chunk()
 {
     if (G > (1LL << 30)) {   // keep the device counter far from int overflow
+        int rc = drain();
+        if (rc != MNAD_OK) return rc;
         for (Request* r : active)
             for (auto& s : r->shots) s->t0 -= G;
         G = 0;
+        G_done = 0;
         if (hipMemsetAsync(dbase.p, 0, sizeof(int), c->stream) != hipSuccess)
             return fail(MNAD_ERR_HIP, "reset of the step counter failed");
         changed = true;
@@ -251,20 +337,72 @@ int Server::chunk_once()
         if (h.empty()) return MNAD_OK;
         std::vector<TileRef> table;
         build_tiles(c, h, table);
-        int rc = upload(c, dsh, h.data(), h.size() * sizeof(ShotDev));
-        if (rc == MNAD_OK) rc = upload(c, dtab, table.data(), table.size() * sizeof(TileRef));
+        table.resize(padded_tiles(table.size()), TileRef{-1, 0});
+        const size_t bh = h.size() * sizeof(ShotDev), bt = table.size() * sizeof(TileRef);
+        if (dsh.n < bh) {   // grow-only, so that captured chunks stay valid
+            drop_graphs(dsh.p);
+            int rc = dalloc(c, dsh, std::max(bh, 2 * dsh.n), false);
+            if (rc != MNAD_OK) return rc;
+        }
+        if (dtab.n < bt) {
+            drop_graphs(dtab.p);
+            int rc = dalloc(c, dtab, std::max(bt, 2 * dtab.n), false);
+            if (rc != MNAD_OK) return rc;
+        }
+        int rc = copy_in(c, dsh.p, h.data(), bh);
+        if (rc == MNAD_OK) rc = copy_in(c, dtab.p, table.data(), bt);
         if (rc != MNAD_OK) return rc;
         ntiles = (int)table.size();
         changed = false;
     }
-    for (int r = 0; r < chunk; ++r)
-        if (mnad_launch_step(dsh.as<ShotDev>(), dtab.as<TileRef>(), ntiles, dbase.as<int>(), r,
-                             c->kernel, c->rz, c->brows, c->stream) != hipSuccess)
-            return fail(MNAD_ERR_HIP, "step launch failed");
-    if (mnad_launch_advance(dbase.as<int>(), chunk, c->stream) != hipSuccess)
-        return fail(MNAD_ERR_HIP, "advance launch failed");
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(MNAD_ERR_HIP, "step sync failed");
+    const ShotDev* ds = dsh.as<ShotDev>();
+    const TileRef* dt = dtab.as<TileRef>();
+    int* base = dbase.as<int>();
+    hipGraphExec_t exec = nullptr;
+    for (const ChunkGraph& e : graphs)
+        if (e.kernel == c->kernel && e.rz == c->rz && e.brows == c->brows && e.ntiles == ntiles &&
+            e.ds == ds && e.dt == dt) {
+            exec = e.exec;
+            break;
+        }
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return fail(MNAD_ERR_HIP, "chunk capture failed to start");
+        hipError_t ce = hipSuccess;
+        for (int r = 0; r < chunk && ce == hipSuccess; ++r)
+            ce = mnad_launch_step(ds, dt, ntiles, base, r, c->kernel, c->rz, c->brows, c->stream);
+        if (ce == hipSuccess) ce = mnad_launch_advance(base, chunk, c->stream);
+        const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        if (ce != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(MNAD_ERR_HIP, "chunk capture failed");
+        }
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(MNAD_ERR_HIP, "chunk graph instantiation failed");
+        }
+        if (graphs.size() >= 16) {   // keep the most recent shapes
+            (void)hipGraphExecDestroy(graphs.front().exec);
+            (void)hipGraphDestroy(graphs.front().graph);
+            graphs.erase(graphs.begin());
+        }
+        graphs.push_back(ChunkGraph{c->kernel, c->rz, c->brows, ntiles, ds, dt, graph, exec});
+    }
+    if (hipGraphLaunch(exec, c->stream) != hipSuccess) return fail(MNAD_ERR_HIP, "chunk launch failed");
+    hipEvent_t ev = nullptr;
+    if (!spare.empty()) {
+        ev = spare.back();
+        spare.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        return fail(MNAD_ERR_HIP, "event creation failed");
+    }
+    if (hipEventRecord(ev, c->stream) != hipSuccess) {
+        spare.push_back(ev);
+        return fail(MNAD_ERR_HIP, "event record failed");
+    }
     G += chunk;
+    inflight.push_back(Inflight{G, ev});
     return MNAD_OK;
 }
 
@@ -274,7 +412,7 @@ void Server::finish_ready()
     std::vector<Request*> still;
     for (Request* r : active) {
         bool done = !r->shots.empty();
-        for (auto& s : r->shots) done = done && (G - s->t0 >= s->d.nt);
+        for (auto& s : r->shots) done = done && (G_done - s->t0 >= s->d.nt);
         if (!done) {
             still.push_back(r);
             continue;
@@ -282,7 +420,7 @@ void Server::finish_ready()
         const mnad_request& q = r->spec;
         int rc = MNAD_OK;
         if (q.kind == MNAD_REQ_MODEL) {
-            rc = download_gather(c, *r->shots[0], q.nt, q.nr, q.out_gather);
+            rc = download_gather(c, *r->shots[0], q.nt, q.nr, q.out_gather, copy);
             if (rc == MNAD_OK && q.out_gather)
                 for (size_t k = 0; k < (size_t)q.nt * q.nr; ++k)
                     if (!std::isfinite(q.out_gather[k])) {
@@ -300,10 +438,10 @@ void Server::finish_ready()
             const mnad_grid& g = q.grid;
             Shot& adj = *r->shots[0];
             rc = download_interior(c, g, adj.img.as<float>(), adj.imgc.as<float>(), (double)r->T,
-                                   q.out_image);
+                                   q.out_image, copy);
             if (rc == MNAD_OK && q.out_illum)
                 rc = download_interior(c, g, r->keep_syn->illum.as<float>(), nullptr,
-                                       (double)r->T, q.out_illum);
+                                       (double)r->T, q.out_illum, copy);
             if (rc == MNAD_OK && q.out_image) {
                 const size_t ni = (size_t)(g.nz - 2 * g.npml) * (g.nx - 2 * g.npml);
                 for (size_t k = 0; k < ni; ++k)
@@ -326,6 +464,12 @@ void Server::run()
         std::vector<Request*> admitted;
         {
             std::unique_lock<std::mutex> lk(mu);
+            if (active.empty() && queued.empty() && !stop) {
+                // idle: let the chunks still on the stream finish first
+                lk.unlock();
+                (void)drain();
+                lk.lock();
+            }
             cv_work.wait(lk, [&] { return stop || !queued.empty() || !active.empty(); });
             if (stop) break;
             // cancellations of active requests take effect at this boundary
@@ -370,9 +514,20 @@ void Server::run()
             }
         }
         if (active.empty()) continue;
-        const int rc = chunk_once();
+        // one chunk more on the stream; with kInflight chunks there, wait for
+        // the oldest and move on the requests it finished, while the newer
+        // chunk keeps the device busy
+        int rc = enqueue_chunk();
+        if (rc == MNAD_OK && (int)inflight.size() >= kInflight) rc = wait_oldest();
         if (rc != MNAD_OK) {   // device error: fail every active request, keep serving
             const std::string msg = last_error();
+            while (!inflight.empty()) {
+                (void)hipEventSynchronize(inflight.front().ev);
+                spare.push_back(inflight.front().ev);
+                inflight.pop_front();
+            }
+            (void)hipGetLastError();
+            G_done = G;
             for (Request* r : active) complete(r, rc, msg.c_str());
             active.clear();
             changed = true;
@@ -381,6 +536,7 @@ void Server::run()
         finish_ready();
     }
     // shutdown: everything still queued or active is cancelled
+    (void)drain();
     for (Request* r : active) complete(r, MNAD_ERR_CANCELLED, "server stopped");
     active.clear();
     std::lock_guard<std::mutex> lk(mu);
@@ -419,6 +575,15 @@ void server_destroy(mnad_ctx* ctx)
         s->cv_work.notify_all();
     }
     if (s->th.joinable()) s->th.join();
+    (void)hipSetDevice(s->c->device);
+    (void)hipStreamSynchronize(s->c->stream);
+    s->drop_graphs(nullptr);
+    for (const Server::Inflight& f : s->inflight) (void)hipEventDestroy(f.ev);
+    s->inflight.clear();
+    for (hipEvent_t e : s->spare) (void)hipEventDestroy(e);
+    s->spare.clear();
+    if (s->copy) (void)hipStreamDestroy(s->copy);
+    s->copy = nullptr;
     s->dsh.reset();
     s->dtab.reset();
     s->dbase.reset();
@@ -439,8 +604,10 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
     auto s = std::make_shared<Server>();
     int rc = mnad_ctx_create(ctx->device, &s->c);
     if (rc != MNAD_OK) return rc;
-    // the server steps one time level per launch (admission at any step), so
-    // the two-step and the whole-batch kernels fall back to the default
+    // The server steps one time level per launch (admission at any step), so
+    // the two-step and the whole-batch kernels fall back to the default.  In
+    // saturated C4 mixes the default row kernel (4) ran 102-104 Gpt-updates/s
+    // and the four-column kernel (6) 89, also with more points admitted.
     const bool other = two_step_kernel(ctx->kernel) || resident_kernel(ctx->kernel);
     s->c->kernel = other ? kDefaultKernel : ctx->kernel;
     s->c->rz = other ? kDefaultRows : ctx->rz;
@@ -457,6 +624,8 @@ extern "C" int mnad_serve_start(mnad_ctx* ctx, uint64_t hbm_budget, int max_acti
     rc = dalloc(s->c, s->dbase, sizeof(int), true);
     if (rc == MNAD_OK && hipStreamSynchronize(s->c->stream) != hipSuccess)
         rc = fail(MNAD_ERR_HIP, "sync failed");
+    if (rc == MNAD_OK && hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(MNAD_ERR_HIP, "copy stream creation failed");
     if (rc != MNAD_OK) {
         s->dbase.reset();
         mnad_ctx_destroy(s->c);
