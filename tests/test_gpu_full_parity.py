@@ -76,3 +76,33 @@ def test_full_size_rtm_vs_fp64(mnad, ctx, case, fmt):
     assert e["obs"] < 1e-4 and e["syn"] < 1e-4
     assert e["illum"] < 1e-4
     assert e["img"] < IMAGE_BOUND[fmt] and e["Im"] < IMAGE_BOUND[fmt]
+
+
+@pytest.mark.parametrize("case", ["c2", "c5win"])
+def test_full_size_rtm_vs_fp64_two_step_kernel(mnad, ctx, case):
+    """The C5 headline's step kernel (13: two steps per launch, 16-row
+    segments of 4 waves) with the BF16 boundary ring, against the same FP64
+    fixtures and bounds."""
+    f = golden(case + "_full")
+    wl = _workload(case)
+    v = wl["v"]
+    g = mnad.Grid(v.shape[0], v.shape[1], wl["npml"], wl["dx"], wl["dx"], wl["dt"])
+    _say(f"full-size RTM {case} {v.shape[0]}x{v.shape[1]} nt={wl['w'].size} ring_bf16, kernel 13")
+    ctx.set_tuning(kernel=13, rows_per_thread=2, block_rows=4)
+    sid = ctx.stack_create(g)
+    try:
+        dobs, dsyn = ctx.rtm_shots(sid, g, v, wl["vs"], wl["w"], [wl["zs"]], [wl["xs"]],
+                                   [(wl["zr"], wl["xr"])], snap_fmt=mnad.SNAP_RING_BF16,
+                                   batch=1, want_gathers=True)
+        img, ill, ns = ctx.stack_image(sid, g)
+    finally:
+        ctx.stack_destroy(sid)
+        ctx.set_tuning(kernel=4, rows_per_thread=1, block_rows=1)
+    sub = f["rsub"]
+    e = dict(obs=rel_l2(dobs[0][:, sub], f["dobs"]), syn=rel_l2(dsyn[0][:, sub], f["dsyn"]),
+             illum=rel_l2(ill, f["illum"]), img=rel_l2(img, f["img"]),
+             Im=rel_l2(img * ill, f["img"].astype(np.float64) * f["illum"]))
+    _say("  rel L2 vs FP64: " + ", ".join(f"{k} {x:.2e}" for k, x in e.items()))
+    assert e["obs"] < 1e-4 and e["syn"] < 1e-4
+    assert e["illum"] < 1e-4
+    assert e["img"] < 1e-3 and e["Im"] < 1e-3

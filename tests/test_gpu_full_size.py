@@ -7,10 +7,11 @@ properties that hold at any size.
   one per launch, bitwise; an RTM stack built from one batched call equals the
   stack built one shot per call, bitwise (each shot's image is added in shot
   order by the same compensated sum).
-- The kernel the autotuner picks for the grid (two rows per wavefront at C3,
-  four columns per lane at C5) agrees with the default kernel to FP32
-  rounding (1e-5 relative L2 on gathers; the gather tolerance against FP64 is
-  1e-4).
+- The step kernels picked for the grids (two rows per wavefront at C3; at C5
+  four columns per lane, and the two-step register-queue march the
+  autotuner picks there, with the BF16 boundary ring in its RTM) agree with
+  the default kernel to FP32 rounding (1e-5 relative L2 on gathers; the
+  gather tolerance against FP64 is 1e-4).
 
 Three shots per grid: on the first physical column (next to the left CPML
 strip), at the centre and on the last column, with
@@ -24,7 +25,7 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("c3", (11, 2, 1), 500), ("c5", (6, 1, 4), 400)]
+CASES = [("c3", (11, 2, 1), 500), ("c5", (6, 1, 4), 400), ("c5", (13, 2, 4), 400)]
 
 
 def _say(msg):
@@ -67,11 +68,12 @@ def test_full_size_rtm_stack_batched_equals_one_shot_per_call(mnad, name, tun, n
     wl, g = _workload(mnad, name, nt)
     v, vs, w, zs, xs, recs = wl["v"], wl["vs"], wl["w"], wl["zs"], wl["xs"], wl["recs"]
     ns = len(xs)
+    snap = mnad.SNAP_RING_BF16 if tun[0] == 13 else mnad.SNAP_F16
     ctx = mnad.Context(0)
     try:
         ctx.set_tuning(kernel=tun[0], rows_per_thread=tun[1], block_rows=tun[2])
         sa = ctx.stack_create(g)
-        ga = ctx.rtm_shots(sa, g, v, vs, w, zs, xs, recs, snap_fmt=mnad.SNAP_F16, batch=ns,
+        ga = ctx.rtm_shots(sa, g, v, vs, w, zs, xs, recs, snap_fmt=snap, batch=ns,
                            want_gathers=True)
         img_a, ill_a, n_a = ctx.stack_image(sa, g)
         ctx.stack_destroy(sa)
@@ -79,7 +81,7 @@ def test_full_size_rtm_stack_batched_equals_one_shot_per_call(mnad, name, tun, n
         gb = ([], [])
         for k in range(ns):
             gk = ctx.rtm_shots(sb, g, v, vs, w, [zs[k]], [xs[k]], [recs[k]],
-                               snap_fmt=mnad.SNAP_F16, batch=1, want_gathers=True)
+                               snap_fmt=snap, batch=1, want_gathers=True)
             gb[0].append(gk[0][0])
             gb[1].append(gk[1][0])
         img_b, ill_b, n_b = ctx.stack_image(sb, g)
