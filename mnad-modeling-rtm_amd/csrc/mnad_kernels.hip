@@ -2892,14 +2892,17 @@ __device__ __forceinline__ void q_phase2(const ShotDev& d, March<QD>& m, int s, 
 }
 
 // Level s row j into slot sl (zeros off the grid on the edge path).
+// msk: QOFF on the interior path when the row is not needed (the load is
+// dropped by the range check and reads 0).
 template <int QD, bool EDGE>
-__device__ __forceinline__ void q_load0(const ShotDev& d, March<QD>& m, int s, int j, int sl, const QCtx& qc)
+__device__ __forceinline__ void q_load0(const ShotDev& d, March<QD>& m, int s, int j, int sl,
+                                        const QCtx& qc, unsigned msk = 0u)
 {
     if (!EDGE || (j >= 0 && j < d.nz)) {
         const unsigned o = qc.b0 + (unsigned)j * qc.pitch4;
-        m.q0[0][sl] = qld(qc.rs, o, qc.vo[0]);
-        m.q0[1][sl] = qld(qc.rs, o, qc.vo[1]);
-        m.q0[2][sl] = qld(qc.rs, o, qc.vo[2]);
+        m.q0[0][sl] = qld(qc.rs, o, qc.vo[0] + msk);
+        m.q0[1][sl] = qld(qc.rs, o, qc.vo[1] + msk);
+        m.q0[2][sl] = qld(qc.rs, o, qc.vo[2] + msk);
     } else {
         m.q0[0][sl] = m.q0[1][sl] = m.q0[2][sl] = 0.0f;
     }
@@ -2911,16 +2914,26 @@ template <int QD, bool EDGE, int PH>
 __device__ __forceinline__ void q_loads(const ShotDev& d, March<QD>& m, int s, int r, int t, int NR,
                                         const QCtx& qc, bool colin)
 {
-    // The interior path loads without conditions (a branch around a load
-    // leaves the wait counts unknown after it): the rows past the segment's
-    // last are inside the plane's padding rows, and those before its first
-    // inside the grid; their values are not used.
-    if (!EDGE || t + QD < NR) {
+    const int a = r - 2 + QD;
+    constexpr int sl = qslot<QD>(PH, QD - 2);
+    if constexpr (!EDGE) {
+        // The interior path issues every load of the march without a branch
+        // (a branch around a load leaves the wait counts unknown after it).
+        // A load whose row the march does not use -- level s past row zB + 3
+        // in the last QD steps, rx and level s-1 outside rows zA - 2 .. zB + 1
+        // in the first 4 - QD and the last QD steps -- goes to the offset the
+        // range check drops, so it costs no memory traffic.
+        const bool want = t + QD < NR;
+        q_load0<QD, false>(d, m, s, r + QD, qslot<QD>(PH, QD), qc, want ? 0u : QOFF);
+        const unsigned o = (unsigned)a * qc.pitch4, mk = (want && t + QD >= 4) ? 0u : QOFF;
+        m.qr[sl] = qld(qc.rr, qc.bx + o, qc.vo[0] + mk);
+        m.qp[0][sl] = qld(qc.rs, qc.bm + o, qc.vo[0] + mk);
+        m.qp[1][sl] = qld(qc.rs, qc.bm + o, qc.vo[1] + mk);
+        m.qp[2][sl] = qld(qc.rs, qc.bm + o, qc.vo[2] + mk);
+    } else if (t + QD < NR) {
         q_load0<QD, EDGE>(d, m, s, r + QD, qslot<QD>(PH, QD), qc);
-        if (!EDGE || t + QD >= 4) {
-            const int a = r - 2 + QD;
-            constexpr int sl = qslot<QD>(PH, QD - 2);
-            if ((!EDGE || (a >= 0 && a < d.nz)) && colin) {
+        if (t + QD >= 4) {
+            if (a >= 0 && a < d.nz && colin) {
                 const unsigned o = (unsigned)a * qc.pitch4;
                 m.qr[sl] = qld(qc.rr, qc.bx + o, qc.vo[0]);
                 m.qp[0][sl] = qld(qc.rs, qc.bm + o, qc.vo[0]);
@@ -2956,12 +2969,12 @@ __device__ __forceinline__ void q_step(const ShotDev& d, March<QD>& m, int s, in
 // so that the wait counts stay exact and a step waits only for the loads
 // issued QD steps earlier.
 template <int QD, bool K1, int PH>
-__device__ __forceinline__ void q_step_main(const ShotDev& d, March<QD>& m, int s, int t, int zA, int zB,
-                                            int i, unsigned lane, const QCtx& qc, bool ownc,
-                                            float amp1, float amp2)
+__device__ __forceinline__ void q_step_main(const ShotDev& d, March<QD>& m, int s, int t, int NR,
+                                            int zA, int zB, int i, unsigned lane, const QCtx& qc,
+                                            bool ownc, float amp1, float amp2)
 {
     const int r = zA - 4 + t;
-    q_loads<QD, false, PH>(d, m, s, r, t, 0, qc, true);
+    q_loads<QD, false, PH>(d, m, s, r, t, NR, qc, true);
     q_phase1<QD, false, K1, PH, true>(d, m, s, r - 2, zA, zB, i, lane, qc, true, ownc, false, amp1, true);
     q_phase2<QD, false, K1, PH, true>(d, m, s, r - 4, i, lane, qc, ownc, false, amp2, 0.0f, 0.0f, 0.0f);
 }
@@ -2987,8 +3000,8 @@ __device__ __forceinline__ bool q_main_round(const ShotDev& d, March<QD>& m, int
                                              bool ownc, float amp1, float amp2,
                                              std::integer_sequence<int, K...>)
 {
-    return ((t + K < NR ? (q_step_main<QD, K1, (8 + K) % (5 + QD)>(d, m, s, t + K, zA, zB, i, lane,
-                                                                   qc, ownc, amp1, amp2),
+    return ((t + K < NR ? (q_step_main<QD, K1, (8 + K) % (5 + QD)>(d, m, s, t + K, NR, zA, zB, i,
+                                                                   lane, qc, ownc, amp1, amp2),
                            true)
                         : false) &&
             ...);
