@@ -43,8 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # workload, and null otherwise.
 MEASURED_TRAFFIC = {
     # C2 RTM shot, nt = 300: forward (2 shots) 69.68 MB, adjoint 35.58 MB,
-    # equal launch counts -> 52.63 MB per launch (profiles/k_stepw_c2_rtm_shot.md)
-    ("c2", "mnad::k_stepw<1, false>"): 52.63e6,
+    # equal launch counts -> 52.63 MB per launch (profiles/k_stepw_c2_rtm_shot.md,
+    # where the kernel's symbol is k_stepw<1, false>; the same code)
+    ("c2", "mnad::k_stepw<1>"): 52.63e6,
     # C5, 20 shots, BF16 ring, groups of 10, pipelined: 2.0 x 6,480,359 +
     # 5,777,001 KiB per launch (profiles/k_stepx_c5_ring_bf16_batched.md)
     ("c5", "mnad::k_stepx<4>", "ring_bf16", 20, 10, True): 19.187e9,
@@ -58,11 +59,9 @@ def kernel_name(t, dx_eq_dz=True):
     """Device symbol (as rocprofv3 prints it) of the step kernel a tuning runs
     (kernel 13: the instance for grids with dx == dz when dx_eq_dz)."""
     k, r, b = t["kernel"], t["rows_per_thread"], t["block_rows"]
-    return {0: f"mnad::k_step<{r}, {b}>", 1: "mnad::k_step_pf", 2: f"mnad::k_step_lds<{4 * r}>",
-            3: "mnad::k_step2", 4: f"mnad::k_stepw<{b}, false>", 5: f"mnad::k_stepw<{b}, true>",
+    return {0: f"mnad::k_step<{r}, {b}>", 4: f"mnad::k_stepw<{b}>",
             6: f"mnad::k_stepx<{b}>", 7: f"mnad::k_step2w<{4 * r}, {b}>",
             9: "mnad::k_res", 10: f"mnad::k_stepx2<{b}>", 11: "mnad::k_stepw2",
-            12: f"mnad::k_step2z<{b}>",
             13: f"mnad::k_step2q<{b}, {'true' if dx_eq_dz else 'false'}>"}.get(k, f"kernel {k}")
 
 

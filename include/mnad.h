@@ -199,16 +199,9 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  * a tile is 64 x (block_rows * rows_per_thread) points, one block per tile;
  * kernel 0 = neighbourhoods loaded through the vector cache: block_rows 4
  *   with rows_per_thread 1, 2 or 4, or block_rows 8 or 16 with 1 row;
- * kernel 1 = block_rows 4, 1 row, every per-point input issued first;
- * kernel 2 = block_rows 4, tiles + halos staged in LDS, rows_per_thread 1 or 2;
- * kernel 3 = two time steps per launch (block_rows 16, 1 row; 60 x 12 output
- *   tiles), shots keep four time levels -- set it before creating shots;
  * kernel 4 (default) = one grid row per wavefront, row origins in scalar
  *   registers and one shared lane offset for every load and store: block_rows
  *   1 (default), 2, 4 or 8, 1 row;
- * kernel 5 = kernel 4 with one set of CPML memories of strip points (z in
- *   z-strip rows, else x) loaded with the neighbourhood: block_rows 1 or 4,
- *   1 row;
  * kernel 6 = kernel 4 on the CPML strips, four columns per lane (16-byte
  *   loads, neighbours by cross-lane shifts) in the interior: block_rows 1, 2
  *   or 4, 1 row;
@@ -221,6 +214,11 @@ MNAD_API int mnad_timing(mnad_ctx *ctx, int reset, double *ms, int64_t *launches
  *   loads, 124 columns per wave, fewer registers): block_rows 1, 2 or 4, 1 row;
  * kernel 11 = two grid rows per wavefront (the rows both updates need are
  *   loaded once): block_rows 1, rows_per_thread 2;
+ * kernel 13 = two time steps per launch, each wavefront marching 56 columns
+ *   down a segment of 8 rows_per_thread rows (1-256) with the rows it needs
+ *   in register rings; block_rows (1, 2 or 4) wavefronts side by side per
+ *   workgroup; shots keep four time levels -- set it before creating shots;
+ *   for large batches (a launch waits for one wavefront's whole march);
  * kernel 9 = resident blocks: one cooperative launch runs every step of a
  *   batch; a workgroup of 8 wavefronts keeps a 64-column block of one shot in
  *   LDS for the whole launch and exchanges only its two-cell edges with its

@@ -47,11 +47,7 @@ static bool tuning_ok(int kernel, int rows, int brows)
 {
     if (kernel == 0 && brows == 4) return rows == 1 || rows == 2 || rows == 4;
     if (kernel == 0 && (brows == 1 || brows == 2 || brows == 8 || brows == 16)) return rows == 1;
-    if (kernel == 1) return brows == 4 && rows == 1;
-    if (kernel == 2) return brows == 4 && (rows == 1 || rows == 2);
-    if (kernel == 3) return brows == 16 && rows == 1;
     if (kernel == 4) return rows == 1 && (brows == 1 || brows == 2 || brows == 4 || brows == 8);
-    if (kernel == 5) return rows == 1 && (brows == 1 || brows == 4);
     if (kernel == 6 || kernel == 10) return rows == 1 && (brows == 1 || brows == 2 || brows == 4);
     if (kernel == 7)   // output rows 4 * rows; (rows + 1) * 4 phase-1 rows split over block_rows waves
         return (rows == 3 && (brows == 4 || brows == 8 || brows == 16)) ||
@@ -59,8 +55,6 @@ static bool tuning_ok(int kernel, int rows, int brows)
                (rows == 7 && (brows == 8 || brows == 16));
     if (kernel == 9) return brows == 8 && rows >= 1 && rows <= 4;   // rows: blocks per CU
     if (kernel == 11) return brows == 1 && rows == 2;                // two rows per wavefront
-    if (kernel == 12)   // z-marching two-step strips: segments of block_rows * rows rows
-        return (brows == 4 || brows == 8) && rows >= 2 && rows <= 512;
     if (kernel == 13)   // register-queue two-step strips: segments of 8 * rows rows
         return (brows == 1 || brows == 2 || brows == 4) && rows >= 1 && rows <= 256;
     return false;

@@ -283,7 +283,7 @@ int make_rec_shot(mnad_ctx* c, Shot& fwd, Shot& adj, Shot& rec);
 // Kernels that implement the ring code and the reverse reconstruction.
 inline bool ring_kernel(int kernel)
 {
-    return kernel == 4 || kernel == 5 || kernel == 6 || kernel == 10 || kernel == 11 || kernel == 13;
+    return kernel == 4 || kernel == 6 || kernel == 10 || kernel == 11 || kernel == 13;
 }
 void attach_snaps(Shot& adj, DevBuf&& snaps, int fmt, float sscale);
 int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data);
@@ -297,7 +297,7 @@ void build_tiles(const mnad_ctx* c, std::vector<ShotDev>& h, std::vector<TileRef
 // Step kernels that advance two time levels per launch (shots keep four).
 inline bool two_step_kernel(int kernel)
 {
-    return kernel == 3 || kernel == 7 || kernel == 12 || kernel == 13;
+    return kernel == 7 || kernel == 13;
 }
 // Columns per lane of the interior tiles of kernels 6 (4) and 10 (2); 0 for others.
 int interior_columns_per_lane(int kernel);

@@ -90,13 +90,6 @@ __global__ void k_setup_model(const double* __restrict__ v, int nz, int nx,
 }
 
 // ------------------------------------------------------------ step kernel
-__device__ __forceinline__ void load9(const float* __restrict__ F, long p, int P, Nbh& o)
-{
-    o.nw = F[p - P - 1]; o.n = F[p - P]; o.ne = F[p - P + 1];
-    o.w = F[p - 1];      o.c = F[p];     o.e = F[p + 1];
-    o.sw = F[p + P - 1]; o.s = F[p + P]; o.se = F[p + P + 1];
-}
-
 // Snapshot element store / load: a two-way branch on the storage width,
 // the half / bfloat16 choice as a select (see snap_image_w).
 __device__ __forceinline__ void store_snap(const ShotDev& d, long idx, float x)
@@ -313,209 +306,6 @@ __global__ __launch_bounds__(BX * BZR) void k_step(const ShotDev* __restrict__ s
         mnad_incr(d.ic, rx, u, v, vf[r][0], vf[r][1], w, wc[r], wc[r + 4], du, dv, dw);
         finish_point(d, s, j, i, p, u, v, w, du, dv, dw, rx, rz, PU, PV, PW);
     }
-}
-
-// LDS-staged variant: the block first copies its tile of U, Vs, Ws plus halos
-// (U: rows -1..TZ, cols -1..64; Vs: rows -1..TZ, cols -2..65; Ws: rows
-// -2..TZ+1, cols -1..64) into LDS with row-contiguous loads, then every
-// thread reads its neighbourhood from LDS (consecutive lanes, consecutive
-// banks).  About 5 global loads per point instead of 35 through the texture
-// path.  TZ rows per tile, TZ/BZ rows per thread.
-template <int TZ>
-__global__ __launch_bounds__(BX * BZ) void k_step_lds(const ShotDev* __restrict__ shots,
-                                                      const TileRef* __restrict__ tiles,
-                                                      const int* __restrict__ base, int rel)
-{
-    static_assert(TZ % BZ == 0 && TZ <= MNAD_ZPAD + 2, "tile height");
-    constexpr int UW = BX + 2, VW = BX + 4, WW = BX + 2;
-    __shared__ float su[(TZ + 2) * UW];
-    __shared__ float sv[(TZ + 2) * VW];
-    __shared__ float sw[(TZ + 4) * WW];
-
-    const TileRef tr = tiles[blockIdx.x];
-    if (tr.shot < 0) return;
-    const ShotDev& d = shots[tr.shot];
-    const int s = *base + rel - d.t0;
-    if (s < 0 || s >= d.nt) return;
-    const int tj = tr.tile / d.ntx;
-    const int ti = tr.tile - tj * d.ntx;
-    const int i0 = ti * BX, j0 = tj * TZ;
-
-    const int cb = s & 1;
-    const float* __restrict__ U = d.f[cb][0];
-    const float* __restrict__ V = d.f[cb][1];
-    const float* __restrict__ W = d.f[cb][2];
-    float* __restrict__ PU = d.f[cb ^ 1][0];
-    float* __restrict__ PV = d.f[cb ^ 1][1];
-    float* __restrict__ PW = d.f[cb ^ 1][2];
-    const int P = d.pitch, nx = d.nx, nz = d.nz;
-    const long org = (long)(j0 + MNAD_ZO) * P + (i0 + MNAD_XO);   // storage of (j0, i0)
-    const int tid = threadIdx.y * BX + threadIdx.x;
-
-    // Columns >= nx and rows >= nz are zero in storage; skipping them also
-    // keeps the loads of the right-most / bottom tiles inside the allocation.
-    for (int e = tid; e < (TZ + 2) * UW; e += BX * BZ) {
-        const int rr = e / UW, cc = e - rr * UW;
-        const int gj = j0 + rr - 1, gi = i0 + cc - 1;
-        su[e] = (gj < nz && gi < nx) ? U[org + (long)(rr - 1) * P + (cc - 1)] : 0.0f;
-    }
-    for (int e = tid; e < (TZ + 2) * VW; e += BX * BZ) {
-        const int rr = e / VW, cc = e - rr * VW;
-        const int gj = j0 + rr - 1, gi = i0 + cc - 2;
-        sv[e] = (gj < nz && gi < nx) ? V[org + (long)(rr - 1) * P + (cc - 2)] : 0.0f;
-    }
-    for (int e = tid; e < (TZ + 4) * WW; e += BX * BZ) {
-        const int rr = e / WW, cc = e - rr * WW;
-        const int gj = j0 + rr - 2, gi = i0 + cc - 1;
-        sw[e] = (gj < nz && gi < nx) ? W[org + (long)(rr - 2) * P + (cc - 1)] : 0.0f;
-    }
-    __syncthreads();
-
-    const int tx = threadIdx.x, i = i0 + tx;
-    if (i >= nx) return;
-#pragma unroll
-    for (int rr = 0; rr < TZ / BZ; ++rr) {
-        const int jl = threadIdx.y * (TZ / BZ) + rr;
-        const int j = j0 + jl;
-        if (j >= nz) break;
-        const long p = org + (long)jl * P + tx;
-        const float* a = su + (jl + 1) * UW + (tx + 1);
-        const float* b = sv + (jl + 1) * VW + (tx + 2);
-        const float* c = sw + (jl + 2) * WW + (tx + 1);
-        Nbh u, v, w;
-        u.nw = a[-UW - 1]; u.n = a[-UW]; u.ne = a[-UW + 1];
-        u.w = a[-1];       u.c = a[0];   u.e = a[1];
-        u.sw = a[UW - 1];  u.s = a[UW];  u.se = a[UW + 1];
-        v.nw = b[-VW - 1]; v.n = b[-VW]; v.ne = b[-VW + 1];
-        v.w = b[-1];       v.c = b[0];   v.e = b[1];
-        v.sw = b[VW - 1];  v.s = b[VW];  v.se = b[VW + 1];
-        w.nw = c[-WW - 1]; w.n = c[-WW]; w.ne = c[-WW + 1];
-        w.w = c[-1];       w.c = c[0];   w.e = c[1];
-        w.sw = c[WW - 1];  w.s = c[WW];  w.se = c[WW + 1];
-        const float rx = d.rx[p];
-        const float rz = rx * d.kap;
-        float du, dv, dw;
-        mnad_incr(d.ic, rx, u, v, b[-2], b[2], w, c[-2 * WW], c[2 * WW], du, dv, dw);
-        finish_point(d, s, j, i, p, u, v, w, du, dv, dw, rx, rz, PU, PV, PW);
-    }
-}
-
-// Same update as k_step<1>, with every per-point input issued before the
-// arithmetic: the receiver lookup (a chain of dependent loads), the previous
-// level, and on CPML strips the nine memories and three coefficients of each
-// axis, so a strip point costs one memory round trip instead of two.
-__global__ __launch_bounds__(BX * BZ) void k_step_pf(const ShotDev* __restrict__ shots,
-                                                     const TileRef* __restrict__ tiles,
-                                                     const int* __restrict__ base, int rel)
-{
-    const TileRef tr = tiles[blockIdx.x];
-    if (tr.shot < 0) return;
-    const ShotDev& d = shots[tr.shot];
-    const int s = *base + rel - d.t0;
-    if (s < 0 || s >= d.nt) return;
-    const int tj = tr.tile / d.ntx;
-    const int ti = tr.tile - tj * d.ntx;
-    const int i = ti * BX + threadIdx.x;
-    const int j = tj * BZ + threadIdx.y;
-    if (i >= d.nx || j >= d.nz) return;
-
-    const int cb = s & 1;
-    const float* __restrict__ U = d.f[cb][0];
-    const float* __restrict__ V = d.f[cb][1];
-    const float* __restrict__ W = d.f[cb][2];
-    float* __restrict__ PU = d.f[cb ^ 1][0];
-    float* __restrict__ PV = d.f[cb ^ 1][1];
-    float* __restrict__ PW = d.f[cb ^ 1][2];
-    const int P = d.pitch;
-    const long p = (long)(j + MNAD_ZO) * P + (i + MNAD_XO);
-    const int np = d.npml;
-    const bool inx = (i < np) || (i >= d.nx - np);
-    const bool inz = (j < np) || (j >= d.nz - np);
-
-    int r = -1;
-    if (d.nrec > 0) {
-        const int off = d.rrow[j];
-        if (off >= 0) r = d.rcol[off + i];
-    }
-    Nbh u, v, w;
-    load9(U, p, P, u);
-    load9(V, p, P, v);
-    load9(W, p, P, w);
-    const float vww = V[p - 2], vee = V[p + 2];
-    const float wnn = W[p - 2 * P], wss = W[p + 2 * P];
-    const float rx = d.rx[p];
-    const float pu = PU[p], pv = PV[p], pw = PW[p];
-
-    float cx[3] = {0.f, 0.f, 0.f}, px[9];
-    float cz[3] = {0.f, 0.f, 0.f}, pz[9];
-    long qx = 0, nsx = 0, qz = 0, nsz = 0;
-    if (inx) {
-        nsx = (long)d.nz * 2 * np;
-        qx = (long)j * 2 * np + ((i < np) ? i : i - (d.nx - 2 * np));
-#pragma unroll
-        for (int k = 0; k < 3; ++k) cx[k] = d.cfx[k * nsx + qx];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) px[k] = d.psx[k * nsx + qx];
-    }
-    if (inz) {
-        nsz = (long)2 * np * d.pitch;
-        qz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.pitch + i;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) cz[k] = d.cfz[k * nsz + qz];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) pz[k] = d.psz[k * nsz + qz];
-    }
-    float radj = 0.0f;
-    if (d.mode == MODE_ADJ && r >= 0) radj = d.radj[(long)s * d.nrec + r];
-
-    const float rz = rx * d.kap;
-    float du, dv, dw;
-    mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
-
-    if (inx || inz) {
-        // The CPML memories see the pre-injection U (fwd_PDE_exact_MNAD.m:95-111).
-        float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
-        if (d.mode == MODE_FWD && d.src_j >= 0) {
-            const float a0 = d.src_amp[s];
-            if (j == d.src_j) {
-                if (i == d.src_i) uc -= a0;
-                if (i - 1 == d.src_i) uw -= a0;
-                if (i + 1 == d.src_i) ue -= a0;
-            }
-            if (i == d.src_i) {
-                if (j - 1 == d.src_j) un -= a0;
-                if (j + 1 == d.src_j) us -= a0;
-            }
-        }
-        if (inx) {
-            du -= rx * mnad_psi(px[0], px[1], px[2], uw, uc, ue, cx[0], cx[1], cx[2]);
-            dv -= rx * mnad_psi(px[3], px[4], px[5], v.w, v.c, v.e, cx[0], cx[1], cx[2]);
-            dw -= rx * mnad_psi(px[6], px[7], px[8], w.w, w.c, w.e, cx[0], cx[1], cx[2]);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) d.psx[k * nsx + qx] = px[k];
-        }
-        if (inz) {
-            du -= rz * mnad_psi(pz[0], pz[1], pz[2], un, uc, us, cz[0], cz[1], cz[2]);
-            dv -= rz * mnad_psi(pz[3], pz[4], pz[5], v.n, v.c, v.s, cz[0], cz[1], cz[2]);
-            dw -= rz * mnad_psi(pz[6], pz[7], pz[8], w.n, w.c, w.s, cz[0], cz[1], cz[2]);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) d.psz[k * nsz + qz] = pz[k];
-        }
-    }
-
-    float nu = (2.0f * u.c + du) - pu;
-    const float nv = (2.0f * v.c + dv) - pv;
-    const float nw = (2.0f * w.c + dw) - pw;
-    if (d.mode == MODE_FWD) {
-        if (j == d.src_j && i == d.src_i) nu += d.src_amp[s + 1];
-        if (r >= 0) d.gather[(long)s * d.nrec + r] = u.c;
-    } else {
-        nu -= radj;
-    }
-    PU[p] = nu;
-    PV[p] = nv;
-    PW[p] = nw;
-    if (d.T > 0 && !inx && !inz) snap_or_image(d, s, j, i, du, nu);
 }
 
 // ------------------------------------------------------ wave-row step kernel
@@ -800,15 +590,9 @@ __device__ __forceinline__ void rec_one(const ShotDev& d, int s, int j, int i0, 
 }
 
 // One point per lane: the wave owns columns i0 .. i0+63 of row j.
-// PF: issue the CPML loads of strip points together with the neighbourhood
-// loads (one memory round trip for a strip point instead of two).  One set of
-// memories is prefetched: the z set in z-strip rows (a wave-uniform choice),
-// the x set in the other rows; corner points of z-strip rows load their x set
-// after the increment.  Holding one set across the increment arithmetic keeps
-// the kernel at 8 waves per SIMD (two sets would take it to 6).
 // SKIPX4: points of the four-column region (rows npml .. nz-npml-1, columns
 // npml .. x4hi-1) belong to step_xn and are left alone.
-template <bool PF, bool SKIPX4>
+template <bool SKIPX4>
 __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0, unsigned lane)
 {
     const int i = i0 + (int)lane;
@@ -822,11 +606,6 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
     const unsigned bx = 4u * ((i < np) ? (unsigned)i : (unsigned)(i - (d.nx - 2 * np)));
     const long nsz = (long)2 * np * d.pitch;
     const long rowz = (long)((j < np) ? j : j - (d.nz - 2 * np)) * d.pitch + i0;
-    PsiW mp;   // PF: the prefetched set (z in z-strip rows, else x)
-    if (PF) {
-        if (inz) psi_load(mp, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
-        else if (inx) psi_load(mp, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
-    }
 
     // levels s and s-1: f[s & 1] and f[(s & 1) ^ 1], selected from both sets
     // (already in SGPRs) rather than loaded by index
@@ -893,15 +672,13 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
         }
         if (inx) {
             PsiW mx;
-            if (PF && !inz) mx = mp;
-            else psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
+            psi_load(mx, gp(d.psx) + rowx, gp(d.cfx) + rowx, nsx, bx);
             psi_step(mx, rx, uw, uc, ue, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
             psi_store(mx, gp(d.psx) + rowx, nsx, bx);
         }
         if (inz) {
             PsiW mz;
-            if (PF) mz = mp;
-            else psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
+            psi_load(mz, gp(d.psz) + rowz, gp(d.cfz) + rowz, nsz, 4u * lane);
             psi_step(mz, rz, un, uc, us, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
             psi_store(mz, gp(d.psz) + rowz, nsz, 4u * lane);
         }
@@ -928,7 +705,7 @@ __device__ __forceinline__ void step_one(const ShotDev& d, int s, int j, int i0,
         snap_image_w(d, s, (long)(j - np) * (d.nx - 2 * np), (unsigned)(i - np), du, nu);
 }
 
-template <int BZR, bool PF>
+template <int BZR>
 __global__ __launch_bounds__(BX * BZR) void k_stepw(const ShotDev* __restrict__ shots,
                                                     const TileRef* __restrict__ tiles,
                                                     const int* __restrict__ base, int rel)
@@ -945,7 +722,7 @@ __global__ __launch_bounds__(BX * BZR) void k_stepw(const ShotDev* __restrict__ 
     const int j = tj * BZR + __builtin_amdgcn_readfirstlane((int)threadIdx.y);   // wave row
     if ((tr.shot < 0) | (s < 0) | (s >= d.nt) | (j >= d.nz) | (d.rx == nullptr)) return;
     if (d.mode == MODE_REC) rec_one<false>(d, s, j, ti * BX, threadIdx.x);
-    else step_one<PF, false>(d, s, j, ti * BX, threadIdx.x);
+    else step_one<false>(d, s, j, ti * BX, threadIdx.x);
 }
 
 // ------------------------------------------------ two rows per wavefront (kernel 11)
@@ -1706,7 +1483,7 @@ __global__ __launch_bounds__(BX * BZR) void k_stepx(const ShotDev* __restrict__ 
         const int j = tj * BZR + wy;
         if (j >= d.nz) return;
         if (d.mode == MODE_REC) rec_one<true>(d, s, j, ti * BX, threadIdx.x);
-        else step_one<false, true>(d, s, j, ti * BX, threadIdx.x);
+        else step_one<true>(d, s, j, ti * BX, threadIdx.x);
     } else {
         const int u = tr.tile - d.nt1;
         const int tb = u / d.nx4;
@@ -1740,7 +1517,7 @@ __global__ __launch_bounds__(BX * BZR) void k_stepx2(const ShotDev* __restrict__
         const int j = tj * BZR + wy;
         if (j >= d.nz) return;
         if (d.mode == MODE_REC) rec_one<true>(d, s, j, ti * BX, threadIdx.x);
-        else step_one<false, true>(d, s, j, ti * BX, threadIdx.x);
+        else step_one<true>(d, s, j, ti * BX, threadIdx.x);
     } else {
         const int u = tr.tile - d.nt1;
         const int tb = u / d.nx4;
@@ -1753,208 +1530,12 @@ __global__ __launch_bounds__(BX * BZR) void k_stepx2(const ShotDev* __restrict__
     }
 }
 
-// ---------------------------------------------------- two steps per launch
-// Temporal blocking: a block of 64 x 16 threads owns a 60 x 12 output tile.
-// Phase 0 stages time level s of U, Vs, Ws on the tile plus a 4-point halo in
-// LDS; phase 1 computes level s+1 on the 64 x 16 extended region (tile plus a
-// 2-point halo, one point per thread, halo points computed redundantly and
-// never written) into LDS; phase 2 computes level s+2 on the tile.  Time
-// level t lives in f[t % 4]: the launch reads levels s-1, s and writes s+1,
-// s+2, so no block overwrites a level a neighbouring block still reads.  CPML
-// memories, receivers, snapshots and imaging are handled per phase exactly as
-// in the one-step kernels (owner thread only for every global write).  The
-// CPML memories have two sets (planes 0-8 and 9-17): a launch starting at step
-// s reads set (s/2)%2 and writes the other.
-constexpr int X2 = 64, Z2 = 16;            // extended region = block shape
-constexpr int OX2 = X2 - 4, OZ2 = Z2 - 4;  // output tile
-constexpr int LX2 = X2 + 4, LZ2 = Z2 + 4;  // staged level s (extended + 2)
-
+// Neighbourhood of a point from an LDS array of row width W.
 __device__ __forceinline__ void nbh_lds(const float* a, int W, Nbh& o)
 {
     o.nw = a[-W - 1]; o.n = a[-W]; o.ne = a[-W + 1];
     o.w = a[-1];      o.c = a[0];  o.e = a[1];
     o.sw = a[W - 1];  o.s = a[W];  o.se = a[W + 1];
-}
-
-// CPML of one point for one step with the memories held in registers.
-__device__ __forceinline__ void psi_regs(const ShotDev& d, int s, int j, int i, bool inx, bool inz,
-                                         float rx, float rz, const Nbh& u, const Nbh& v,
-                                         const Nbh& w, const float* cx, float* px,
-                                         const float* cz, float* pz, float& du, float& dv,
-                                         float& dw)
-{
-    float uc = u.c, un = u.n, us = u.s, ue = u.e, uw = u.w;
-    if (d.mode == MODE_FWD && d.src_j >= 0) {
-        const float a0 = d.src_amp[s];
-        if (j == d.src_j) {
-            if (i == d.src_i) uc -= a0;
-            if (i - 1 == d.src_i) uw -= a0;
-            if (i + 1 == d.src_i) ue -= a0;
-        }
-        if (i == d.src_i) {
-            if (j - 1 == d.src_j) un -= a0;
-            if (j + 1 == d.src_j) us -= a0;
-        }
-    }
-    if (inx) {
-        du -= rx * mnad_psi(px[0], px[1], px[2], uw, uc, ue, cx[0], cx[1], cx[2]);
-        dv -= rx * mnad_psi(px[3], px[4], px[5], v.w, v.c, v.e, cx[0], cx[1], cx[2]);
-        dw -= rx * mnad_psi(px[6], px[7], px[8], w.w, w.c, w.e, cx[0], cx[1], cx[2]);
-    }
-    if (inz) {
-        du -= rz * mnad_psi(pz[0], pz[1], pz[2], un, uc, us, cz[0], cz[1], cz[2]);
-        dv -= rz * mnad_psi(pz[3], pz[4], pz[5], v.n, v.c, v.s, cz[0], cz[1], cz[2]);
-        dw -= rz * mnad_psi(pz[6], pz[7], pz[8], w.n, w.c, w.s, cz[0], cz[1], cz[2]);
-    }
-}
-
-__global__ __launch_bounds__(X2 * Z2) void k_step2(const ShotDev* __restrict__ shots,
-                                                   const TileRef* __restrict__ tiles,
-                                                   const int* __restrict__ base, int rel)
-{
-    __shared__ float cu[LZ2 * LX2], cv[LZ2 * LX2], cw[LZ2 * LX2];
-    __shared__ float nu_[Z2 * X2], nv_[Z2 * X2], nw_[Z2 * X2];
-
-    const TileRef tr = tiles[blockIdx.x];
-    if (tr.shot < 0) return;
-    const ShotDev& d = shots[tr.shot];
-    const int s = *base + rel - d.t0;
-    if (s < 0 || s >= d.nt) return;   // uniform over the block
-    // CPML memories are double-buffered across launches: halo points read
-    // them too, so an owner must not overwrite the set its neighbours read
-    const int pin = (s >> 1) & 1;
-    const int tj = tr.tile / d.ntx;
-    const int ti = tr.tile - tj * d.ntx;
-    const int x0 = ti * OX2, z0 = tj * OZ2;   // first output point
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = x0 - 2 + tx, j = z0 - 2 + ty;   // this thread's extended point
-    const int nx = d.nx, nz = d.nz, P = d.pitch, np = d.npml;
-    const bool in_grid = i >= 0 && i < nx && j >= 0 && j < nz;
-    const bool owned = in_grid && tx >= 2 && tx < X2 - 2 && ty >= 2 && ty < Z2 - 2;
-
-    const float* __restrict__ Lp[3] = {d.f[(s + 3) & 3][0], d.f[(s + 3) & 3][1], d.f[(s + 3) & 3][2]};
-    const float* __restrict__ Lc[3] = {d.f[s & 3][0], d.f[s & 3][1], d.f[s & 3][2]};
-    float* __restrict__ L1[3] = {d.f[(s + 1) & 3][0], d.f[(s + 1) & 3][1], d.f[(s + 1) & 3][2]};
-    float* __restrict__ L2[3] = {d.f[(s + 2) & 3][0], d.f[(s + 2) & 3][1], d.f[(s + 2) & 3][2]};
-
-    // ---- phase 0: stage level s (rows z0-4 .., columns x0-4 ..) in LDS
-    const int tid = ty * X2 + tx;
-    for (int e = tid; e < LZ2 * LX2; e += X2 * Z2) {
-        const int rr = e / LX2, c = e - rr * LX2;
-        const int gj = z0 - 4 + rr, gi = x0 - 4 + c;
-        const bool ok = gj >= 0 && gj < nz && gi >= 0 && gi < nx;
-        const long q = (long)(gj + MNAD_ZO) * P + (gi + MNAD_XO);
-        cu[e] = ok ? Lc[0][q] : 0.0f;
-        cv[e] = ok ? Lc[1][q] : 0.0f;
-        cw[e] = ok ? Lc[2][q] : 0.0f;
-    }
-    const long p = (long)(j + MNAD_ZO) * P + (i + MNAD_XO);
-    float pu = 0.f, pv = 0.f, pw = 0.f, rx = 0.f;
-    bool inx = false, inz = false;
-    float cx[3] = {0.f, 0.f, 0.f}, px[9], cz[3] = {0.f, 0.f, 0.f}, pz[9];
-    long qx = 0, nsx = 0, qz = 0, nsz = 0;
-    int r = -1;
-    if (in_grid) {
-        pu = Lp[0][p];
-        pv = Lp[1][p];
-        pw = Lp[2][p];
-        rx = d.rx[p];
-        inx = (i < np) || (i >= nx - np);
-        inz = (j < np) || (j >= nz - np);
-        if (inx) {
-            nsx = (long)nz * 2 * np;
-            qx = (long)j * 2 * np + ((i < np) ? i : i - (nx - 2 * np));
-#pragma unroll
-            for (int k = 0; k < 3; ++k) cx[k] = d.cfx[k * nsx + qx];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) px[k] = d.psx[(pin * 9 + k) * nsx + qx];
-        }
-        if (inz) {
-            nsz = (long)2 * np * P;
-            qz = (long)((j < np) ? j : j - (nz - 2 * np)) * P + i;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) cz[k] = d.cfz[k * nsz + qz];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) pz[k] = d.psz[(pin * 9 + k) * nsz + qz];
-        }
-        if (d.nrec > 0) {
-            const int off = d.rrow[j];
-            if (off >= 0) r = d.rcol[off + i];
-        }
-    }
-    const float rz = rx * d.kap;
-    __syncthreads();
-
-    // ---- phase 1: level s+1 on the extended region
-    float n1u = 0.f, n1v = 0.f, n1w = 0.f;
-    if (in_grid) {
-        const int l = (ty + 2) * LX2 + (tx + 2);
-        Nbh u, v, w;
-        nbh_lds(cu + l, LX2, u);
-        nbh_lds(cv + l, LX2, v);
-        nbh_lds(cw + l, LX2, w);
-        float du, dv, dw;
-        mnad_incr(d.ic, rx, u, v, cv[l - 2], cv[l + 2], w, cw[l - 2 * LX2], cw[l + 2 * LX2], du, dv,
-                  dw);
-        if (inx || inz) psi_regs(d, s, j, i, inx, inz, rx, rz, u, v, w, cx, px, cz, pz, du, dv, dw);
-        n1u = (2.0f * u.c + du) - pu;
-        n1v = (2.0f * v.c + dv) - pv;
-        n1w = (2.0f * w.c + dw) - pw;
-        if (d.mode == MODE_FWD) {
-            if (j == d.src_j && i == d.src_i) n1u += d.src_amp[s + 1];
-            if (owned && r >= 0) d.gather[(long)s * d.nrec + r] = u.c;
-        } else if (r >= 0) {
-            n1u -= d.radj[(long)s * d.nrec + r];
-        }
-        if (owned) {
-            L1[0][p] = n1u;
-            L1[1][p] = n1v;
-            L1[2][p] = n1w;
-            if (d.T > 0 && !inx && !inz) snap_or_image(d, s, j, i, du, n1u);
-        }
-    }
-    nu_[tid] = n1u;
-    nv_[tid] = n1v;
-    nw_[tid] = n1w;
-    __syncthreads();
-
-    const bool second = s + 1 < d.nt;
-    if (owned && second) {
-        // ---- phase 2: level s+2 on the tile
-        Nbh u, v, w;
-        nbh_lds(nu_ + tid, X2, u);
-        nbh_lds(nv_ + tid, X2, v);
-        nbh_lds(nw_ + tid, X2, w);
-        float du, dv, dw;
-        mnad_incr(d.ic, rx, u, v, nv_[tid - 2], nv_[tid + 2], w, nw_[tid - 2 * X2],
-                  nw_[tid + 2 * X2], du, dv, dw);
-        if (inx || inz)
-            psi_regs(d, s + 1, j, i, inx, inz, rx, rz, u, v, w, cx, px, cz, pz, du, dv, dw);
-        const int l = (ty + 2) * LX2 + (tx + 2);   // level s at this point
-        float n2u = (2.0f * u.c + du) - cu[l];
-        const float n2v = (2.0f * v.c + dv) - cv[l];
-        const float n2w = (2.0f * w.c + dw) - cw[l];
-        if (d.mode == MODE_FWD) {
-            if (j == d.src_j && i == d.src_i) n2u += d.src_amp[s + 2];
-            if (r >= 0) d.gather[(long)(s + 1) * d.nrec + r] = u.c;
-        } else if (r >= 0) {
-            n2u -= d.radj[(long)(s + 1) * d.nrec + r];
-        }
-        L2[0][p] = n2u;
-        L2[1][p] = n2v;
-        L2[2][p] = n2w;
-        if (d.T > 0 && !inx && !inz) snap_or_image(d, s + 1, j, i, du, n2u);
-    }
-    if (owned) {   // CPML memories after the last step this launch ran
-        if (inx) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) d.psx[((pin ^ 1) * 9 + k) * nsx + qx] = px[k];
-        }
-        if (inz) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) d.psz[((pin ^ 1) * 9 + k) * nsz + qz] = pz[k];
-        }
-    }
 }
 
 // ------------------------------------------- two steps per launch, wave rows
@@ -2229,9 +1810,9 @@ __global__ __launch_bounds__(BX * NW) void k_step2w(const ShotDev* __restrict__ 
     }
 }
 
-// ------------------------------------ two steps per launch, marching down z (kernel 12)
-// Snapshot / imaging work of step s (snap_image_w), decided once per block
-// instead of once per row: kind 0 none, 1 forward d2u (illumination, and the
+// ------------------------------------ snapshot / imaging work of a step
+// Snapshot / imaging work of step s (snap_image_w), decided once per pass
+// instead of once per point: kind 0 none, 1 forward d2u (illumination, and the
 // snapshot when the shot keeps them), 2 adjoint imaging, 3 reconstruction
 // hand-off slot.  `plane` is the snapshot (or slot) plane of step s.
 struct SnapStep {
@@ -2305,303 +1886,6 @@ __device__ __forceinline__ void snap_apply(const ShotDev& d, const SnapStep& st,
         ic[q] = (t - sum) - y;
         im[q] = t;
     }
-}
-
-// A workgroup of NW wavefronts owns a strip of X2W = 60 output columns and a
-// segment of `seg` rows, and marches down the segment in chunks of NW rows,
-// two time steps per launch.  Rolling LDS windows hold the rows the next
-// chunk still needs, so each level-s row is loaded from HBM once per strip
-// and each level-s+1 row computed once: only the strip's x edges are
-// recomputed (level s+1 on 64 columns for 60 outputs, level s staged on 68),
-// and only 4 halo rows per segment.  Per chunk z0:
-//   stage level s rows z0+4 .. z0+NW+3   (one row per wavefront),
-//   level s+1 on rows z0+2 .. z0+NW+1     (needs level s rows z0 .. z0+NW+3),
-//   level s+2 on rows z0 .. z0+NW-1       (needs level s+1 rows z0-2 .. z0+NW+1
-//                                          and level s rows z0-1 .. z0+NW).
-// Hence a level-s window of NW+5 rows and a level-s+1 window of NW+4.  The
-// global loads of the next chunk are issued after the staging barrier and
-// land while the two updates run.  The per-point arithmetic, CPML memories
-// (two sets, as kernel 7), injections, receivers and snapshots/imaging are
-// kernel 7's.  Everything that is uniform over a launch (array bases with
-// 32-bit row offsets, the snapshot decision of each step, source amplitudes)
-// is computed once per workgroup; rows outside the CPML skip its code.
-// Time level t lives in f[t % 4].
-template <int NW>
-__global__ __launch_bounds__(BX * NW) void k_step2z(const ShotDev* __restrict__ shots,
-                                                    const TileRef* __restrict__ tiles,
-                                                    const int* __restrict__ base, int rel, int seg)
-{
-    constexpr int R0 = NW + 5, R1 = NW + 4, W0 = BX + 4;
-    static_assert(NW >= 4, "the prologue's four level-s+1 rows take one wavefront each");
-    __shared__ float L0[3 * R0 * W0];   // level s, columns x0-4 .. x0+63
-    __shared__ float L1[3 * R1 * BX];   // level s+1, columns x0-2 .. x0+61
-    __shared__ float LR[R1 * BX];       // rx of the level-s+1 rows
-    __shared__ int LK[R1 * BX];         // receiver slot of the level-s+1 rows
-
-    const TileRef tr = tiles[blockIdx.x];
-    if (tr.shot < 0) return;
-    // The descriptor is re-read at the start of every phase: the empty asm
-    // makes the shot index opaque (readfirstlane keeps it uniform, and the
-    // base stays the read-only kernel argument, so the reads stay scalar
-    // loads), and its 36 stencil weights and field bases are then a few wide
-    // scalar loads per phase rather than loop invariants held in scalar
-    // registers across the march (which spill).
-    int shq = tr.shot;
-    auto refresh = [&]() {
-        asm volatile("" : "+s"(shq));
-        shq = __builtin_amdgcn_readfirstlane(shq);
-    };
-#define d (shots[shq])
-    const int s = *base + rel - d.t0;
-    if (s < 0 || s >= d.nt) return;   // uniform over the block
-    const int tj = tr.tile / d.ntx;
-    const int ti = tr.tile - tj * d.ntx;
-    const int x0 = ti * X2W, zA = tj * seg;
-    const int nx = d.nx, nz = d.nz, np = d.npml;
-    const int zB = min(zA + seg, nz);
-    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-    const unsigned lane = threadIdx.x;
-    const unsigned P = (unsigned)d.pitch;
-    const bool second = s + 1 < d.nt;
-    const int pin = (s >> 1) & 1;
-    const int i = x0 - 2 + (int)lane;   // this lane's column (level s+1 and s+2)
-    const bool colin = i >= 0 && i < nx;
-    const bool xint = i >= np && i < nx - np;             // interior column
-    const bool xedge = x0 - 2 < np || x0 + 62 > nx - np;  // the strip meets an x CPML strip
-    const bool fwd = d.mode == MODE_FWD;
-    // array bases at (row 0, column x0-4) and (row 0, column x0-2); a row adds j * pitch
-    const unsigned o4 = (unsigned)MNAD_ZO * P + (unsigned)(x0 - 4 + MNAD_XO), o2 = o4 + 2;
-    auto F0 = [&](int f) { return (const gfloat*)gp(d.f[s & 3][f]) + o4; };        // level s
-    auto FM = [&](int f) { return (const gfloat*)gp(d.f[(s + 3) & 3][f]) + o2; };  // level s-1
-    auto F1 = [&](int f) { return gp(d.f[(s + 1) & 3][f]) + o2; };                 // level s+1
-    auto F2 = [&](int f) { return gp(d.f[(s + 2) & 3][f]) + o2; };                 // level s+2
-    const float amp1 = fwd && d.src_j >= 0 ? d.src_amp[s + 1] : 0.0f;
-    const float amp2 = fwd && d.src_j >= 0 && second ? d.src_amp[s + 2] : 0.0f;
-    gfloat* tr1 = fwd ? gp(d.gather) + (long)s * d.nrec : nullptr;          // gather row of step s
-    const gfloat* ra1 = fwd ? nullptr : gp(d.radj) + (long)s * d.nrec;     // adjoint injection
-    const SnapStep sn1 = snap_step(d, s), sn2 = snap_step(d, s + 1);
-    const long qw = nx - 2 * np;
-    const unsigned qi = (unsigned)(i - np);
-    auto slot0 = [&](int j) { return (j - zA + 5) % R0; };   // j >= zA - 5
-    auto slot1 = [&](int j) { return (j - zA + 2) % R1; };   // j >= zA - 2
-    // level-s value of field f, row j, at this lane's column (+ offsets)
-    auto l0 = [&](int f, int j) { return L0 + (f * R0 + slot0(j)) * W0 + lane + 2; };
-    auto l1 = [&](int f, int j) { return L1 + (f * R1 + slot1(j)) * BX + lane; };
-
-    // one level-s row of the strip: loaded into registers, then stored to LDS
-    struct Row0 {
-        float a[3], e[3];
-    };
-    auto fetch0 = [&](int j) {
-        Row0 o{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-        if (j >= 0 && j < nz) {
-            const unsigned r = (unsigned)j * P;
-#pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                const gfloat* src = F0(f) + r;
-                o.a[f] = src[lane];
-                if (lane < 4) o.e[f] = src[BX + lane];
-            }
-        }
-        return o;
-    };
-    auto put0 = [&](int j, const Row0& o) {
-        const int sl = slot0(j);
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            L0[(f * R0 + sl) * W0 + lane] = o.a[f];
-            if (lane < 4) L0[(f * R0 + sl) * W0 + BX + lane] = o.e[f];
-        }
-    };
-
-    // per-point global inputs of a level-s+1 row (rx, level s-1, receiver slot)
-    struct In1 {
-        float rx, pu, pv, pw;
-        int r;
-    };
-    auto load1 = [&](int j) {
-        In1 a{0.0f, 0.0f, 0.0f, 0.0f, -1};
-        if (j >= 0 && j < nz) {
-            const unsigned r = (unsigned)j * P;
-            if (colin) {
-                a.rx = (gp(d.rx) + o2)[r + lane];
-                a.pu = FM(0)[r + lane];
-                a.pv = FM(1)[r + lane];
-                a.pw = FM(2)[r + lane];
-            }
-            if (d.nrec > 0) {
-                const int off = __builtin_amdgcn_readfirstlane(d.rrow[j]);
-                if (off >= 0 && colin) a.r = uni(gp(d.rcol) + off + x0 - 2)[lane];
-            }
-        }
-        return a;
-    };
-
-    // level s+1 at row j (a row of the segment is written out; rows outside it
-    // are the halo the level-s+2 update needs)
-    auto phase1 = [&](int j, const In1& in) {
-        float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
-        const float rx = in.rx;
-        const int r = in.r;
-        const bool rowin = j >= 0 && j < nz;
-        if (rowin && colin) {
-            Nbh u, v, w;
-            const float *un = l0(0, j - 1), *uc = l0(0, j), *us = l0(0, j + 1);
-            const float *vn = l0(1, j - 1), *vc = l0(1, j), *vs = l0(1, j + 1);
-            const float *wn = l0(2, j - 1), *wc = l0(2, j), *ws = l0(2, j + 1);
-            u.nw = un[-1]; u.n = un[0]; u.ne = un[1]; u.w = uc[-1]; u.c = uc[0]; u.e = uc[1];
-            u.sw = us[-1]; u.s = us[0]; u.se = us[1];
-            v.nw = vn[-1]; v.n = vn[0]; v.ne = vn[1]; v.w = vc[-1]; v.c = vc[0]; v.e = vc[1];
-            v.sw = vs[-1]; v.s = vs[0]; v.se = vs[1];
-            w.nw = wn[-1]; w.n = wn[0]; w.ne = wn[1]; w.w = wc[-1]; w.c = wc[0]; w.e = wc[1];
-            w.sw = ws[-1]; w.s = ws[0]; w.se = ws[1];
-            const float vww = vc[-2], vee = vc[2];
-            const float wnn = l0(2, j - 2)[0], wss = l0(2, j + 2)[0];
-            float du, dv, dw;
-            mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
-            const bool rown = j >= zA && j < zB;
-            const bool own = rown && lane >= 2 && lane < BX - 2;
-            const bool zedge = j < np || j >= nz - np;
-            if (xedge || zedge) {
-                const PsiPt q = psi_point(d, j, i, x0 - 2, lane);
-                if (q.inx || q.inz) {
-                    const float rz = rx * d.kap;
-                    float a_c = u.c, a_n = u.n, a_s = u.s, a_e = u.e, a_w = u.w;
-                    pre_injection(d, s, j, i, a_c, a_n, a_s, a_e, a_w);
-                    const bool keep = own && !second;   // a one-step launch stores the s+1 set
-                    if (q.inx) {
-                        PsiW m;
-                        psi_load(m, gp(d.psx) + pin * 9 * q.nsx + q.rowx, gp(d.cfx) + q.rowx, q.nsx, q.bx);
-                        psi_step(m, rx, a_w, a_c, a_e, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
-                        if (keep) psi_store(m, gp(d.psx) + (pin ^ 1) * 9 * q.nsx + q.rowx, q.nsx, q.bx);
-                    }
-                    if (q.inz) {
-                        PsiW m;
-                        psi_load(m, gp(d.psz) + pin * 9 * q.nsz + q.rowz, gp(d.cfz) + q.rowz, q.nsz, q.bz);
-                        psi_step(m, rz, a_n, a_c, a_s, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
-                        if (keep) psi_store(m, gp(d.psz) + (pin ^ 1) * 9 * q.nsz + q.rowz, q.nsz, q.bz);
-                    }
-                }
-            }
-            n0 = (2.0f * u.c + du) - in.pu;
-            n1 = (2.0f * v.c + dv) - in.pv;
-            n2 = (2.0f * w.c + dw) - in.pw;
-            if (fwd) {
-                if (j == d.src_j && i == d.src_i) n0 += amp1;
-                if (own && r >= 0) tr1[(unsigned)r] = u.c;
-            } else if (r >= 0) {
-                n0 -= ra1[(unsigned)r];
-            }
-            if (own) {
-                const unsigned o = (unsigned)j * P + lane;
-                F1(0)[o] = n0;
-                F1(1)[o] = n1;
-                F1(2)[o] = n2;
-                if (sn1.kind && !zedge && xint) snap_apply(d, sn1, (long)(j - np) * qw, qi, du, n0);
-            }
-        }
-        const int sl = slot1(j);
-        l1(0, j)[0] = n0;
-        l1(1, j)[0] = n1;
-        l1(2, j)[0] = n2;
-        LR[sl * BX + lane] = rx;
-        LK[sl * BX + lane] = r;
-    };
-
-    // level s+2 at row j of the segment
-    auto phase2 = [&](int j) {
-        if (j >= zB || lane < 2 || lane >= BX - 2 || !colin) return;
-        Nbh u, v, w;
-        const float *un = l1(0, j - 1), *uc = l1(0, j), *us = l1(0, j + 1);
-        const float *vn = l1(1, j - 1), *vc = l1(1, j), *vs = l1(1, j + 1);
-        const float *wn = l1(2, j - 1), *wc = l1(2, j), *ws = l1(2, j + 1);
-        u.nw = un[-1]; u.n = un[0]; u.ne = un[1]; u.w = uc[-1]; u.c = uc[0]; u.e = uc[1];
-        u.sw = us[-1]; u.s = us[0]; u.se = us[1];
-        v.nw = vn[-1]; v.n = vn[0]; v.ne = vn[1]; v.w = vc[-1]; v.c = vc[0]; v.e = vc[1];
-        v.sw = vs[-1]; v.s = vs[0]; v.se = vs[1];
-        w.nw = wn[-1]; w.n = wn[0]; w.ne = wn[1]; w.w = wc[-1]; w.c = wc[0]; w.e = wc[1];
-        w.sw = ws[-1]; w.s = ws[0]; w.se = ws[1];
-        const float vww = vc[-2], vee = vc[2];
-        const float wnn = l1(2, j - 2)[0], wss = l1(2, j + 2)[0];
-        const int sl = slot1(j);
-        const float rx = LR[sl * BX + lane];
-        const int r = LK[sl * BX + lane];
-        float du, dv, dw;
-        mnad_incr(d.ic, rx, u, v, vww, vee, w, wnn, wss, du, dv, dw);
-        const float *c0 = l0(0, j), *cv = l0(1, j), *cw = l0(2, j);   // level s at (j, i)
-        const bool zedge = j < np || j >= nz - np;
-        if (xedge || zedge) {
-            const PsiPt q = psi_point(d, j, i, x0 - 2, lane);
-            if (q.inx || q.inz) {
-                const float rz = rx * d.kap;
-                // level s neighbourhood along the axes (staged), pre-injection at step s
-                float a_c = c0[0], a_n = l0(0, j - 1)[0], a_s = l0(0, j + 1)[0], a_e = c0[1],
-                      a_w = c0[-1];
-                pre_injection(d, s, j, i, a_c, a_n, a_s, a_e, a_w);
-                float uc1 = u.c, un1 = u.n, us1 = u.s, ue1 = u.e, uw1 = u.w;
-                pre_injection(d, s + 1, j, i, uc1, un1, us1, ue1, uw1);
-                float t0, t1, t2;   // discarded increments of the repeated level s+1 update
-                if (q.inx) {
-                    PsiW m;
-                    psi_load(m, gp(d.psx) + pin * 9 * q.nsx + q.rowx, gp(d.cfx) + q.rowx, q.nsx, q.bx);
-                    t0 = t1 = t2 = 0.0f;
-                    psi_step(m, rx, a_w, a_c, a_e, cv[-1], cv[0], cv[1], cw[-1], cw[0], cw[1], t0, t1, t2);
-                    psi_step(m, rx, uw1, uc1, ue1, v.w, v.c, v.e, w.w, w.c, w.e, du, dv, dw);
-                    psi_store(m, gp(d.psx) + (pin ^ 1) * 9 * q.nsx + q.rowx, q.nsx, q.bx);
-                }
-                if (q.inz) {
-                    PsiW m;
-                    psi_load(m, gp(d.psz) + pin * 9 * q.nsz + q.rowz, gp(d.cfz) + q.rowz, q.nsz, q.bz);
-                    t0 = t1 = t2 = 0.0f;
-                    psi_step(m, rz, a_n, a_c, a_s, l0(1, j - 1)[0], cv[0], l0(1, j + 1)[0],
-                             l0(2, j - 1)[0], cw[0], l0(2, j + 1)[0], t0, t1, t2);
-                    psi_step(m, rz, un1, uc1, us1, v.n, v.c, v.s, w.n, w.c, w.s, du, dv, dw);
-                    psi_store(m, gp(d.psz) + (pin ^ 1) * 9 * q.nsz + q.rowz, q.nsz, q.bz);
-                }
-            }
-        }
-        float n0 = (2.0f * u.c + du) - c0[0];
-        const float n1 = (2.0f * v.c + dv) - cv[0];
-        const float n2 = (2.0f * w.c + dw) - cw[0];
-        if (fwd) {
-            if (j == d.src_j && i == d.src_i) n0 += amp2;
-            if (r >= 0) (tr1 + d.nrec)[(unsigned)r] = u.c;
-        } else if (r >= 0) {
-            n0 -= (ra1 + d.nrec)[(unsigned)r];
-        }
-        const unsigned o = (unsigned)j * P + lane;
-        F2(0)[o] = n0;
-        F2(1)[o] = n1;
-        F2(2)[o] = n2;
-        if (sn2.kind && !zedge && xint) snap_apply(d, sn2, (long)(j - np) * qw, qi, du, n0);
-    };
-
-    // prologue: level s rows zA-4 .. zA+3, level s+1 rows zA-2 .. zA+1
-    {
-        const In1 in = load1(zA - 2 + wv);
-        for (int q = wv; q < 8; q += NW) put0(zA - 4 + q, fetch0(zA - 4 + q));
-        __syncthreads();
-        if (wv < 4) phase1(zA - 2 + wv, in);
-        __syncthreads();
-    }
-    In1 in = load1(zA + 2 + wv);
-    Row0 st = fetch0(zA + 4 + wv);
-    for (int z0 = zA; z0 < zB; z0 += NW) {
-        put0(z0 + 4 + wv, st);
-        __syncthreads();
-        refresh();
-        const In1 cur = in;
-        if (z0 + NW < zB) {
-            in = load1(z0 + NW + 2 + wv);
-            st = fetch0(z0 + NW + 4 + wv);
-        }
-        phase1(z0 + 2 + wv, cur);
-        __syncthreads();
-        refresh();
-        if (second) phase2(z0 + wv);
-        __syncthreads();
-    }
-#undef d
 }
 
 // ------------------------------------ two steps per launch, register queues (kernel 13)
@@ -3348,20 +2632,14 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL((k_step<1, 8>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
     else if (kernel == 0 && brows == 16 && rows == 1)
         hipLaunchKernelGGL((k_step<1, 16>), g, dim3(BX, 16), 0, st, shots, tiles, base, rel);
-    else if (kernel == 1 && brows == 4 && rows == 1)
-        hipLaunchKernelGGL(k_step_pf, g, b, 0, st, shots, tiles, base, rel);
-    else if (kernel == 2 && brows == 4 && rows == 1)
-        hipLaunchKernelGGL(k_step_lds<4>, g, b, 0, st, shots, tiles, base, rel);
-    else if (kernel == 2 && brows == 4 && rows == 2)
-        hipLaunchKernelGGL(k_step_lds<8>, g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 4 && brows == 1 && rows == 1)
-        hipLaunchKernelGGL((k_stepw<1, false>), g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
+        hipLaunchKernelGGL((k_stepw<1>), g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
     else if (kernel == 4 && brows == 2 && rows == 1)
-        hipLaunchKernelGGL((k_stepw<2, false>), g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
+        hipLaunchKernelGGL((k_stepw<2>), g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
     else if (kernel == 4 && brows == 4 && rows == 1)
-        hipLaunchKernelGGL((k_stepw<4, false>), g, b, 0, st, shots, tiles, base, rel);
+        hipLaunchKernelGGL((k_stepw<4>), g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 4 && brows == 8 && rows == 1)
-        hipLaunchKernelGGL((k_stepw<8, false>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
+        hipLaunchKernelGGL((k_stepw<8>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
     else if (kernel == 6 && brows == 1 && rows == 1)
         hipLaunchKernelGGL(k_stepx<1>, g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
     else if (kernel == 6 && brows == 2 && rows == 1)
@@ -3376,10 +2654,6 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL(k_stepx2<2>, g, dim3(BX, 2), 0, st, shots, tiles, base, rel);
     else if (kernel == 10 && brows == 4 && rows == 1)
         hipLaunchKernelGGL(k_stepx2<4>, g, b, 0, st, shots, tiles, base, rel);
-    else if (kernel == 5 && brows == 1 && rows == 1)
-        hipLaunchKernelGGL((k_stepw<1, true>), g, dim3(BX, 1), 0, st, shots, tiles, base, rel);
-    else if (kernel == 5 && brows == 4 && rows == 1)
-        hipLaunchKernelGGL((k_stepw<4, true>), g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 7 && brows == 4 && rows == 3)
         hipLaunchKernelGGL((k_step2w<12, 4>), g, b, 0, st, shots, tiles, base, rel);
     else if (kernel == 7 && brows == 8 && rows == 3)
@@ -3394,12 +2668,6 @@ hipError_t mnad_launch_step(const ShotDev* shots, const TileRef* tiles, int ntil
         hipLaunchKernelGGL((k_step2w<28, 8>), g, dim3(BX, 8), 0, st, shots, tiles, base, rel);
     else if (kernel == 7 && brows == 16 && rows == 7)
         hipLaunchKernelGGL((k_step2w<28, 16>), g, dim3(BX, 16), 0, st, shots, tiles, base, rel);
-    else if (kernel == 3 && brows == 16 && rows == 1)
-        hipLaunchKernelGGL(k_step2, g, dim3(X2, Z2), 0, st, shots, tiles, base, rel);
-    else if (kernel == 12 && brows == 4 && rows >= 1)
-        hipLaunchKernelGGL(k_step2z<4>, g, dim3(BX, 4), 0, st, shots, tiles, base, rel, 4 * rows);
-    else if (kernel == 12 && brows == 8 && rows >= 1)
-        hipLaunchKernelGGL(k_step2z<8>, g, dim3(BX, 8), 0, st, shots, tiles, base, rel, 8 * rows);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

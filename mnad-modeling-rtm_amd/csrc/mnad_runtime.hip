@@ -724,17 +724,9 @@ int fill_radj_host(mnad_ctx* c, Shot& s, int nt, int nr, const double* data)
 // ------------------------------------------------------------ batch runner
 void tile_shape(const mnad_ctx* c, int* tw, int* th, int* spl)
 {
-    if (c->kernel == 3) {   // two steps per launch: 60 x 12 outputs of a 64 x 16 block
-        *tw = 60;
-        *th = 12;
-        *spl = 2;
-    } else if (c->kernel == 7) {   // two steps per launch: 60 x (4 rows_per_thread) outputs
+    if (c->kernel == 7) {   // two steps per launch: 60 x (4 rows_per_thread) outputs
         *tw = 60;
         *th = 4 * c->rz;
-        *spl = 2;
-    } else if (c->kernel == 12) {  // two steps per launch, z-marching: 60 x (block_rows x rows) segments
-        *tw = 60;
-        *th = c->brows * c->rz;
         *spl = 2;
     } else if (c->kernel == 13) {  // two steps per launch, register queues: block_rows strips of 56
         *tw = 56 * c->brows;       // columns x segments of 8 x rows rows

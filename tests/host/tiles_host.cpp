@@ -250,8 +250,6 @@ int main()
         {"k10_b2_mixed", 10, 1, 2, 1, {c1, c2, tiny, nocpml}},
         {"k7_r5_b8_c2x3", 7, 5, 8, 1, {c2, c2, c2}},
         {"k7_r3_b16_mixed", 7, 3, 16, 1, {c1, tiny, c2}},
-        {"k3_c1", 3, 1, 16, 1, {c1}},
-        {"k12_r16_b4_mixed", 12, 16, 4, 1, {c2, c1, tiny}},
         {"k13_r8_b1_c2", 13, 8, 1, 1, {c2}},
         {"k13_r4_b4_mixed", 13, 4, 4, 1, {c1, c2, tiny, nocpml}},
         {"k13_r2_b2_c5w", 13, 2, 2, 0, {c5w, c2}},

@@ -169,8 +169,8 @@ def test_batched_shots_equal_single_shots(mnad, ctx):
         assert np.array_equal(batched[k], alone["gather"]), k
 
 
-@pytest.mark.parametrize("kernel,rz,br", [(3, 1, 16), (7, 3, 16), (7, 5, 8), (6, 1, 2), (9, 2, 8),
-                                          (10, 1, 2), (11, 2, 1), (12, 2, 4), (12, 16, 4), (13, 1, 1),
+@pytest.mark.parametrize("kernel,rz,br", [(7, 3, 16), (7, 5, 8), (6, 1, 2), (9, 2, 8),
+                                          (10, 1, 2), (11, 2, 1), (13, 1, 1),
                                           (13, 2, 4)])
 def test_two_step_and_four_column_kernels_batched(mnad, kernel, rz, br):
     """Eight C1 shots in one launch sequence (several rounds of workgroups
@@ -217,13 +217,11 @@ def test_invalid_inputs_raise(mnad, ctx):
 
 
 @pytest.mark.parametrize("kernel,rz,br", [(0, 1, 4), (0, 2, 4), (0, 4, 4), (0, 1, 8), (0, 1, 16),
-                                          (1, 1, 4), (2, 1, 4), (2, 2, 4), (3, 1, 16),
                                           (4, 1, 1), (4, 1, 2), (4, 1, 4), (4, 1, 8),
-                                          (5, 1, 1), (5, 1, 4), (6, 1, 1), (6, 1, 2), (6, 1, 4),
+                                          (6, 1, 1), (6, 1, 2), (6, 1, 4),
                                           (7, 3, 4), (7, 3, 8), (7, 3, 16), (7, 5, 4), (7, 5, 8),
                                           (7, 7, 8), (7, 7, 16), (10, 1, 1), (10, 1, 2),
-                                          (10, 1, 4), (11, 2, 1), (12, 2, 4), (12, 3, 8),
-                                          (12, 16, 4), (13, 1, 1), (13, 1, 4), (13, 4, 2)])
+                                          (10, 1, 4), (11, 2, 1), (13, 1, 1), (13, 1, 4), (13, 4, 2)])
 @pytest.mark.parametrize("strip_first", [0, 1])
 def test_launch_shapes_agree(mnad, ctx, oracle, kernel, rz, br, strip_first):
     """Every kernel family / rows-per-thread / dispatch-order variant of the
@@ -245,10 +243,9 @@ def test_launch_shapes_agree(mnad, ctx, oracle, kernel, rz, br, strip_first):
         assert rel_l2(r[k], base[k]) < 1e-5, k
 
 
-@pytest.mark.parametrize("kernel,rz,br", [(3, 1, 16), (4, 1, 4), (4, 1, 1), (5, 1, 4), (6, 1, 4),
+@pytest.mark.parametrize("kernel,rz,br", [(4, 1, 4), (4, 1, 1), (6, 1, 4),
                                           (6, 1, 1), (7, 3, 4), (7, 3, 16), (7, 7, 8), (9, 1, 8),
-                                          (9, 4, 8), (10, 1, 1), (10, 1, 4), (11, 2, 1), (12, 2, 4),
-                                          (12, 8, 8), (13, 1, 1), (13, 4, 4)])
+                                          (9, 4, 8), (10, 1, 1), (10, 1, 4), (11, 2, 1), (13, 1, 1), (13, 4, 4)])
 def test_kernel_families_rtm_and_multi_tile(mnad, kernel, rz, br):
     """Each step-kernel family through the whole RTM (snapshots, imaging,
     receiver injection; f32, f16 and bf16 snapshots and the bf16 boundary ring,

@@ -159,7 +159,7 @@ def test_work_limit_admits_first_come_first_served(mnad):
         ctx.close()
 
 
-@pytest.mark.parametrize("kernel,rz,br", [(9, 2, 8), (7, 3, 16), (12, 8, 8), (13, 4, 4)])
+@pytest.mark.parametrize("kernel,rz,br", [(9, 2, 8), (7, 3, 16), (13, 4, 4)])
 def test_server_on_whole_batch_or_two_step_tuning(mnad, kernel, rz, br):
     """A context tuned to a kernel that cannot step one level per launch (the
     resident-block kernel runs a whole batch per launch, the two-step kernels

@@ -38,6 +38,6 @@ def test_tile_table_covers_balances_and_orders(tmp_path, san):
     assert b.returncode == 0, b.stderr[-4000:]
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     lines = r.stdout.strip().splitlines()
-    assert len(lines) == 25, r.stdout
+    assert len(lines) == 23, r.stdout
     bad = [ln for ln in lines if not ln.startswith("ok ")]
     assert not bad and r.returncode == 0, "\n".join(bad)
