@@ -230,3 +230,47 @@ def test_stop_and_double_wait_while_waiting(mnad):
         assert ei.value.code == mnad.ERR_STATE   # no server any more
     finally:
         ctx.close()
+
+
+def test_churn_cancellations_and_changing_mixes(mnad):
+    """Thirty-six requests of four grid sizes and lengths from 140 to 1200
+    steps, all submitted at once to a server with 16-step chunks, every fifth
+    cancelled right after it is submitted (queued or already running): the
+    active set changes at most chunk boundaries, so the padded tile count of
+    the chunk graphs keeps changing, and cancelled requests give their memory
+    back while the next chunk is already queued.  Every request that is not
+    cancelled equals the same request served alone, bitwise; a cancelled one
+    reports MNAD_ERR_CANCELLED unless it had already finished."""
+    _say("serving: churn with cancellations")
+    base = [_model_req(mnad, "fwd_rect")[1], _model_req(mnad, "fwd_square")[1],
+            _rtm_req(mnad, "rtm_two_shot", 0)[1], _rtm_req(mnad, "rtm_rect", 0)[1],
+            _rtm_req(mnad, "rtm_two_shot", 1)[1]]
+    reqs = []
+    for k in range(36):
+        r = dict(base[k % len(base)])
+        r["w"] = np.concatenate([r["w"]] * (1 + k % 6))[: 140 + 30 * k]
+        reqs.append(r)
+    ctx = mnad.Context(0)
+    try:
+        ctx.serve_start(hbm_budget=0, max_active=64, chunk_steps=16, max_points=0)
+        alone = [ctx.wait(_submit(ctx, r))[0] for r in reqs]
+        tickets = []
+        for k, r in enumerate(reqs):
+            tickets.append(_submit(ctx, r))
+            if k % 5 == 4:
+                ctx.cancel(tickets[-1])
+        n_cancelled = 0
+        for k, t in enumerate(tickets):
+            try:
+                out, info = ctx.wait(t)
+            except mnad.MnadError as e:
+                assert k % 5 == 4 and e.code == mnad.ERR_CANCELLED, (k, e)
+                n_cancelled += 1
+                continue
+            assert info["status"] == mnad.OK
+            key = "image" if reqs[k]["kind"] == mnad.REQ_RTM else "gather"
+            assert np.array_equal(out[key], alone[k][key]), k
+        assert n_cancelled >= 1
+        ctx.serve_stop()
+    finally:
+        ctx.close()
