@@ -1848,46 +1848,6 @@ __device__ __forceinline__ SnapStep snap_step(const ShotDev& d, int s)
     return o;
 }
 
-// snap_image_w's per-point work for a step decided by snap_step
-__device__ __forceinline__ void snap_apply(const ShotDev& d, const SnapStep& st, long qrow,
-                                           unsigned q, float du, float nu)
-{
-    const bool f32 = d.sfmt == SNAP_F32 || d.sfmt == SNAP_SLOT, f16 = d.sfmt == SNAP_F16;
-    if (st.kind == 3) {
-        (gp(static_cast<float*>(st.plane)) + qrow)[q] = du * d.inv_dt2;
-    } else if (st.kind == 1) {
-        const float d2u = du * d.inv_dt2;
-        gfloat* il = gp(d.illum) + qrow;
-        il[q] += d2u * d2u;
-        if (st.plane) {
-            if (f32) {
-                (gp(static_cast<float*>(st.plane)) + qrow)[q] = d2u;
-            } else {
-                const unsigned short h16 = __half_as_ushort(__float2half(d2u * d.sscale));
-                const unsigned short b16 = __bfloat16_as_ushort(__float2bfloat16(d2u));
-                ((gu16*)st.plane + qrow)[q] = f16 ? h16 : b16;
-            }
-        }
-    } else if (st.kind == 2) {
-        float sn;
-        if (f32) {
-            sn = (gp(static_cast<const float*>(st.plane)) + qrow)[q];
-        } else {
-            const unsigned short h = ((const gu16*)st.plane + qrow)[q];
-            sn = f16 ? __half2float(__ushort_as_half(h)) * d.sinv
-                     : __bfloat162float(__ushort_as_bfloat16(h));
-        }
-        const float x = sn * nu;
-        gfloat* im = gp(d.img) + qrow;
-        gfloat* ic = gp(d.imgc) + qrow;
-        const float sum = im[q], c = ic[q];
-        const float y = x - c;
-        const float t = sum + y;
-        ic[q] = (t - sum) - y;
-        im[q] = t;
-    }
-}
-
 // ------------------------------------ two steps per launch, register queues (kernel 13)
 // Each wavefront owns XQ = 56 output columns: lane l holds column x0 - 4 + l,
 // so level s is known on 64 columns, level s+1 on the 60 of lanes 2..61 and
@@ -2422,115 +2382,187 @@ __device__ __forceinline__ long q_at(const ShotDev& d, int j, int i)
     return (long)(j + MNAD_ZO) * d.pitch + (i + MNAD_XO);
 }
 
-// After a launch of steps s, s+1 of every shot, off the stored levels.
-// PASS 0: snapshots and illumination (forward), imaging (adjoint), gathers
-// and ring codes (forward).  PASS 1 (launched after PASS 0): the
+// Points per thread per iteration of the passes' row loops: a thread issues
+// every load of its QPT points before its first store, so that a wave keeps
+// several rows' worth of loads in flight.
+constexpr int QPT = 4;
+
+// Plane work of step q on the interior rows blockIdx.x, blockIdx.x +
+// gridDim.x, ... of a shot (the workgroups of blockIdx.y = shot).  KIND:
+// 1 forward d2u (illumination, and the snapshot when the shot keeps them),
+// 2 adjoint imaging, 3 the reconstruction's hand-off slot (snap_step's
+// kinds); un, uc, up: levels q + 1, q, q - 1; (sj, si, amp): the source point
+// and the source term the step added there.  Per point the arithmetic is
+// snap_image_w's.
+template <int KIND>
+__device__ __forceinline__ void q_plane(const ShotDev& d, const SnapStep& st, const float* un,
+                                        const float* uc, const float* up, int sj, int si,
+                                        float amp)
+{
+    const int np = d.npml, nxi = d.nx - 2 * np, nzi = d.nz - 2 * np;
+    if (nxi <= 0 || nzi <= 0) return;
+    const bool f32 = d.sfmt == SNAP_F32 || d.sfmt == SNAP_SLOT, f16 = d.sfmt == SNAP_F16;
+    const int t = threadIdx.x;
+    for (int jj = blockIdx.x; jj < nzi; jj += gridDim.x) {
+        const long o0 = q_at(d, np + jj, np);
+        const long qrow = (long)jj * nxi;
+        const int isrc = np + jj == sj ? si - np : -1;   // the source column on this row
+        const gfloat* a = gp(un) + o0;
+        const gfloat* b = gp(uc) + o0;
+        const gfloat* c = gp(up) + o0;
+        for (int i0 = t; i0 < nxi; i0 += 256 * QPT) {
+            float vn[QPT], vc[QPT], vp[QPT], x0[QPT], x1[QPT];
+#pragma unroll
+            for (int k = 0; k < QPT; ++k) {
+                const int ii = i0 + 256 * k;
+                if (ii >= nxi) continue;
+                vn[k] = a[ii];
+                if (KIND == 2) {
+                    if (f32) {
+                        x0[k] = (gp(static_cast<const float*>(st.plane)) + qrow)[ii];
+                    } else {
+                        const unsigned short h = ((const gu16*)st.plane + qrow)[ii];
+                        x0[k] = f16 ? __half2float(__ushort_as_half(h)) * d.sinv
+                                    : __bfloat162float(__ushort_as_bfloat16(h));
+                    }
+                    vc[k] = (gp(d.img) + qrow)[ii];
+                    vp[k] = (gp(d.imgc) + qrow)[ii];
+                } else {
+                    vc[k] = b[ii];
+                    vp[k] = c[ii];
+                    if (KIND == 1) x1[k] = (gp(d.illum) + qrow)[ii];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < QPT; ++k) {
+                const int ii = i0 + 256 * k;
+                if (ii >= nxi) continue;
+                if (KIND == 2) {
+                    const float x = x0[k] * vn[k];
+                    const float sum = vc[k], cc = vp[k];
+                    const float y = x - cc;
+                    const float tt = sum + y;
+                    (gp(d.imgc) + qrow)[ii] = (tt - sum) - y;
+                    (gp(d.img) + qrow)[ii] = tt;
+                    continue;
+                }
+                float du = (vn[k] - 2.0f * vc[k]) + vp[k];
+                if (ii == isrc) du -= amp;
+                const float d2u = du * d.inv_dt2;
+                if (KIND == 3) {
+                    (gp(static_cast<float*>(st.plane)) + qrow)[ii] = d2u;
+                    continue;
+                }
+                (gp(d.illum) + qrow)[ii] = x1[k] + d2u * d2u;
+                if (st.plane) {
+                    if (f32) {
+                        (gp(static_cast<float*>(st.plane)) + qrow)[ii] = d2u;
+                    } else {
+                        const unsigned short h16 = __half_as_ushort(__float2half(d2u * d.sscale));
+                        const unsigned short b16 = __bfloat16_as_ushort(__float2bfloat16(d2u));
+                        ((gu16*)st.plane + qrow)[ii] = f16 ? h16 : b16;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// After a launch of steps s, s+1 of every shot, off the stored levels; the
+// workgroups of blockIdx.y = k work on shot k, so that every shot's work runs
+// at once.  PASS 0: snapshots and illumination (forward), imaging (adjoint),
+// gathers and ring codes (forward).  PASS 1 (launched after PASS 0): the
 // reconstruction's hand-off slot, so that an adjoint reads a slot before the
 // reconstruction overwrites it.  The increment of step q at a point is the
 // stored second difference of the levels, (U(q+1) - 2 U(q)) + U(q-1), less
 // the source term at the source point: the update's own increment up to
 // float rounding of the levels (the reverse pass and the snapshots need it to
-// about 1e-5, the image tolerance being 1e-3).  Every workgroup loops over
-// the shots and takes a slice of each shot's points.
+// about 1e-5, the image tolerance being 1e-3).
 template <int PASS>
-__global__ __launch_bounds__(256) void k_q_post(const ShotDev* __restrict__ shots, int nshots,
+__global__ __launch_bounds__(256) void k_q_post(const ShotDev* __restrict__ shots,
                                                 const int* __restrict__ base, int rel)
 {
+    const ShotDev& d = shots[blockIdx.y];
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;   // uniform
+    const bool second = s + 1 < d.nt;
+    for (int h = 0; h < (second ? 2 : 1); ++h) {
+        const int q = s + h;   // step q turns level q into level q+1
+        const SnapStep st = snap_step(d, q);
+        if (PASS == 0 ? !(st.kind == 1 || st.kind == 2) : st.kind != 3) continue;
+        const float* un = q_level(d, s, h + 1, 0);
+        const float* uc = q_level(d, s, h, 0);
+        const float* up = q_level(d, s, h - 1, 0);
+        int sj = -1, si = -1;
+        float amp = 0.0f;
+        if (d.src_j >= 0 && d.mode != MODE_ADJ) {
+            sj = d.src_j;
+            si = d.src_i;
+            amp = d.mode == MODE_REC ? d.src_amp[d.ntf - q] : d.src_amp[q + 1];
+        }
+        if (PASS == 1) q_plane<3>(d, st, un, uc, up, sj, si, amp);
+        else if (st.kind == 1) q_plane<1>(d, st, un, uc, up, sj, si, amp);
+        else q_plane<2>(d, st, un, uc, up, sj, si, amp);
+    }
+    if (PASS != 0 || d.mode != MODE_FWD) return;
+    const int np = d.npml;
     const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nth = (long)gridDim.x * blockDim.x;
-    for (int k = 0; k < nshots; ++k) {
-        const ShotDev& d = shots[k];
-        const int s = *base + rel - d.t0;
-        if (s < 0 || s >= d.nt) continue;   // uniform
-        const bool second = s + 1 < d.nt;
-        const int np = d.npml, nxi = d.nx - 2 * np, nzi = d.nz - 2 * np;
-        const long ni = nxi > 0 && nzi > 0 ? (long)nzi * nxi : 0;
-        for (int h = 0; h < (second ? 2 : 1); ++h) {
-            const int q = s + h;   // step q turns level q into level q+1
-            const SnapStep st = snap_step(d, q);
-            if (PASS == 0 ? !(st.kind == 1 || st.kind == 2) : st.kind != 3) continue;
-            const float* un = q_level(d, s, h + 1, 0);
-            const float* uc = q_level(d, s, h, 0);
-            const float* up = q_level(d, s, h - 1, 0);
-            int sj = -1, si = -1;
-            float amp = 0.0f;
-            if (d.src_j >= 0 && d.mode != MODE_ADJ) {
-                sj = d.src_j;
-                si = d.src_i;
-                amp = d.mode == MODE_REC ? d.src_amp[d.ntf - q] : d.src_amp[q + 1];
-            }
-            for (long p = tid; p < ni; p += nth) {
-                const int jj = (int)(p / nxi), ii = (int)(p - (long)jj * nxi);
-                const int j = np + jj, i = np + ii;
-                const long o = q_at(d, j, i);
-                const float nu = un[o];
-                float du = 0.0f;
-                if (st.kind != 2) {
-                    du = (nu - 2.0f * uc[o]) + up[o];
-                    if (j == sj && i == si) du -= amp;
-                }
-                snap_apply(d, st, (long)jj * nxi, (unsigned)ii, du, nu);
+    if (d.nrec > 0) {   // gathers: step q records level q at each receiver
+        const float* l0 = q_level(d, s, 0, 0);
+        const float* l1 = q_level(d, s, 1, 0);
+        for (int j = d.rlo; j <= d.rhi; ++j) {
+            const int off = d.rrow[j];
+            if (off < 0) continue;
+            for (long i = tid; i < d.nx; i += nth) {
+                const int r = d.rcol[off + i];
+                if (r < 0) continue;
+                const long o = q_at(d, j, (int)i);
+                d.gather[(long)s * d.nrec + r] = l0[o];
+                if (second) d.gather[(long)(s + 1) * d.nrec + r] = l1[o];
             }
         }
-        if (PASS != 0 || d.mode != MODE_FWD) continue;
-        if (d.nrec > 0) {   // gathers: step q records level q at each receiver
-            const float* l0 = q_level(d, s, 0, 0);
-            const float* l1 = q_level(d, s, 1, 0);
-            for (int j = d.rlo; j <= d.rhi; ++j) {
-                const int off = d.rrow[j];
-                if (off < 0) continue;
-                for (long i = tid; i < d.nx; i += nth) {
-                    const int r = d.rcol[off + i];
-                    if (r < 0) continue;
-                    const long o = q_at(d, j, (int)i);
-                    d.gather[(long)s * d.nrec + r] = l0[o];
-                    if (second) d.gather[(long)(s + 1) * d.nrec + r] = l1[o];
-                }
-            }
-        }
-        if (d.rord > 0) {   // ring codes of levels s+1 and s+2, in order per cell
-            for (long r = tid; r < d.nring; r += nth) {
-                int j, i;
-                ring_point((int)r, d.nz, d.nx, np, &j, &i);
-                const long o = q_at(d, j, i);
-                ring_put(d, s + 1, (unsigned)r, q_level(d, s, 1, 0)[o], q_level(d, s, 1, 1)[o],
-                         q_level(d, s, 1, 2)[o]);
-                if (second)
-                    ring_put(d, s + 2, (unsigned)r, q_level(d, s, 2, 0)[o], q_level(d, s, 2, 1)[o],
-                             q_level(d, s, 2, 2)[o]);
-            }
+    }
+    if (d.rord > 0) {   // ring codes of levels s+1 and s+2, in order per cell
+        for (long r = tid; r < d.nring; r += nth) {
+            int j, i;
+            ring_point((int)r, d.nz, d.nx, np, &j, &i);
+            const long o = q_at(d, j, i);
+            ring_put(d, s + 1, (unsigned)r, q_level(d, s, 1, 0)[o], q_level(d, s, 1, 1)[o],
+                     q_level(d, s, 1, 2)[o]);
+            if (second)
+                ring_put(d, s + 2, (unsigned)r, q_level(d, s, 2, 0)[o], q_level(d, s, 2, 1)[o],
+                         q_level(d, s, 2, 2)[o]);
         }
     }
 }
 
 // Before a launch of steps s, s+1: a reconstruction's ring cells get levels
 // c-1 and c-2 (c = ntf - 1 - s) decoded from the saved ring, in order per
-// cell (rec_one's decode, two levels at a time).
-__global__ __launch_bounds__(256) void k_q_pre(const ShotDev* __restrict__ shots, int nshots,
+// cell (rec_one's decode, two levels at a time); blockIdx.y = shot.
+__global__ __launch_bounds__(256) void k_q_pre(const ShotDev* __restrict__ shots,
                                                const int* __restrict__ base, int rel)
 {
+    const ShotDev& d = shots[blockIdx.y];
+    if (d.mode != MODE_REC || d.rord <= 0) return;
+    const int s = *base + rel - d.t0;
+    if (s < 0 || s >= d.nt) return;
+    const bool second = s + 1 < d.nt;
+    const int c = d.ntf - 1 - s;
     const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nth = (long)gridDim.x * blockDim.x;
-    for (int k = 0; k < nshots; ++k) {
-        const ShotDev& d = shots[k];
-        if (d.mode != MODE_REC || d.rord <= 0) continue;
-        const int s = *base + rel - d.t0;
-        if (s < 0 || s >= d.nt) continue;
-        const bool second = s + 1 < d.nt;
-        const int c = d.ntf - 1 - s;
-        for (long r = tid; r < d.nring; r += nth) {
-            int j, i;
-            ring_point((int)r, d.nz, d.nx, d.npml, &j, &i);
-            const long o = q_at(d, j, i);
-            float v[3];
-            ring_get(d, c + 1, (unsigned)r, v);
+    for (long r = tid; r < d.nring; r += nth) {
+        int j, i;
+        ring_point((int)r, d.nz, d.nx, d.npml, &j, &i);
+        const long o = q_at(d, j, i);
+        float v[3];
+        ring_get(d, c + 1, (unsigned)r, v);
 #pragma unroll
-            for (int f = 0; f < 3; ++f) ((float*)q_level(d, s, 1, f))[o] = v[f];
-            if (second) {
-                ring_get(d, c, (unsigned)r, v);
+        for (int f = 0; f < 3; ++f) ((float*)q_level(d, s, 1, f))[o] = v[f];
+        if (second) {
+            ring_get(d, c, (unsigned)r, v);
 #pragma unroll
-                for (int f = 0; f < 3; ++f) ((float*)q_level(d, s, 2, f))[o] = v[f];
-            }
+            for (int f = 0; f < 3; ++f) ((float*)q_level(d, s, 2, f))[o] = v[f];
         }
     }
 }
@@ -2727,8 +2759,13 @@ hipError_t mnad_launch_step2q(const ShotDev* shots, int nshots, const TileRef* t
     if (rows < 1 || ntiles < 0) return hipErrorInvalidValue;
     if (brows != 1 && brows != 2 && brows != 4) return hipErrorInvalidValue;
     const int seg = 8 * rows;
-    const dim3 pg(1024), pb(256);
-    if (passes & 1) hipLaunchKernelGGL(k_q_pre, pg, pb, 0, st, shots, nshots, base, rel);
+    // passes: blockIdx.y = shot; the workgroups per shot shrink as the batch
+    // grows (about 8192 in all, 64 .. 2048 per shot)
+    const dim3 pg(std::min(2048, std::max(64, (8192 + nshots - 1) / std::max(nshots, 1))),
+                  std::max(nshots, 0)),
+        pb(256);
+    if (nshots <= 0) passes = 0;
+    if (passes & 1) hipLaunchKernelGGL(k_q_pre, pg, pb, 0, st, shots, base, rel);
     if (ntiles > 0) {
 #define MNAD_Q2(NW)                                                                            \
     do {                                                                                       \
@@ -2746,8 +2783,8 @@ hipError_t mnad_launch_step2q(const ShotDev* shots, int nshots, const TileRef* t
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (passes & 2) hipLaunchKernelGGL(k_q_post<0>, pg, pb, 0, st, shots, nshots, base, rel);
-    if (passes & 4) hipLaunchKernelGGL(k_q_post<1>, pg, pb, 0, st, shots, nshots, base, rel);
+    if (passes & 2) hipLaunchKernelGGL(k_q_post<0>, pg, pb, 0, st, shots, base, rel);
+    if (passes & 4) hipLaunchKernelGGL(k_q_post<1>, pg, pb, 0, st, shots, base, rel);
     return hipGetLastError();
 }
 
