@@ -49,9 +49,9 @@ MEASURED_TRAFFIC = {
     # C5, 20 shots, BF16 ring, groups of 10, pipelined: 2.0 x 6,480,359 +
     # 5,777,001 KiB per launch (profiles/k_stepx_c5_ring_bf16_batched.md)
     ("c5", "mnad::k_stepx<4>", "ring_bf16", 20, 10, True): 19.187e9,
-    # the same with step kernel 13 (16-row segments, 4 waves): 30,744,540 KiB per
+    # the same with step kernel 13 (16-row segments, 4 waves): 30,835,394 KiB per
     # launch with its passes (profiles/k_step2q_c5_ring_bf16_batched.md)
-    ("c5", "mnad::k_step2q<4, true>", "ring_bf16", 20, 10, True): 31.482e9,
+    ("c5", "mnad::k_step2q<4, true>", "ring_bf16", 20, 10, True): 31.575e9,
 }
 
 
