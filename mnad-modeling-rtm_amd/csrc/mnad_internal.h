@@ -96,8 +96,12 @@ struct TunePick {
 // One cached launch sequence of run_batch: a graph chunk of step launches
 // that reads the batch descriptors from the context's persistent buffers, so
 // it can be replayed for any batch with the same kernel and tile count.
+// A captured chunk of step launches.  Its key is everything the capture
+// bakes into the launches: the kernel and shape, the tile count, the batch's
+// shot count, dx == dz (kernel 13) and which passes run around each launch
+// (kernel 13), and the device buffers the launches read.
 struct GraphEntry {
-    int kern, rz, brows, ntiles, chunk, spl;
+    int kern, rz, brows, ntiles, chunk, spl, nshots, k1, passes;
     const void* ds;
     const void* dt;
     const int* base;

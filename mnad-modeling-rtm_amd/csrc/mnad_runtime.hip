@@ -1086,8 +1086,8 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
         for (size_t k = 0; k < c->graphs.size(); ++k) {
             const GraphEntry& e = c->graphs[k];
             if (e.kern == kern && e.rz == rz && e.brows == br && e.ntiles == ntiles &&
-                e.chunk == chunk && e.spl == spl && e.ds == ds &&
-                e.dt == dt && e.base == base) {
+                e.chunk == chunk && e.spl == spl && e.nshots == n && e.k1 == k1 &&
+                e.passes == passes && e.ds == ds && e.dt == dt && e.base == base) {
                 exec = e.exec;
                 break;
             }
@@ -1112,8 +1112,8 @@ int run_batch(mnad_ctx* c, const std::vector<Shot*>& shots, int flags)
                 (void)hipGraphDestroy(c->graphs.front().graph);
                 c->graphs.erase(c->graphs.begin());
             }
-            c->graphs.push_back(
-                GraphEntry{kern, rz, br, ntiles, chunk, spl, ds, dt, base, graph, exec});
+            c->graphs.push_back(GraphEntry{kern, rz, br, ntiles, chunk, spl, n, k1, passes, ds, dt,
+                                           base, graph, exec});
         }
         const int full = nlaunch / chunk, rem = nlaunch - full * chunk;
         HIPCHK(hipEventRecord(ev0, c->stream));   // after instantiation

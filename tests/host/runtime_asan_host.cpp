@@ -18,11 +18,17 @@
 // ring, step kernels 4 and 13), batched modelling, the autotuner, and the
 // serving loop with a mix of requests.  Each step prints "ok <step>" or
 // "FAIL <step>: <message>"; the exit status is the number of failures.
+// Kernel-13 graph chunks: the stub of its launcher records what a capture
+// bakes into a chunk (shot count, dx == dz, the passes around each launch,
+// the tile count), and a replayed chunk must match the direct launches that
+// follow it in the same launch sequence (the chunk cache must not hand a
+// sequence a chunk captured for another batch with the same tile count).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -49,6 +55,19 @@ hipError_t fake_delete(T p)
     return hipSuccess;
 }
 bool g_capturing = false;
+// kernel-13 launches: what a captured chunk holds, and the chunk replayed last
+struct Baked {
+    int nshots = -1, k1 = -1, passes = -1, ntiles = -1;
+    bool operator!=(const Baked& o) const
+    {
+        return nshots != o.nshots || k1 != o.k1 || passes != o.passes || ntiles != o.ntiles;
+    }
+};
+Baked g_cap;                     // the capture in progress
+std::map<void*, Baked> g_baked;  // graph or exec -> its kernel-13 launches
+Baked g_pending;                 // the chunk replayed last in this sequence
+bool g_pending_set = false;
+int g_replays_checked = 0, g_replay_mismatches = 0;
 }  // namespace
 
 extern "C" {
@@ -94,22 +113,29 @@ hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode)
 {
     g_capturing = true;
+    g_cap = Baked{};
+    g_pending_set = false;
     return hipSuccess;
 }
 hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g)
 {
     g_capturing = false;
     *g = fake_new<hipGraph_t>();
+    g_baked[(void*)*g] = g_cap;
     return hipSuccess;
 }
-hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t, hipGraphNode_t*, char*, size_t)
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t)
 {
     *e = fake_new<hipGraphExec_t>();
+    g_baked[(void*)*e] = g_baked[(void*)g];
     return hipSuccess;
 }
 hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t)
 {
-    return reinterpret_cast<FakeObj*>(e)->magic == 0x5eed ? hipSuccess : hipErrorInvalidValue;
+    if (reinterpret_cast<FakeObj*>(e)->magic != 0x5eed) return hipErrorInvalidValue;
+    g_pending = g_baked[(void*)e];
+    g_pending_set = g_pending.nshots >= 0;
+    return hipSuccess;
 }
 hipError_t hipGraphDestroy(hipGraph_t g) { return fake_delete(g); }
 hipError_t hipGraphExecDestroy(hipGraphExec_t e) { return fake_delete(e); }
@@ -120,12 +146,17 @@ hipError_t hipEventCreate(hipEvent_t* e)
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t e) { return fake_delete(e); }
-hipError_t hipEventRecord(hipEvent_t e, hipStream_t)
+static hipError_t event_ok(hipEvent_t e)
 {
     return reinterpret_cast<FakeObj*>(e)->magic == 0x5eed ? hipSuccess : hipErrorInvalidValue;
 }
-hipError_t hipEventSynchronize(hipEvent_t e) { return hipEventRecord(e, nullptr); }
-hipError_t hipEventQuery(hipEvent_t e) { return hipEventRecord(e, nullptr); }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t)
+{
+    g_pending_set = false;   // a launch sequence ends with an event record
+    return event_ok(e);
+}
+hipError_t hipEventSynchronize(hipEvent_t e) { return event_ok(e); }
+hipError_t hipEventQuery(hipEvent_t e) { return event_ok(e); }
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t)
 {
     *ms = 0.25f;
@@ -159,8 +190,19 @@ hipError_t mnad_launch_setup_model(const double*, int, int, int, int, double, do
                                    float*, float*, float*, hipStream_t) { return hipSuccess; }
 hipError_t mnad_launch_step(const ShotDev*, const TileRef*, int, const int*, int, int, int, int,
                             hipStream_t) { return hipSuccess; }
-hipError_t mnad_launch_step2q(const ShotDev*, int, const TileRef*, int, const int*, int, int, int,
-                              int, int, hipStream_t) { return hipSuccess; }
+hipError_t mnad_launch_step2q(const ShotDev*, int nshots, const TileRef*, int ntiles, const int*,
+                              int, int, int, int k1, int passes, hipStream_t)
+{
+    const Baked b{nshots, k1, passes, ntiles};
+    if (g_capturing) {
+        g_cap = b;
+    } else if (g_pending_set) {   // the first direct launch after a replayed chunk
+        ++g_replays_checked;
+        if (g_pending != b) ++g_replay_mismatches;
+        g_pending_set = false;
+    }
+    return hipSuccess;
+}
 hipError_t mnad_launch_advance(int*, int, hipStream_t) { return hipSuccess; }
 hipError_t mnad_launch_init_source(ShotDev*, int, hipStream_t) { return hipSuccess; }
 hipError_t mnad_launch_make_radj(const float*, const float*, const float*, int, int, float*,
@@ -270,6 +312,29 @@ int main(int argc, char** argv)
                                                       nullptr, nullptr));
             check("stack_destroy_13", mnad_stack_destroy(ctx, sid));
         }
+        {   // kernel 13 on a 241 x 241 grid (npml 20), BF16 ring, two shots in one
+            // group (the second grid of the GPU ring tests): the forward sequence
+            // and the sequence of adjoints and reconstructions replay chunks
+            const int n1 = 241, p1 = 20, nt1 = 122;
+            const mnad_grid g1{n1, n1, p1, 10.0, 10.0, 1e-3};
+            std::vector<double> v1((size_t)n1 * n1), w1(nt1, 0.0);
+            for (int i = 0; i < n1; ++i)
+                for (int j = 0; j < n1; ++j) v1[(size_t)j + (size_t)i * n1] = j < 120 ? 1500.0 : 2500.0;
+            w1[5] = 1.0;
+            std::vector<int> z1(2, p1), x1 = {p1 + 100, p1 + 137}, o1 = {0, 201, 402}, zr1, xr1;
+            for (int k = 0; k < 2; ++k)
+                for (int r = 0; r < 201; ++r) {
+                    zr1.push_back(p1 + 1);
+                    xr1.push_back(p1 + r);
+                }
+            const mnad_shots sh1{2, z1.data(), x1.data(), o1.data(), zr1.data(), xr1.data()};
+            int sid = -1;
+            check("stack_create_13_c1", mnad_stack_create(ctx, &g1, &sid));
+            check("rtm_shots_ring_13_c1", mnad_rtm_shots(ctx, sid, &g1, v1.data(), v1.data(), nt1,
+                                                         w1.data(), &sh1, MNAD_SNAP_RING_BF16, 2,
+                                                         0, nullptr, nullptr));
+            check("stack_destroy_13_c1", mnad_stack_destroy(ctx, sid));
+        }
         check("set_tuning_4", mnad_set_tuning(ctx, &t));
         check("fwd_shots", mnad_fwd_shots(ctx, &g, v.data(), nt, w.data(), &sh, 2, 0, dobs.data()));
     }
@@ -313,6 +378,13 @@ int main(int argc, char** argv)
         check("serve_stop", mnad_serve_stop(ctx));
     }
     mnad_ctx_destroy(ctx);
+    if (g_replays_checked == 0 || g_replay_mismatches > 0) {
+        printf("FAIL graph_chunk_replays: %d of %d kernel-13 replays do not match their sequence\n",
+               g_replay_mismatches, g_replays_checked);
+        ++g_bad;
+    } else {
+        printf("ok graph_chunk_replays (%d)\n", g_replays_checked);
+    }
     printf("%d failure(s)\n", g_bad);
     return g_bad;
 }
