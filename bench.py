@@ -42,16 +42,29 @@ HBM_PEAK_GBS = 8000.0
 # figure of the committed counter runs when the same kernel runs the same
 # workload, and null otherwise.
 MEASURED_TRAFFIC = {
-    # C2 RTM shot, nt = 300: forward (2 shots) 69.68 MB, adjoint 35.58 MB,
-    # equal launch counts -> 52.63 MB per launch (profiles/k_stepw_c2_rtm_shot.md,
-    # where the kernel's symbol is k_stepw<1, false>; the same code)
-    ("c2", "mnad::k_stepw<1>"): 52.63e6,
+    # C2 RTM shot with f16 snapshots, nt = 300: forward (2 shots) 69.68 MB,
+    # adjoint 35.58 MB, equal launch counts -> 52.63 MB per launch
+    # (profiles/k_stepw_c2_rtm_shot.md, where the kernel's symbol is
+    # k_stepw<1, false>; the same code)
+    ("c2", "mnad::k_stepw<1>", "f16"): 52.63e6,
     # C5, 20 shots, BF16 ring, groups of 10, pipelined: 2.0 x 6,480,359 +
     # 5,777,001 KiB per launch (profiles/k_stepx_c5_ring_bf16_batched.md)
     ("c5", "mnad::k_stepx<4>", "ring_bf16", 20, 10, True): 19.187e9,
     # the same with step kernel 13 (16-row segments, 4 waves): 30,835,394 KiB per
     # launch with its passes (profiles/k_step2q_c5_ring_bf16_batched.md)
     ("c5", "mnad::k_step2q<4, true>", "ring_bf16", 20, 10, True): 31.575e9,
+}
+
+
+SNAP_NOTES = {
+    "ring_bf16": ("BF16 boundary ring (the configuration's storage): the source wavefield is "
+                  "rebuilt by a reverse reconstruction beside the adjoint"),
+    "ring_f16": ("FP16 boundary ring: the source wavefield is rebuilt by a reverse "
+                 "reconstruction beside the adjoint"),
+    "f16": "f16 d2u snapshots with a per-shot power-of-two scale",
+    "bf16": ("bf16 d2u snapshots: the same bytes as f16, but the normalized C2 image misses "
+             "the 1e-3 bound (2.2e-3)"),
+    "f32": "f32 d2u snapshots",
 }
 
 
@@ -660,7 +673,8 @@ def main():
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--snap", default="",
                     choices=["", "f32", "f16", "bf16", "ring_f16", "ring_bf16"],
-                    help="source-wavefield storage (default: ring_bf16 for C5, f16 otherwise)")
+                    help="source-wavefield storage (default: the configuration's boundary "
+                         "storage, ring_bf16 for C2 and C5, ring_f16 for C3)")
     ap.add_argument("--no-compare-snap", dest="compare_snap", action="store_false",
                     help="C5: skip the f16-snapshot comparison")
     ap.add_argument("--no-c2", action="store_true", help="C5: skip the C2 single-shot extra keys")
@@ -745,8 +759,8 @@ def main():
     if args.workload == "c4":
         run_c4(args, mnad)
         return
-    if not args.snap:
-        args.snap = "ring_bf16" if args.workload == "c5" else "f16"
+    if not args.snap:   # the boundary storage each configuration names
+        args.snap = "ring_f16" if args.workload == "c3" else "ring_bf16"
     if args.cpu_steps == 0:
         args.cpu_steps = 300 if args.workload == "c5" else 400
     if args.workload == "c5":
@@ -845,13 +859,11 @@ def main():
                                 "20 Hz, 1001 receivers; modelling + 1-shot RTM per step"),
                    "grid_padded": [int(v.shape[0]), int(v.shape[1])], "nt": nt,
                    "snapshots": args.snap, "updates_per_step": 3 * N * nt,
-                   "snapshots_note": ("f16 with a per-shot power-of-two scale; bf16 (--snap "
-                                      "bf16) moves the same bytes, but its normalized C2 "
-                                      "image misses the 1e-3 bound (2.2e-3)")},
+                   "snapshots_note": SNAP_NOTES.get(args.snap, "")},
         "shots_per_hour": round(3600.0 / (ms_per_step / 1e3), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((args.workload, kname)),
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, kname, args.snap)),
                      "traffic_unit": "bytes per launch (rocprofv3 counter runs)",
                      "algorithmic_bytes_per_launch": round(k_bytes / max(k_launches, 1)),
                      "kernel": kname, "avg_launch_us": round(avg_launch_us, 3),
