@@ -176,7 +176,12 @@ def run_multi(args, mnad, wl, label):
     plain_flags = mnad.FLAG_SERIAL | mnad.FLAG_NO_GRAPH
     with mnad.Context(0) as ctx:
         t0 = time.perf_counter()
-        tuning, per_launch, us = ctx.autotune(g, v, max_shots=32)
+        if args.tuning:   # a fixed kernel variant and --batch shots per group (no autotune)
+            k, rz, br = (int(x) for x in args.tuning.split(","))
+            ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
+            tuning, per_launch, us = ctx.tuning(), args.batch, 0.0
+        else:
+            tuning, per_launch, us = ctx.autotune(g, v, max_shots=32)
         tune_s = time.perf_counter() - t0
         print(f"# autotune {tune_s:.2f} s: {tuning} shots/launch {per_launch} ({us:.2f} us/shot-step)",
               file=sys.stderr, flush=True)
@@ -697,8 +702,9 @@ def main():
     ap.add_argument("--compare-calls", type=int, default=1, help="--compare: timed calls")
     ap.add_argument("--compare-warm", type=int, default=1, help="--compare: untimed calls")
     ap.add_argument("--tuning", default="",
-                    help="C2 / C5 window: step kernel K,RZ,BR for the timed shots "
-                         "(default: the library's)")
+                    help="step kernel K,RZ,BR for the timed shots, without the autotuner "
+                         "(batched forms: --batch shots per group; default: autotuned, or the "
+                         "library's kernel for the single shot)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (x3) to calibrate the "
                          "FETCH_SIZE / WRITE_SIZE counters")
