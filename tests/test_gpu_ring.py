@@ -273,3 +273,48 @@ def test_ring_needs_two_cpml_cells(mnad, ctx):
     with pytest.raises(mnad.MnadError) as ei:
         ctx.fwd(g, v, f["w"], 2, 5, [2, 2], [4, 5], ss=1, indx=1, snap_fmt=mnad.SNAP_RING_BF16)
     assert ei.value.code == mnad.ERR_ARG
+
+
+def test_kernel13_ring_sequences_replay_their_own_graph_chunks(mnad):
+    """A kernel-13 ring RTM runs a forward sequence and then a sequence of
+    adjoints and reconstructions whose padded tile tables can have the same
+    length (here: a 48 x 64 grid, npml 8, three shots in one group, 16-row
+    segments of four waves).  Each sequence must replay a graph chunk captured
+    with its own passes (the reconstructions' ring decode and hand-off slot):
+    with graph chunks of 8 launches the image equals the one without graphs
+    bit for bit, and kernel 4's to 1e-4."""
+    nz, nx, npml, nt = 48, 64, 8, 120
+    _say(f"kernel-13 ring graph chunks {nz}x{nx}")
+    j = np.arange(nz)[:, None] * np.ones((1, nx))
+    v = np.asfortranarray(np.where(j < nz // 2, 1800.0, 2400.0))
+    vs = np.asfortranarray(1800.0 + 600.0 * j / nz)
+    w = np.zeros(nt)
+    w[5] = 1.0
+    g = mnad.Grid(nz, nx, npml, 10.0, 10.0, 1e-3)
+    nr = nx - 2 * npml
+    zs, xs = [npml] * 3, [npml + 2, nx // 2, nx - npml - 3]
+    recs = []
+    for k in range(3):
+        xr = np.arange(npml, npml + nr, 1 + k)
+        recs.append((np.full(xr.size, npml + 1), xr))
+    images = {}
+    for label, tune, flags in (("k13 graphs", (13, 2, 4), 0),
+                               ("k13 no graph", (13, 2, 4), mnad.FLAG_NO_GRAPH),
+                               ("k4 graphs", (4, 1, 1), 0)):
+        ctx = mnad.Context(0)
+        try:
+            ctx.set_tuning(kernel=tune[0], rows_per_thread=tune[1], block_rows=tune[2],
+                           graph_chunk=8)
+            sid = ctx.stack_create(g)
+            ctx.rtm_shots(sid, g, v, vs, w, zs, xs, recs, snap_fmt=mnad.SNAP_RING_BF16, batch=3,
+                          flags=flags)
+            img, ill, _ = ctx.stack_image(sid, g)
+            ctx.stack_destroy(sid)
+        finally:
+            ctx.close()
+        images[label] = np.where(ill > 0, img * ill, 0.0)
+        assert np.isfinite(images[label]).all()
+    e = rel_l2(images["k13 graphs"], images["k4 graphs"])
+    _say(f"  kernel 13 vs kernel 4: {e:.2e}")
+    assert np.array_equal(images["k13 graphs"], images["k13 no graph"])
+    assert e < 1e-4
