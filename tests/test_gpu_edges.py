@@ -69,6 +69,19 @@ def test_no_cpml(mnad, oracle, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
+def test_unequal_spacing_with_interior_tiles(mnad, oracle, kernel):
+    """dx != dz on a grid large enough for every kernel's interior tiles (the
+    fixtures with dx != dz are too small for them): kernel 13's interior waves
+    then take the stencil weights from the shot instead of the dx == dz
+    literals."""
+    _say(f"edge: dx 12.5 m, dz 9 m, kernel {kernel}")
+    v = _model(120, 400, 7)
+    w = _wavelet(200, 1e-3)
+    xr = np.arange(10, 390, 2)
+    _compare(mnad, oracle, kernel, v, 10, w, 12, 200, np.full(xr.size, 11), xr, dx=12.5, dz=9.0)
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
 @pytest.mark.parametrize("nt", [1, 2, 3])
 def test_shortest_runs(mnad, oracle, kernel, nt):
     _say(f"edge: nt {nt}, kernel {kernel}")
