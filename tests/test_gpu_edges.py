@@ -150,3 +150,57 @@ def test_two_step_kernels_odd_step_count_rtm(mnad, kernel, rz, br):
     assert rel_l2(gat[0][0], f["data0"]) < TOL
     assert rel_l2(gat[1][0], f["datas0"]) < TOL
     assert rel_l2(img, f["img"]) < 1e-3
+
+
+_BOREHOLE = {}
+
+
+def _borehole_case(oracle):
+    """120 x 400 grid (npml 10), source at the surface, receivers on the
+    surface and down a borehole through the interior; the FP64 oracle's
+    observed and synthetic gathers and its image (computed once)."""
+    if not _BOREHOLE:
+        nz, nx, npml, nt, dt = 120, 400, 10, 300, 1e-3
+        v = _model(nz, nx, 11)
+        vs = np.clip(v - 40.0, 1500.0, 3000.0)
+        w = _wavelet(nt, dt)
+        xr = np.concatenate([np.arange(npml, nx - npml, 4), np.full(nz - 2 * npml - 4, 300)])
+        zr = np.concatenate([np.full(xr.size - (nz - 2 * npml - 4), npml + 1),
+                             np.arange(npml + 2, nz - npml - 2)])
+        go = oracle.grid(nz, nx, npml, 10.0, 10.0, dt)
+        obs = oracle.forward(go, v, w, npml + 2, 200, zr, xr, want_state=False)["gather"]
+        syn = oracle.forward(go, vs, w, npml + 2, 200, zr, xr, snap=True, want_state=False)
+        im = oracle.adjoint(go, vs, zr, xr, obs - syn["gather"], syn["UU"])
+        _BOREHOLE.update(nz=nz, nx=nx, npml=npml, dt=dt, v=v, vs=vs, w=w, zr=zr, xr=xr,
+                         obs=obs, syn=syn["gather"],
+                         im=im.reshape(nz - 2 * npml, nx - 2 * npml))
+    return _BOREHOLE
+
+
+@pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
+def test_borehole_receivers_rtm(mnad, oracle, kernel):
+    """RTM with receivers down a borehole through the interior (the adjoint
+    injects inside interior tiles at every depth, where kernel 13's interior
+    waves take the march with receiver rows): gathers at 1e-4 and the image
+    at 1e-3 against the FP64 oracle, with f32 snapshots and the BF16 ring."""
+    _say(f"edge: borehole receivers RTM, kernel {kernel}")
+    c = _borehole_case(oracle)
+    g = mnad.Grid(c["nz"], c["nx"], c["npml"], 10.0, 10.0, c["dt"])
+    nzi, nxi = c["nz"] - 2 * c["npml"], c["nx"] - 2 * c["npml"]
+    for fmt in (mnad.SNAP_F32, mnad.SNAP_RING_BF16):
+        ctx = mnad.Context(0)
+        try:
+            k, rz, br = kernel
+            ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
+            obs = ctx.fwd(g, c["v"], c["w"], c["npml"] + 2, 200, c["zr"], c["xr"],
+                          want_state=False)["gather"]
+            syn = ctx.fwd(g, c["vs"], c["w"], c["npml"] + 2, 200, c["zr"], c["xr"], ss=2,
+                          indx=1, snap_fmt=fmt, want_state=False)["gather"]
+            im = ctx.born_adj(g, c["vs"], obs - syn, c["zr"], c["xr"], ss=2)
+        finally:
+            ctx.close()
+        assert rel_l2(obs, c["obs"]) < TOL
+        assert rel_l2(syn, c["syn"]) < TOL
+        e = rel_l2(im.reshape(nxi, nzi).T, c["im"])
+        _say(f"  format {fmt}: image vs FP64 {e:.2e}")
+        assert e < 1e-3
