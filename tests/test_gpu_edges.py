@@ -155,36 +155,42 @@ def test_two_step_kernels_odd_step_count_rtm(mnad, kernel, rz, br):
 _BOREHOLE = {}
 
 
-def _borehole_case(oracle):
-    """120 x 400 grid (npml 10), source at the surface, receivers on the
-    surface and down a borehole through the interior; the FP64 oracle's
-    observed and synthetic gathers and its image (computed once)."""
-    if not _BOREHOLE:
+def _borehole_case(oracle, zsrc):
+    """120 x 400 grid (npml 10), source at row zsrc of column 200, receivers on
+    the surface and down a borehole 300 m away (column 230) through the
+    interior, so that the direct and reflected waves reach receivers within
+    the 300 steps from every source depth; the FP64 oracle's observed and
+    synthetic gathers and its image (computed once per zsrc)."""
+    if zsrc not in _BOREHOLE:
         nz, nx, npml, nt, dt = 120, 400, 10, 300, 1e-3
         v = _model(nz, nx, 11)
         vs = np.clip(v - 40.0, 1500.0, 3000.0)
         w = _wavelet(nt, dt)
-        xr = np.concatenate([np.arange(npml, nx - npml, 4), np.full(nz - 2 * npml - 4, 300)])
+        xr = np.concatenate([np.arange(npml, nx - npml, 4), np.full(nz - 2 * npml - 4, 230)])
         zr = np.concatenate([np.full(xr.size - (nz - 2 * npml - 4), npml + 1),
                              np.arange(npml + 2, nz - npml - 2)])
         go = oracle.grid(nz, nx, npml, 10.0, 10.0, dt)
-        obs = oracle.forward(go, v, w, npml + 2, 200, zr, xr, want_state=False)["gather"]
-        syn = oracle.forward(go, vs, w, npml + 2, 200, zr, xr, snap=True, want_state=False)
+        obs = oracle.forward(go, v, w, zsrc, 200, zr, xr, want_state=False)["gather"]
+        syn = oracle.forward(go, vs, w, zsrc, 200, zr, xr, snap=True, want_state=False)
         im = oracle.adjoint(go, vs, zr, xr, obs - syn["gather"], syn["UU"])
-        _BOREHOLE.update(nz=nz, nx=nx, npml=npml, dt=dt, v=v, vs=vs, w=w, zr=zr, xr=xr,
-                         obs=obs, syn=syn["gather"],
-                         im=im.reshape(nz - 2 * npml, nx - 2 * npml))
-    return _BOREHOLE
+        _BOREHOLE[zsrc] = dict(nz=nz, nx=nx, npml=npml, dt=dt, v=v, vs=vs, w=w, zr=zr, xr=xr,
+                               obs=obs, syn=syn["gather"],
+                               im=im.reshape(nz - 2 * npml, nx - 2 * npml))
+    return _BOREHOLE[zsrc]
 
 
 @pytest.mark.parametrize("kernel", KERNELS, ids=IDS)
-def test_borehole_receivers_rtm(mnad, oracle, kernel):
+@pytest.mark.parametrize("zsrc", [12, 64, 90])
+def test_borehole_receivers_rtm(mnad, oracle, kernel, zsrc):
     """RTM with receivers down a borehole through the interior (the adjoint
     injects inside interior tiles at every depth, where kernel 13's interior
-    waves take the march with receiver rows): gathers at 1e-4 and the image
-    at 1e-3 against the FP64 oracle, with f32 snapshots and the BF16 ring."""
-    _say(f"edge: borehole receivers RTM, kernel {kernel}")
-    c = _borehole_case(oracle)
+    waves take the march with receiver rows), the source near the surface or
+    deep in the interior (kernel 13 then injects it on the interior path, and
+    the reconstruction takes it back out there) or 20 rows above the bottom
+    CPML: gathers at 1e-4 and the image at 1e-3 against the FP64 oracle, with
+    f32 snapshots and the BF16 ring."""
+    _say(f"edge: borehole receivers RTM, source row {zsrc}, kernel {kernel}")
+    c = _borehole_case(oracle, zsrc)
     g = mnad.Grid(c["nz"], c["nx"], c["npml"], 10.0, 10.0, c["dt"])
     nzi, nxi = c["nz"] - 2 * c["npml"], c["nx"] - 2 * c["npml"]
     for fmt in (mnad.SNAP_F32, mnad.SNAP_RING_BF16):
@@ -192,9 +198,9 @@ def test_borehole_receivers_rtm(mnad, oracle, kernel):
         try:
             k, rz, br = kernel
             ctx.set_tuning(kernel=k, rows_per_thread=rz, block_rows=br)
-            obs = ctx.fwd(g, c["v"], c["w"], c["npml"] + 2, 200, c["zr"], c["xr"],
+            obs = ctx.fwd(g, c["v"], c["w"], zsrc, 200, c["zr"], c["xr"],
                           want_state=False)["gather"]
-            syn = ctx.fwd(g, c["vs"], c["w"], c["npml"] + 2, 200, c["zr"], c["xr"], ss=2,
+            syn = ctx.fwd(g, c["vs"], c["w"], zsrc, 200, c["zr"], c["xr"], ss=2,
                           indx=1, snap_fmt=fmt, want_state=False)["gather"]
             im = ctx.born_adj(g, c["vs"], obs - syn, c["zr"], c["xr"], ss=2)
         finally:
