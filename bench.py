@@ -673,8 +673,11 @@ def main():
     ap.add_argument("--plain-shots", type=int, default=2, help="many-shot runs: shots timed on the plain path")
     ap.add_argument("--shots-c2", type=int, default=1,
                     help="C2 grid with this many shots, batched vs plain (default: the 1-shot bench)")
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (C5: shots of the batched job; default 20 for C5, "
+                         "whose groups of 10 then pipeline, else 5)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed warm-up steps (default 5 for C5, else 1)")
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--snap", default="",
                     choices=["", "f32", "f16", "bf16", "ring_f16", "ring_bf16"],
@@ -752,6 +755,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.steps is None:
+        args.steps = 20 if args.workload == "c5" else 5
+    if args.warmup is None:
+        args.warmup = 5 if args.workload == "c5" else 1
     from __graft_entry__ import load_package
     mnad = load_package()
     if args.probe:
