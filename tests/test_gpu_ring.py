@@ -318,3 +318,51 @@ def test_kernel13_ring_sequences_replay_their_own_graph_chunks(mnad):
     _say(f"  kernel 13 vs kernel 4: {e:.2e}")
     assert np.array_equal(images["k13 graphs"], images["k13 no graph"])
     assert e < 1e-4
+
+
+_MIN_CPML = {}
+
+
+@pytest.mark.parametrize("npml", [2, 3])
+@pytest.mark.parametrize("kernel,rz,br", [(4, 1, 1), (6, 1, 2), (10, 1, 1), (11, 2, 1), (13, 1, 1),
+                                          (13, 2, 4)])
+def test_ring_at_the_narrowest_cpml(mnad, oracle, npml, kernel, rz, br):
+    """The ring takes the two CPML cells next to the interior, so with npml = 2
+    it covers the whole CPML and its outer cells are the grid's edge (and with
+    npml = 3 one CPML cell lies beyond it): a BF16-ring RTM with receivers on
+    the surface and down a borehole near the source images within 1e-3 of the
+    FP64 oracle on every kernel family that codes the ring."""
+    _say(f"ring at npml {npml}, kernel {kernel},{rz},{br}")
+    nz, nx, nt, dt = 100, 300, 300, 1e-3
+    if npml not in _MIN_CPML:
+        rng = np.random.default_rng(17)
+        z = np.arange(nz)[:, None]
+        v = np.clip(1800.0 + 700.0 * (z > 55) + 50.0 * rng.standard_normal((nz, nx)), 1500.0, 3000.0)
+        vs = np.clip(v - 40.0, 1500.0, 3000.0)
+        t = np.arange(nt) * dt
+        tau = np.pi * (t - 1.1 / 25.0) * 25.0
+        w = (1 - 2 * tau**2) * np.exp(-tau**2)
+        xr = np.concatenate([np.arange(npml, nx - npml, 3), np.full(nz - 2 * npml - 2, 180)])
+        zr = np.concatenate([np.full(xr.size - (nz - 2 * npml - 2), npml),
+                             np.arange(npml + 1, nz - npml - 1)])
+        go = oracle.grid(nz, nx, npml, 10.0, 10.0, dt)
+        obs = oracle.forward(go, v, w, npml + 1, 150, zr, xr, want_state=False)["gather"]
+        syn = oracle.forward(go, vs, w, npml + 1, 150, zr, xr, snap=True, want_state=False)
+        im = oracle.adjoint(go, vs, zr, xr, obs - syn["gather"], syn["UU"])
+        _MIN_CPML[npml] = dict(v=v, vs=vs, w=w, zr=zr, xr=xr, obs=obs,
+                               im=im.reshape(nz - 2 * npml, nx - 2 * npml))
+    c = _MIN_CPML[npml]
+    g = mnad.Grid(nz, nx, npml, 10.0, 10.0, dt)
+    ctx = mnad.Context(0)
+    try:
+        ctx.set_tuning(kernel=kernel, rows_per_thread=rz, block_rows=br)
+        obs = ctx.fwd(g, c["v"], c["w"], npml + 1, 150, c["zr"], c["xr"], want_state=False)["gather"]
+        syn = ctx.fwd(g, c["vs"], c["w"], npml + 1, 150, c["zr"], c["xr"], ss=4, indx=1,
+                      snap_fmt=mnad.SNAP_RING_BF16, want_state=False)["gather"]
+        im = ctx.born_adj(g, c["vs"], obs - syn, c["zr"], c["xr"], ss=4)
+    finally:
+        ctx.close()
+    assert rel_l2(obs, c["obs"]) < GATHER_TOL
+    e = rel_l2(im.reshape(nx - 2 * npml, nz - 2 * npml).T, c["im"])
+    _say(f"  image vs FP64: {e:.2e}")
+    assert e < IMAGE_TOL
